@@ -1,0 +1,308 @@
+#!/usr/bin/env python3
+"""Generate class-specialised McMurchie-Davidson ERI device code for gfx950.
+
+For each angular-momentum class (la lb | lc ld) with la >= lb, lc >= ld and
+(la,lb) >= (lc,ld) this emits
+
+    template <typename T> __device__ void eri_<la><lb><lc><ld>(const EriArgs&, ..., T* V)
+
+which evaluates one contracted shell quartet: it loops over the bra and ket
+primitive pairs, evaluates the reference's Taylor-table Boys function (or the
+accurate one), builds the Hermite integrals R_tuv by the recursion of
+Integrals/IntegralsEvaluation.hs:430-449, contracts them with the ket and bra
+Hermite expansion coefficients (IntegralsEvaluation.hs:452-477, same
+mathematics as the reference's E.R.E product), applies the reference's
+per-primitive, per-Cartesian-component screen (IntegralsEvaluation.hs:305-315)
+and accumulates the contracted integrals V[(a*nb+b)*(nc*nd) + c*nd + d].
+
+The generated code is fully unrolled straight-line arithmetic with every
+index resolved at generation time, so the compiler keeps it in registers.
+
+Usage: python gen_eri.py LMAX OUTDIR
+"""
+import sys
+
+
+def comps(l):
+    """Cartesian components of angular momentum l in canonical order
+    (x^l first, then decreasing x power, y before z): s; x,y,z;
+    xx,xy,xz,yy,yz,zz."""
+    out = []
+    for i in range(l, -1, -1):
+        for j in range(l - i, -1, -1):
+            out.append((i, j, l - i - j))
+    return out
+
+
+class Emitter:
+    def __init__(self):
+        self.lines = []
+        self.n = 0
+
+    def tmp(self, expr, prefix="t"):
+        name = "%s%d" % (prefix, self.n)
+        self.n += 1
+        self.lines.append("    const T %s = %s;" % (name, expr))
+        return name
+
+    def raw(self, s):
+        self.lines.append(s)
+
+
+def hermite_E(em, la, lb, PA, PB, oo2p, tag):
+    """E[i][j][t] for i<=la, j<=lb (per axis); returns dict -> expr names.
+
+    Helgaker's recursion (equivalent to recursiveHermCoeff with seed 1):
+      E^{i+1,j}_t = oo2p*E^{ij}_{t-1} + PA*E^{ij}_t + (t+1)*E^{ij}_{t+1}
+      E^{i,j+1}_t = oo2p*E^{ij}_{t-1} + PB*E^{ij}_t + (t+1)*E^{ij}_{t+1}
+    """
+    E = {(0, 0, 0): "T(1)"}
+
+    def get(i, j, t):
+        if t < 0 or t > i + j:
+            return None
+        return E[(i, j, t)]
+
+    def build(i, j, src, X):
+        # src = (i', j') with one fewer; X = PA or PB
+        si, sj = src
+        for t in range(0, i + j + 1):
+            terms = []
+            a = get(si, sj, t - 1)
+            if a is not None:
+                terms.append("%s*%s" % (oo2p, a) if a != "T(1)" else oo2p)
+            b = get(si, sj, t)
+            if b is not None:
+                terms.append("%s*%s" % (X, b) if b != "T(1)" else X)
+            c = get(si, sj, t + 1)
+            if c is not None:
+                terms.append("T(%d)*%s" % (t + 1, c) if t + 1 != 1 else c)
+            expr = " + ".join(terms) if terms else "T(0)"
+            E[(i, j, t)] = em.tmp(expr, "e%s" % tag)
+
+    for i in range(1, la + 1):
+        build(i, 0, (i - 1, 0), PA)
+    for i in range(0, la + 1):
+        for j in range(1, lb + 1):
+            build(i, j, (i, j - 1), PB)
+    return E
+
+
+def gen_class(la, lb, lc, ld):
+    L = la + lb + lc + ld
+    ca, cb, cc, cd = comps(la), comps(lb), comps(lc), comps(ld)
+    na, nb, nc, nd = len(ca), len(cb), len(cc), len(cd)
+    nbra, nket = na * nb, nc * nd
+    name = "eri_%d%d%d%d" % (la, lb, lc, ld)
+    out = []
+    out.append("// (%s%s|%s%s): %d integrals, L=%d" % ("spdf"[la], "spdf"[lb], "spdf"[lc], "spdf"[ld], nbra * nket, L))
+    out.append("template <typename T, int BOYS>")
+    out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp," % name)
+    out.append("    const double* __restrict__ qcomp, const BoysTables& bt, T* __restrict__ V, double qtau)")
+    out.append("{")
+    out.append("#pragma unroll")
+    out.append("  for (int z = 0; z < %d; ++z) V[z] = T(0);" % (nbra * nket))
+    out.append("  for (int ib = 0; ib < nbp; ++ib) {")
+    out.append("    const double* pb = bra + (size_t)ib * PP_STRIDE;")
+    em = Emitter()
+    em.raw("    const T bp = T(pb[PP_P]);")
+    em.raw("    const T bPx = T(pb[PP_PX]), bPy = T(pb[PP_PY]), bPz = T(pb[PP_PZ]);")
+    em.raw("    const T bAx = T(pb[PP_AX]), bAy = T(pb[PP_AY]), bAz = T(pb[PP_AZ]);")
+    em.raw("    const T bBx = T(pb[PP_BX]), bBy = T(pb[PP_BY]), bBz = T(pb[PP_BZ]);")
+    em.raw("    const T bK = T(pb[PP_K]);")
+    em.raw("    const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
+    em.raw("    const int bqoff = (int)pb[PP_QOFF];")
+    em.raw("    const T boo2p = T(pb[PP_OO2P]);")
+    Eb = [hermite_E(em, la, lb, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
+    out.extend(em.lines)
+    out.append("    for (int ik = 0; ik < nkp; ++ik) {")
+    out.append("      const double* pk = ket + (size_t)ik * PP_STRIDE;")
+    out.append("      const double kqlo = pk[PP_QLO], kqhi = pk[PP_QHI];")
+    out.append("      // reference screen: keep iff !(q_ab*q_cd < 1e-8), per component")
+    out.append("      if (qtau > 0.0 && bqhi * kqhi < qtau) continue;   // every component screened")
+    out.append("      const bool allin = !(qtau > 0.0) || (bqlo * kqlo >= qtau);")
+    em2 = Emitter()
+    em2.n = em.n
+    em2.raw("      const T kq = T(pk[PP_P]);")
+    em2.raw("      const T kQx = T(pk[PP_PX]), kQy = T(pk[PP_PY]), kQz = T(pk[PP_PZ]);")
+    em2.raw("      const T kCx = T(pk[PP_AX]), kCy = T(pk[PP_AY]), kCz = T(pk[PP_AZ]);")
+    em2.raw("      const T kDx = T(pk[PP_BX]), kDy = T(pk[PP_BY]), kDz = T(pk[PP_BZ]);")
+    em2.raw("      const T kK = T(pk[PP_K]);")
+    em2.raw("      const T koo2q = T(pk[PP_OO2P]);")
+    em2.raw("      const T pq = bp + kq;")
+    em2.raw("      const T alpha = bp * kq / pq;")
+    em2.raw("      const T Xpq = bPx - kQx, Ypq = bPy - kQy, Zpq = bPz - kQz;")
+    em2.raw("      const T Targ = alpha * ((Xpq * Xpq + Ypq * Ypq) + Zpq * Zpq);")
+    em2.raw("      T F[%d];" % (L + 1))
+    em2.raw("      boys_eval<T, %d, BOYS>(Targ, bt, F);" % L)
+    em2.raw("      const T pref = T(HFK_TWO_PI_2_5) * bK * kK / sqrt_t(pq);")
+    # base R^n_000 = pref * (-2 alpha)^n * F_n
+    em2.raw("      const T m2a = T(-2) * alpha;")
+    R = {}
+    pw = "pref"
+    for n in range(0, L + 1):
+        if n == 0:
+            R[(0, 0, 0, 0)] = em2.tmp("pref * F[0]", "r")
+        else:
+            pw = em2.tmp("%s * m2a" % pw, "w")
+            R[(n, 0, 0, 0)] = em2.tmp("%s * F[%d]" % (pw, n), "r")
+
+    def getR(n, t, u, v):
+        if t < 0 or u < 0 or v < 0:
+            return None
+        key = (n, t, u, v)
+        if key in R:
+            return R[key]
+        if t >= 1:
+            a = getR(n + 1, t - 2, u, v)
+            b = getR(n + 1, t - 1, u, v)
+            terms = []
+            if a is not None and t - 1 != 0:
+                terms.append("T(%d)*%s" % (t - 1, a))
+            terms.append("Xpq*%s" % b)
+        elif u >= 1:
+            a = getR(n + 1, t, u - 2, v)
+            b = getR(n + 1, t, u - 1, v)
+            terms = []
+            if a is not None and u - 1 != 0:
+                terms.append("T(%d)*%s" % (u - 1, a))
+            terms.append("Ypq*%s" % b)
+        else:
+            a = getR(n + 1, t, u, v - 2)
+            b = getR(n + 1, t, u, v - 1)
+            terms = []
+            if a is not None and v - 1 != 0:
+                terms.append("T(%d)*%s" % (v - 1, a))
+            terms.append("Zpq*%s" % b)
+        R[key] = em2.tmp(" + ".join(terms), "r")
+        return R[key]
+
+    # ket E (with (-1)^tau folded in later), per axis
+    Ek = [hermite_E(em2, lc, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
+    lab = la + lb
+    # hermite tuv list for the bra side
+    btuv = [(t, u, v) for t in range(lab + 1) for u in range(lab + 1 - t) for v in range(lab + 1 - t - u)]
+    # V accumulation per ket component pair
+    for ic, c in enumerate(cc):
+        for id_, d in enumerate(cd):
+            ket_idx = ic * nd + id_
+            # W^{cd}_{tuv} for tuv in btuv
+            W = {}
+            for (t, u, v) in btuv:
+                terms_x = []
+                for tau in range(c[0] + d[0] + 1):
+                    ex = Ek[0][(c[0], d[0], tau)]
+                    terms_y = []
+                    for nu in range(c[1] + d[1] + 1):
+                        ey = Ek[1][(c[1], d[1], nu)]
+                        terms_z = []
+                        for phi in range(c[2] + d[2] + 1):
+                            ez = Ek[2][(c[2], d[2], phi)]
+                            r = getR(0, t + tau, u + nu, v + phi)
+                            sg = "-" if (phi % 2) else ""
+                            terms_z.append("%s%s" % (sg, r) if ez == "T(1)" else "%s%s*%s" % (sg, ez, r))
+                        sz = " + ".join(terms_z).replace("+ -", "- ")
+                        sgy = (nu % 2) == 1
+                        if ey == "T(1)":
+                            ty = "(%s)" % sz
+                        else:
+                            ty = "%s*(%s)" % (ey, sz)
+                        terms_y.append(("-" if sgy else "") + ty)
+                    sy = " + ".join(terms_y).replace("+ -", "- ")
+                    sgx = (tau % 2) == 1
+                    tx = "(%s)" % sy if ex == "T(1)" else "%s*(%s)" % (ex, sy)
+                    terms_x.append(("-" if sgx else "") + tx)
+                sx = " + ".join(terms_x).replace("+ -", "- ")
+                W[(t, u, v)] = em2.tmp(sx, "W")
+            for ia, a in enumerate(ca):
+                for ib_, b in enumerate(cb):
+                    bra_idx = ia * nb + ib_
+                    terms_x = []
+                    for t in range(a[0] + b[0] + 1):
+                        ex = Eb[0][(a[0], b[0], t)]
+                        terms_y = []
+                        for u in range(a[1] + b[1] + 1):
+                            ey = Eb[1][(a[1], b[1], u)]
+                            terms_z = []
+                            for v in range(a[2] + b[2] + 1):
+                                ez = Eb[2][(a[2], b[2], v)]
+                                w = W[(t, u, v)]
+                                terms_z.append(w if ez == "T(1)" else "%s*%s" % (ez, w))
+                            sz = " + ".join(terms_z)
+                            terms_y.append("(%s)" % sz if ey == "T(1)" else "%s*(%s)" % (ey, sz))
+                        sy = " + ".join(terms_y)
+                        terms_x.append("(%s)" % sy if ex == "T(1)" else "%s*(%s)" % (ex, sy))
+                    val = " + ".join(terms_x)
+                    vi = bra_idx * nket + ket_idx
+                    em2.raw("      { const T v_ = %s;" % val)
+                    em2.raw("        V[%d] += (allin || !(qcomp[bqoff + %d] * qcomp[(int)pk[PP_QOFF] + %d] < qtau)) ? v_ : T(0); }"
+                            % (vi, bra_idx, ket_idx))
+    out.extend(em2.lines)
+    out.append("    }")
+    out.append("  }")
+    out.append("}")
+    return "\n".join(out), (na, nb, nc, nd)
+
+
+def pair_classes(lmax):
+    return [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
+
+
+def quartet_classes(lmax):
+    pcs = pair_classes(lmax)
+    out = []
+    for i, (la, lb) in enumerate(pcs):
+        for (lc, ld) in pcs[: i + 1]:
+            out.append((la, lb, lc, ld))
+    return out
+
+
+def main():
+    """gen_eri.py LMAX OUTDIR: writes OUTDIR/eri_<class>.hip (one translation
+    unit per class, compiled in parallel) and OUTDIR/eri_dispatch.hip."""
+    lmax = int(sys.argv[1]) if len(sys.argv) > 1 else 1
+    outdir = sys.argv[2] if len(sys.argv) > 2 else "."
+    import os
+    os.makedirs(outdir, exist_ok=True)
+    qs = quartet_classes(lmax)
+    for q in qs:
+        tag = "%d%d%d%d" % q
+        src, _ = gen_class(*q)
+        body = []
+        body.append("// Generated by gen_eri.py -- do not edit.")
+        body.append("#include <hip/hip_runtime.h>")
+        body.append('#include "hfk_internal.hpp"')
+        body.append("namespace {")
+        body.append(src)
+        body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS>")
+        body.append("__device__ __forceinline__ void eri_class(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp,")
+        body.append("    const double* __restrict__ qcomp, const BoysTables& bt, T* __restrict__ V, double qtau)")
+        body.append("{ eri_%s<T, BOYS>(bra, nbp, ket, nkp, qcomp, bt, V, qtau); }" % tag)
+        body.append("}  // namespace")
+        body.append('#include "hfk_eri_impl.hpp"')
+        body.append("int hfk_dispatch_%s(int op, int prec, int boys, const QuartetArgs& a, hipStream_t s)" % tag)
+        body.append("{ return dispatch<%d, %d, %d, %d>(op, prec, boys, a, s); }" % q)
+        path = os.path.join(outdir, "eri_%s.hip" % tag)
+        txt = "\n".join(body) + "\n"
+        if not os.path.exists(path) or open(path).read() != txt:
+            with open(path, "w") as f:
+                f.write(txt)
+    d = ["// Generated by gen_eri.py -- do not edit.", "#include <hip/hip_runtime.h>", '#include "hfk_internal.hpp"']
+    for q in qs:
+        d.append("int hfk_dispatch_%d%d%d%d(int, int, int, const QuartetArgs&, hipStream_t);" % q)
+    d.append("int hfk_launch_quartets(int la, int lb, int lc, int ld, int op, int prec, int boys, const QuartetArgs& a, hipStream_t s)")
+    d.append("{")
+    for q in qs:
+        d.append("    if (la == %d && lb == %d && lc == %d && ld == %d) return hfk_dispatch_%d%d%d%d(op, prec, boys, a, s);" % (q + q))
+    d.append("    return -2;")
+    d.append("}")
+    path = os.path.join(outdir, "eri_dispatch.hip")
+    txt = "\n".join(d) + "\n"
+    if not os.path.exists(path) or open(path).read() != txt:
+        with open(path, "w") as f:
+            f.write(txt)
+    print(" ".join("eri_%d%d%d%d" % q for q in qs) + " eri_dispatch")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,112 @@
+// Device-side definitions shared by the hfk kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Primitive-pair record: one per (primitive of shell A, primitive of shell B)
+// of a shell pair, 16 doubles (128 B, one cache line).
+enum {
+    PP_P = 0,       // p = a + b
+    PP_PX, PP_PY, PP_PZ,   // Gaussian product centre P
+    PP_AX, PP_AY, PP_AZ,   // P - A
+    PP_BX, PP_BY, PP_BZ,   // P - B
+    PP_K,           // c_a c_b exp(-ab/p |A-B|^2) / p   (base coefficients)
+    PP_QLO,         // min over components of the reference screen value (NaN -> +inf)
+    PP_QHI,         // max over components                             (NaN -> +inf)
+    PP_QOFF,        // offset of this pair's per-component screen values in qcomp[]
+    PP_OO2P,        // 1 / (2p)
+    PP_PAD,
+    PP_STRIDE = 16
+};
+
+#define HFK_TWO_PI_2_5 34.986836655249725   // 2 pi^(5/2)
+#define HFK_PI 3.141592653589793
+
+// Reference Boys grid (NumericalTools/TableBoys.hs:58-63): m = 0..14, x = 0.1 i, i = 0..1000
+#define BOYS_REF_M 15
+#define BOYS_NX 1001
+// Accurate grid: m = 0..ACC_M-1, x = 0.1 i, i = 0..ACC_NX-1 (x <= 50); above, asymptotic + recursion
+#define BOYS_ACC_M 26
+#define BOYS_ACC_NX 501
+#define BOYS_ACC_XMAX 50.0
+
+struct BoysTables {
+    const double* ref64;   // [BOYS_REF_M][BOYS_NX]
+    const float* ref32;
+    const double* acc64;   // [BOYS_ACC_M][BOYS_ACC_NX]
+    const float* acc32;
+    double asym[24];       // a_m = facOdd(floor(2m-1)) / 2^(m+1), with the reference's Int64 wrap
+};
+
+template <typename T> __device__ __forceinline__ const T* boys_ref_tab(const BoysTables& bt);
+template <> __device__ __forceinline__ const double* boys_ref_tab<double>(const BoysTables& bt) { return bt.ref64; }
+template <> __device__ __forceinline__ const float* boys_ref_tab<float>(const BoysTables& bt) { return bt.ref32; }
+template <typename T> __device__ __forceinline__ const T* boys_acc_tab(const BoysTables& bt);
+template <> __device__ __forceinline__ const double* boys_acc_tab<double>(const BoysTables& bt) { return bt.acc64; }
+template <> __device__ __forceinline__ const float* boys_acc_tab<float>(const BoysTables& bt) { return bt.acc32; }
+
+__device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
+
+// F_m(x) for m = 0..L.
+//  BOYS == 0: the reference's 6-term Taylor step from the 0.1-spaced node
+//             table, asymptotic formula evaluated at the node above x_k = 100
+//             (TableBoys.hs:45-51,65-69; Boys.hs:155-157).
+//  BOYS == 1: accurate: 8-term Taylor about the nearest 0.1 node (|dx| <= 0.05)
+//             of an accurate table for x < 50; F_0 = sqrt(pi/x)/2 and upward
+//             recursion above.
+template <typename T, int L, int BOYS>
+__device__ __forceinline__ void boys_eval(T x, const BoysTables& bt, T* F)
+{
+    if constexpr (BOYS == 0) {
+        const T fl = floor(x / T(0.1));
+        const T xk = fl * T(0.1);
+        const T dx = x - xk;
+        T f[L + 6];
+        if (xk > T(100)) {
+            const T inv = T(1) / xk;
+            T s = sqrt_t(T(HFK_PI) * inv);
+#pragma unroll
+            for (int m = 0; m < L + 6; ++m) {
+                f[m] = T(bt.asym[m]) * s;
+                s *= inv;
+            }
+        } else {
+            int idx = (int)fl;
+            idx = idx < 0 ? 0 : (idx > BOYS_NX - 1 ? BOYS_NX - 1 : idx);   // NaN/negative guard
+            const T* tab = boys_ref_tab<T>(bt) + idx;
+#pragma unroll
+            for (int m = 0; m < L + 6; ++m) f[m] = tab[m * BOYS_NX];
+        }
+        const T d2 = dx * dx, d3 = d2 * dx, d4 = d2 * d2, d5 = d4 * dx;
+#pragma unroll
+        for (int m = 0; m <= L; ++m)
+            F[m] = f[m] - f[m + 1] * dx + (T(0.5) * f[m + 2]) * d2 - (T(1.0 / 6.0) * f[m + 3]) * d3
+                   + (T(1.0 / 24.0) * f[m + 4]) * d4 - (T(1.0 / 120.0) * f[m + 5]) * d5;
+    } else {
+        if (x < T(BOYS_ACC_XMAX)) {
+            const T fl = rint(x * T(10));
+            const T dx = x - fl * T(0.1);
+            int idx = (int)fl;
+            idx = idx < 0 ? 0 : (idx > BOYS_ACC_NX - 1 ? BOYS_ACC_NX - 1 : idx);
+            const T* tab = boys_acc_tab<T>(bt) + idx;
+            const T mdx = -dx;
+#pragma unroll
+            for (int m = 0; m <= L; ++m) {
+                T acc = tab[(m + 7) * BOYS_ACC_NX];
+#pragma unroll
+                for (int k = 7; k >= 1; --k) acc = tab[(m + k - 1) * BOYS_ACC_NX] + (mdx / T(k)) * acc;
+                F[m] = acc;
+            }
+        } else {
+            const T inv = T(1) / x;
+            const T e = (T)exp((double)-x);
+            F[0] = T(0.5) * sqrt_t(T(HFK_PI) * inv);
+#pragma unroll
+            for (int m = 0; m < L; ++m) F[m + 1] = (T(2 * m + 1) * F[m] - e) * (T(0.5) * inv);
+        }
+    }
+}
+
+// Atomic add of a double into global memory with the hardware FP64 atomic.
+__device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }

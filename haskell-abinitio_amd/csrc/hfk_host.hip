@@ -1,0 +1,648 @@
+// Host side of the C ABI (include/hfk.h): context construction (shells,
+// primitive pairs, Boys tables, Schwarz bounds) and the integral-direct
+// Fock build.  All arithmetic runs in HIP kernels; this file only plans
+// and launches.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hfk.h"
+#include "hfk_internal.hpp"
+#include "hfk_host.hpp"
+
+static thread_local std::string g_last_error;
+
+int hfk_set_error(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+extern "C" int hfk_last_error(char* buf, size_t len)
+{
+    if (!buf || len == 0) return HFK_EINVAL;
+    snprintf(buf, len, "%s", g_last_error.c_str());
+    return HFK_OK;
+}
+
+extern "C" void hfk_default_options(hfk_options* opt)
+{
+    opt->screen_mode = HFK_SCREEN_REFERENCE;
+    opt->boys_mode = HFK_BOYS_REFERENCE;
+    opt->screen_tau = 1e-8;     // IntegralsEvaluation.hs:309
+    opt->schwarz_cut = 0.0;
+    opt->fp32_threshold = 0.0;
+}
+
+namespace {
+
+int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+int cart_index(int l, int x, int y, int z)
+{
+    int k = 0;
+    for (int i = l; i >= 0; --i)
+        for (int j = l - i; j >= 0; --j) {
+            if (i == x && j == y && (l - i - j) == z) return k;
+            ++k;
+        }
+    return -1;
+}
+
+int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }   // la >= lb
+
+// Gauss-Legendre 256: positive nodes ascending, by Newton on P_256 in long
+// double (the values of NumericalTools/PointsWeights.hs:25-31).
+void gl256(double* x, double* w)
+{
+    const int N = 256;
+    for (int i = 0; i < N / 2; i++) {
+        long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (N + 0.5L));
+        long double pp = 0.0L;
+        for (int it = 0; it < 100; it++) {
+            long double p1 = 1.0L, p2 = 0.0L;
+            for (int j = 1; j <= N; j++) {
+                long double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+            }
+            pp = N * (z * p1 - p2) / (z * z - 1.0L);
+            long double z1 = z;
+            z = z1 - p1 / pp;
+            if (fabsl(z - z1) < 1e-19L) break;
+        }
+        long double p1 = 1.0L, p2 = 0.0L;
+        for (int j = 1; j <= N; j++) {
+            long double p3 = p2;
+            p2 = p1;
+            p1 = ((2.0L * j - 1.0L) * z * p2 - (j - 1.0L) * p3) / j;
+        }
+        pp = N * (z * p1 - p2) / (z * z - 1.0L);
+        x[N / 2 - 1 - i] = (double)z;
+        w[N / 2 - 1 - i] = (double)(2.0L / ((1.0L - z * z) * pp * pp));
+    }
+}
+
+// facOdd (Boys.hs:159-164) with Int (64-bit wrapping) factorials.
+double hs_facodd(int i)
+{
+    if (i < 2) return 1.0;
+    int k = (1 + i) / 2;
+    uint64_t f2k = 1, fk = 1;
+    for (int t = 1; t <= 2 * k; ++t) f2k *= (uint64_t)t;
+    for (int t = 1; t <= k; ++t) fk *= (uint64_t)t;
+    double p2 = 1.0;
+    for (int t = 0; t < k; ++t) p2 *= 2.0;
+    return (double)(int64_t)f2k / (p2 * (double)(int64_t)fk);
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------
+// context construction
+// ----------------------------------------------------------------------
+
+static int group_shells(hfk_ctx* c, const hfk_basis* b)
+{
+    const int n = b->nbf;
+    int i = 0;
+    while (i < n) {
+        const int* l = b->lxyz + 3 * i;
+        const int L = l[0] + l[1] + l[2];
+        if (L > HFK_LMAX || l[0] < 0 || l[1] < 0 || l[2] < 0)
+            return hfk_set_error(HFK_EUNSUP, "basis function %d has angular momentum %d (max %d)", i, L, HFK_LMAX);
+        HostShell s;
+        s.atom = b->atom_of_bf ? b->atom_of_bf[i] : -1;
+        for (int x = 0; x < 3; ++x) s.R[x] = b->center[3 * i + x];
+        s.L = L;
+        const int p0 = b->prim_off[i], np = b->prim_off[i + 1] - p0;
+        if (np <= 0) return hfk_set_error(HFK_EINVAL, "basis function %d has no primitives", i);
+        s.prim0 = (int)c->sh_expo.size();
+        s.nprim = np;
+        for (int k = 0; k < np; ++k) {
+            c->sh_expo.push_back(b->expo[p0 + k]);
+            c->sh_coef.push_back(b->coef[p0 + k]);
+        }
+        for (int k = 0; k < HFK_MAXCOMP; ++k) { s.bf[k] = -1; s.scale[k] = 0.0; }
+        int ci = cart_index(L, l[0], l[1], l[2]);
+        s.bf[ci] = i;
+        s.scale[ci] = 1.0;
+        int j = i + 1;
+        for (; j < n; ++j) {
+            const int* lj = b->lxyz + 3 * j;
+            if (lj[0] + lj[1] + lj[2] != L) break;
+            bool same = b->center[3 * j] == s.R[0] && b->center[3 * j + 1] == s.R[1] && b->center[3 * j + 2] == s.R[2];
+            if (!same) break;
+            const int q0 = b->prim_off[j], nq = b->prim_off[j + 1] - q0;
+            if (nq != np) break;
+            bool ok = true;
+            for (int k = 0; k < np && ok; ++k) ok = b->expo[q0 + k] == b->expo[p0 + k];
+            if (!ok) break;
+            const int cj = cart_index(L, lj[0], lj[1], lj[2]);
+            if (s.bf[cj] >= 0) break;
+            // coefficients must be proportional (component normalisation)
+            const double r = b->coef[q0] / b->coef[p0];
+            for (int k = 0; k < np && ok; ++k) {
+                const double rk = b->coef[q0 + k] / b->coef[p0 + k];
+                ok = fabs(rk - r) <= 1e-13 * fabs(r);
+            }
+            if (!ok) break;
+            s.bf[cj] = j;
+            s.scale[cj] = r;
+        }
+        c->shells.push_back(s);
+        i = j;
+    }
+    return HFK_OK;
+}
+
+int hfk_ctx_build_pairs(hfk_ctx* c);
+int hfk_ctx_schwarz(hfk_ctx* c);
+
+extern "C" int hfk_create(const hfk_basis* b, const hfk_options* opt, hfk_ctx** out)
+{
+    if (!b || !out || b->nbf <= 0 || !b->center || !b->lxyz || !b->prim_off || !b->coef || !b->expo)
+        return hfk_set_error(HFK_EINVAL, "hfk_create: null or empty basis");
+    *out = nullptr;
+    hfk_ctx* c = new hfk_ctx();
+    if (opt) c->opt = *opt;
+    else hfk_default_options(&c->opt);
+    c->n = b->nbf;
+    c->natom = b->natom;
+    c->center.assign(b->center, b->center + 3 * (size_t)b->nbf);
+    c->lxyz.assign(b->lxyz, b->lxyz + 3 * (size_t)b->nbf);
+    c->prim_off.assign(b->prim_off, b->prim_off + b->nbf + 1);
+    const int nprim = b->prim_off[b->nbf];
+    c->coef.assign(b->coef, b->coef + nprim);
+    c->expo.assign(b->expo, b->expo + nprim);
+    if (b->natom > 0 && b->Z && b->R) {
+        c->Z.assign(b->Z, b->Z + b->natom);
+        c->R.assign(b->R, b->R + 3 * (size_t)b->natom);
+    }
+    if (b->atom_of_bf) c->atom_of_bf.assign(b->atom_of_bf, b->atom_of_bf + b->nbf);
+    int rc = group_shells(c, b);
+    if (rc == HFK_OK) rc = hfk_ctx_init_device(c);
+    if (rc == HFK_OK) rc = hfk_ctx_build_pairs(c);
+    if (rc == HFK_OK) rc = hfk_ctx_schwarz(c);
+    if (rc != HFK_OK) {
+        hfk_ctx_free_device(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return HFK_OK;
+}
+
+extern "C" int hfk_destroy(hfk_ctx* c)
+{
+    if (!c) return HFK_OK;
+    hfk_ctx_free_device(c);
+    delete c;
+    return HFK_OK;
+}
+
+extern "C" int hfk_set_options(hfk_ctx* c, const hfk_options* opt)
+{
+    if (!c || !opt) return hfk_set_error(HFK_EINVAL, "hfk_set_options: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    const bool rebuild = opt->boys_mode != c->opt.boys_mode || opt->screen_mode != c->opt.screen_mode ||
+                         opt->screen_tau != c->opt.screen_tau;
+    c->opt = *opt;
+    int rc = HFK_OK;
+    if (rebuild) {
+        rc = hfk_ctx_classify_pairs(c);
+        if (rc == HFK_OK) rc = hfk_ctx_schwarz(c);
+    }
+    return rc;
+}
+
+int hfk_ctx_init_device(hfk_ctx* c)
+{
+    HFK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // Boys tables
+    std::vector<double> glx(128), glw(128);
+    gl256(glx.data(), glw.data());
+    HFK_TRY(dev_upload(c, &c->d_glx, glx));
+    HFK_TRY(dev_upload(c, &c->d_glw, glw));
+    HFK_TRY(dev_alloc(&c->d_boys_ref64, (size_t)BOYS_REF_M * BOYS_NX));
+    HFK_TRY(dev_alloc(&c->d_boys_ref32, (size_t)BOYS_REF_M * BOYS_NX));
+    HFK_TRY(dev_alloc(&c->d_boys_acc64, (size_t)BOYS_ACC_M * BOYS_ACC_NX));
+    HFK_TRY(dev_alloc(&c->d_boys_acc32, (size_t)BOYS_ACC_M * BOYS_ACC_NX));
+    hfk_launch_boys_tables(c->d_glx, c->d_glw, c->d_boys_ref64, c->d_boys_ref32, c->d_boys_acc64, c->d_boys_acc32,
+                           c->stream);
+    HFK_HIP(hipGetLastError());
+    c->bt.ref64 = c->d_boys_ref64;
+    c->bt.ref32 = c->d_boys_ref32;
+    c->bt.acc64 = c->d_boys_acc64;
+    c->bt.acc32 = c->d_boys_acc32;
+    for (int m = 0; m < 24; ++m) {
+        // asymptBoysF m x = (facOdd (floor (2m-1)) / 2^(m+1)) * sqrt (pi / x^(2m+1))  (Boys.hs:155-157)
+        c->bt.asym[m] = hs_facodd((int)floor(2.0 * m - 1.0)) / pow(2.0, m + 1.0);
+    }
+    // raw basis on device (one-electron integrals)
+    HFK_TRY(dev_upload(c, &c->d_center, c->center));
+    HFK_TRY(dev_upload(c, &c->d_lxyz, c->lxyz));
+    HFK_TRY(dev_upload(c, &c->d_prim_off, c->prim_off));
+    HFK_TRY(dev_upload(c, &c->d_coef, c->coef));
+    HFK_TRY(dev_upload(c, &c->d_expo, c->expo));
+    if (!c->Z.empty()) {
+        HFK_TRY(dev_upload(c, &c->d_Z, c->Z));
+        HFK_TRY(dev_upload(c, &c->d_R, c->R));
+    }
+    // shells on device
+    const int ns = (int)c->shells.size();
+    std::vector<double> shc(3 * (size_t)ns), shs((size_t)ns * HFK_MAXCOMP);
+    std::vector<int> shL(ns), shp0(ns), shnp(ns), shbf((size_t)ns * HFK_MAXCOMP);
+    for (int s = 0; s < ns; ++s) {
+        const HostShell& h = c->shells[s];
+        for (int x = 0; x < 3; ++x) shc[3 * s + x] = h.R[x];
+        shL[s] = h.L;
+        shp0[s] = h.prim0;
+        shnp[s] = h.nprim;
+        for (int k = 0; k < HFK_MAXCOMP; ++k) {
+            shbf[(size_t)s * HFK_MAXCOMP + k] = h.bf[k];
+            shs[(size_t)s * HFK_MAXCOMP + k] = h.scale[k];
+        }
+    }
+    HFK_TRY(dev_upload(c, &c->d_sh_center, shc));
+    HFK_TRY(dev_upload(c, &c->d_sh_L, shL));
+    HFK_TRY(dev_upload(c, &c->d_sh_prim0, shp0));
+    HFK_TRY(dev_upload(c, &c->d_sh_nprim, shnp));
+    HFK_TRY(dev_upload(c, &c->d_sh_bf, shbf));
+    HFK_TRY(dev_upload(c, &c->d_sh_scale, shs));
+    HFK_TRY(dev_upload(c, &c->d_sh_expo, c->sh_expo));
+    HFK_TRY(dev_upload(c, &c->d_sh_coef, c->sh_coef));
+    // work matrices
+    const size_t nn = (size_t)c->n * c->n, nf = (size_t)c->n * (c->n + 1) / 2;
+    HFK_TRY(dev_alloc(&c->d_D, nn));
+    HFK_TRY(dev_alloc(&c->d_J, nn));
+    HFK_TRY(dev_alloc(&c->d_K, nn));
+    HFK_TRY(dev_alloc(&c->d_flatA, nf));
+    HFK_TRY(dev_alloc(&c->d_flatB, nf));
+    HFK_TRY(dev_alloc(&c->d_flatC, nf));
+    HFK_HIP(hipEventCreate(&c->ev0));
+    HFK_HIP(hipEventCreate(&c->ev1));
+    return HFK_OK;
+}
+
+void hfk_ctx_free_device(hfk_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) hipFree(p);
+    c->allocs.clear();
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->blas) hfk_linalg_destroy(c);
+    if (c->stream) hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+}
+
+// Shell pairs: every (A >= B) by shell index, stored with the shell of the
+// larger angular momentum first.  Primitive-pair records and the
+// reference's screen values come from k_pair_setup.
+int hfk_ctx_build_pairs(hfk_ctx* c)
+{
+    const int ns = (int)c->shells.size();
+    std::vector<int> pA, pB, pp0, npp, q0;
+    long long npp_tot = 0, nq_tot = 0;
+    for (int a = 0; a < ns; ++a)
+        for (int b = 0; b <= a; ++b) {
+            int A = a, B = b;
+            if (c->shells[B].L > c->shells[A].L) std::swap(A, B);
+            pA.push_back(A);
+            pB.push_back(B);
+            pp0.push_back((int)npp_tot);
+            const int k = c->shells[A].nprim * c->shells[B].nprim;
+            npp.push_back(k);
+            q0.push_back((int)nq_tot);
+            npp_tot += k;
+            nq_tot += (long long)k * ncart(c->shells[A].L) * ncart(c->shells[B].L);
+        }
+    if (npp_tot > INT32_MAX || nq_tot > INT32_MAX)
+        return hfk_set_error(HFK_EUNSUP, "too many primitive pairs (%lld)", npp_tot);
+    c->npair = (int)pA.size();
+    c->h_pair_A = pA;
+    c->h_pair_B = pB;
+    c->h_pair_npp = npp;
+    HFK_TRY(dev_upload(c, &c->d_pair_A, pA));
+    HFK_TRY(dev_upload(c, &c->d_pair_B, pB));
+    HFK_TRY(dev_upload(c, &c->d_pair_pp0, pp0));
+    HFK_TRY(dev_upload(c, &c->d_pair_npp, npp));
+    std::vector<int> q0v(q0.begin(), q0.end());
+    int* d_q0 = nullptr;
+    HFK_TRY(dev_upload(c, &d_q0, q0v));
+    HFK_TRY(dev_alloc(&c->d_pp, (size_t)npp_tot * PP_STRIDE));
+    HFK_TRY(dev_alloc(&c->d_qcomp, (size_t)std::max(1LL, nq_tot)));
+    HFK_TRY(dev_alloc(&c->d_pair_qhi, (size_t)c->npair));
+    HFK_TRY(dev_alloc(&c->d_pair_sch, (size_t)c->npair));
+    HFK_TRY(dev_alloc(&c->d_pair_diag, (size_t)c->npair));
+    ShellDev sd;
+    sd.center = c->d_sh_center;
+    sd.L = c->d_sh_L;
+    sd.prim0 = c->d_sh_prim0;
+    sd.nprim = c->d_sh_nprim;
+    sd.expo = c->d_sh_expo;
+    sd.coef = c->d_sh_coef;
+    sd.scale = c->d_sh_scale;
+    hfk_launch_pair_setup(sd, c->npair, c->d_pair_A, c->d_pair_B, c->d_pair_pp0, d_q0, c->d_pp, c->d_qcomp,
+                          c->d_pair_qhi, c->stream);
+    HFK_HIP(hipGetLastError());
+    c->h_pair_qhi.resize(c->npair);
+    HFK_HIP(hipMemcpyAsync(c->h_pair_qhi.data(), c->d_pair_qhi, sizeof(double) * c->npair, hipMemcpyDeviceToHost,
+                           c->stream));
+    HFK_HIP(hipStreamSynchronize(c->stream));
+    c->stats.nshell = ns;
+    c->stats.nprim_pair = npp_tot;
+    return hfk_ctx_classify_pairs(c);
+}
+
+// Pair classes and the reference-exact pair screen: a pair whose largest
+// screen value times the largest over all pairs is below tau can never pass
+// the primitive screen, so it is dropped.
+int hfk_ctx_classify_pairs(hfk_ctx* c)
+{
+    const bool ref = c->opt.screen_mode == HFK_SCREEN_REFERENCE && c->opt.screen_tau > 0.0;
+    double gmax = 0.0;
+    for (double q : c->h_pair_qhi) gmax = std::max(gmax, q);   // NaN screened as +inf upstream
+    std::vector<std::vector<int>> lists(HFK_NPC);
+    long long kept = 0;
+    for (int p = 0; p < c->npair; ++p) {
+        if (ref && c->h_pair_qhi[p] * gmax < c->opt.screen_tau) continue;
+        const int la = c->shells[c->h_pair_A[p]].L, lb = c->shells[c->h_pair_B[p]].L;
+        lists[pair_class(la, lb)].push_back(p);
+        ++kept;
+    }
+    for (int k = 0; k < HFK_NPC; ++k) {
+        if (c->d_class[k]) { dev_free(c, c->d_class[k]); c->d_class[k] = nullptr; }
+        c->class_count[k] = (long long)lists[k].size();
+        c->h_class[k] = lists[k];
+        if (!lists[k].empty()) HFK_TRY(dev_upload(c, &c->d_class[k], lists[k]));
+    }
+    c->stats.nshell_pair = kept;
+    return HFK_OK;
+}
+
+QuartetArgs hfk_ctx_args(hfk_ctx* c)
+{
+    QuartetArgs a;
+    memset(&a, 0, sizeof a);
+    a.pp = c->d_pp;
+    a.qcomp = c->d_qcomp;
+    a.pair_pp0 = c->d_pair_pp0;
+    a.pair_npp = c->d_pair_npp;
+    a.pair_A = c->d_pair_A;
+    a.pair_B = c->d_pair_B;
+    a.pair_qhi = c->d_pair_qhi;
+    a.pair_sch = c->d_pair_sch;
+    a.sh_bf = c->d_sh_bf;
+    a.sh_scale = c->d_sh_scale;
+    a.bt = c->bt;
+    a.qtau = (c->opt.screen_mode == HFK_SCREEN_REFERENCE) ? c->opt.screen_tau : 0.0;
+    a.n = c->n;
+    return a;
+}
+
+// True Schwarz bounds: sqrt(max_ab |(ab|ab)|) per shell pair, unscreened.
+int hfk_ctx_schwarz(hfk_ctx* c)
+{
+    QuartetArgs a = hfk_ctx_args(c);
+    a.qtau = 0.0;
+    a.diag = c->d_pair_diag;
+    hfk_launch_fill(c->d_pair_diag, 0.0, c->npair, c->stream);
+    for (int k = 0; k < HFK_NPC; ++k) {
+        if (c->class_count[k] == 0) continue;
+        int la = 0, lb = 0;
+        hfk_pair_class_l(k, &la, &lb);
+        a.bra_list = c->d_class[k];
+        a.ket_list = c->d_class[k];
+        a.nbra = a.nket = c->class_count[k];
+        a.nquart = c->class_count[k];
+        int rc = hfk_launch_quartets(la, lb, la, lb, HFK_OP_DIAG, 0, c->opt.boys_mode, a, c->stream);
+        if (rc) return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", la, lb, la, lb);
+    }
+    HFK_HIP(hipGetLastError());
+    std::vector<double> diag(c->npair);
+    HFK_HIP(hipMemcpyAsync(diag.data(), c->d_pair_diag, sizeof(double) * c->npair, hipMemcpyDeviceToHost, c->stream));
+    HFK_HIP(hipStreamSynchronize(c->stream));
+    c->h_pair_sch.resize(c->npair);
+    for (int p = 0; p < c->npair; ++p) c->h_pair_sch[p] = sqrt(diag[p]);
+    HFK_HIP(hipMemcpyAsync(c->d_pair_sch, c->h_pair_sch.data(), sizeof(double) * c->npair, hipMemcpyHostToDevice,
+                           c->stream));
+    HFK_HIP(hipStreamSynchronize(c->stream));
+    return HFK_OK;
+}
+
+void hfk_pair_class_l(int k, int* la, int* lb)
+{
+    int a = 0;
+    while ((a + 1) * (a + 2) / 2 <= k) ++a;
+    *la = a;
+    *lb = k - a * (a + 1) / 2;
+}
+
+// Launch every class pair (bra class >= ket class) for one operation.
+int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
+{
+    long long total = 0;
+    for (int kb = 0; kb < HFK_NPC; ++kb) {
+        if (c->class_count[kb] == 0) continue;
+        for (int kk = 0; kk <= kb; ++kk) {
+            if (c->class_count[kk] == 0) continue;
+            int la, lb, lc, ld;
+            hfk_pair_class_l(kb, &la, &lb);
+            hfk_pair_class_l(kk, &lc, &ld);
+            a.bra_list = c->d_class[kb];
+            a.ket_list = c->d_class[kk];
+            a.nbra = c->class_count[kb];
+            a.nket = c->class_count[kk];
+            a.same = (kb == kk);
+            a.nquart = a.same ? a.nbra * (a.nbra + 1) / 2 : a.nbra * a.nket;
+            total += a.nquart;
+            const bool split = (op == HFK_OP_JK) && c->opt.fp32_threshold > 0.0;
+            if (!split) {
+                a.prec_sel = 0;
+                a.fp32_thr = 0.0;
+                int rc = hfk_launch_quartets(la, lb, lc, ld, op, 0, c->opt.boys_mode, a, c->stream);
+                if (rc) return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", la, lb, lc, ld);
+            } else {
+                a.fp32_thr = c->opt.fp32_threshold;
+                a.prec_sel = 1;
+                int rc = hfk_launch_quartets(la, lb, lc, ld, op, 0, c->opt.boys_mode, a, c->stream);
+                a.prec_sel = 2;
+                if (!rc) rc = hfk_launch_quartets(la, lb, lc, ld, op, 1, c->opt.boys_mode, a, c->stream);
+                if (rc) return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", la, lb, lc, ld);
+            }
+        }
+    }
+    HFK_HIP(hipGetLastError());
+    c->stats.quartets_launched = total;
+    return HFK_OK;
+}
+
+// Device Fock build: G (or H+G) flat from a flat density, all on device.
+int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat)
+{
+    const size_t nn = (size_t)c->n * c->n;
+    hfk_launch_flat_to_full(c->n, D_dev_flat, c->d_D, c->stream);
+    hfk_launch_fill(c->d_J, 0.0, (long long)nn, c->stream);
+    hfk_launch_fill(c->d_K, 0.0, (long long)nn, c->stream);
+    QuartetArgs a = hfk_ctx_args(c);
+    a.sch_cut = c->opt.schwarz_cut;
+    a.D = c->d_D;
+    a.J = c->d_J;
+    a.K = c->d_K;
+    HFK_HIP(hipEventRecord(c->ev0, c->stream));
+    HFK_TRY(hfk_ctx_run_quartets(c, a, HFK_OP_JK));
+    HFK_HIP(hipEventRecord(c->ev1, c->stream));
+    hfk_launch_symmetrize_g(c->n, c->d_J, c->d_K, H_dev, F_dev_flat, nullptr, c->stream);
+    HFK_HIP(hipGetLastError());
+    HFK_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HFK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.last_eri_ms = ms;
+    return HFK_OK;
+}
+
+extern "C" int hfk_fock_dev(hfk_ctx* c, const double* H_dev, const double* D_dev, double* F_dev)
+{
+    if (!c || !D_dev || !F_dev) return hfk_set_error(HFK_EINVAL, "hfk_fock_dev: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = hfk_ctx_fock_device(c, H_dev, D_dev, F_dev);
+    c->stats.last_fock_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+extern "C" int hfk_fock(hfk_ctx* c, const double* H_flat, const double* D_flat, double* F_flat)
+{
+    if (!c || !D_flat || !F_flat) return hfk_set_error(HFK_EINVAL, "hfk_fock: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    auto t0 = std::chrono::steady_clock::now();
+    const size_t nf = (size_t)c->n * (c->n + 1) / 2;
+    HFK_HIP(hipMemcpyAsync(c->d_flatA, D_flat, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+    const double* Hd = nullptr;
+    if (H_flat) {
+        HFK_HIP(hipMemcpyAsync(c->d_flatC, H_flat, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+        Hd = c->d_flatC;
+    }
+    HFK_TRY(hfk_ctx_fock_device(c, Hd, c->d_flatA, c->d_flatB));
+    HFK_HIP(hipMemcpyAsync(F_flat, c->d_flatB, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
+    HFK_HIP(hipStreamSynchronize(c->stream));
+    c->stats.last_fock_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return HFK_OK;
+}
+
+extern "C" int hfk_fock_G(hfk_ctx* c, const double* D_flat, double* G_flat)
+{
+    return hfk_fock(c, nullptr, D_flat, G_flat);
+}
+
+extern "C" int hfk_eri_flat(hfk_ctx* c, double* out, size_t out_len)
+{
+    if (!c || !out) return hfk_set_error(HFK_EINVAL, "hfk_eri_flat: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    const long long n = c->n, m = n * (n + 1) / 2, len = m * (m + 1) / 2;
+    if ((long long)out_len != len)
+        return hfk_set_error(HFK_EINVAL, "hfk_eri_flat: out_len %zu, need %lld", out_len, len);
+    // flat_base[i][j] = rank of (i,j,i,j) in calcIntegrals order
+    std::vector<long long> base((size_t)n * n, 0);
+    long long r = 0;
+    for (long long i = 0; i < n; ++i)
+        for (long long j = i; j < n; ++j) {
+            base[(size_t)i * n + j] = r;
+            r += (n - 1 - i) * (n - i) / 2 + (n - j);
+        }
+    long long* d_base = nullptr;
+    double* d_flat = nullptr;
+    HFK_TRY(dev_upload(c, &d_base, base));
+    HFK_TRY(dev_alloc(&d_flat, (size_t)len));
+    hfk_launch_fill(d_flat, 0.0, len, c->stream);
+    QuartetArgs a = hfk_ctx_args(c);
+    a.flat = d_flat;
+    a.flat_base = d_base;
+    int rc = hfk_ctx_run_quartets(c, a, HFK_OP_FLAT);
+    if (rc == HFK_OK) {
+        if (hipMemcpyAsync(out, d_flat, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = hfk_set_error(HFK_EHIP, "hfk_eri_flat: copy-out failed");
+    }
+    dev_free(c, d_flat);
+    dev_free(c, d_base);
+    return rc;
+}
+
+extern "C" int hfk_get_stats(hfk_ctx* c, hfk_stats* st)
+{
+    if (!c || !st) return hfk_set_error(HFK_EINVAL, "hfk_get_stats: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    *st = c->stats;
+    return HFK_OK;
+}
+
+static OneEArgs onee_args(hfk_ctx* c)
+{
+    OneEArgs a;
+    a.n = c->n;
+    a.center = c->d_center;
+    a.lxyz = c->d_lxyz;
+    a.prim_off = c->d_prim_off;
+    a.coef = c->d_coef;
+    a.expo = c->d_expo;
+    a.natom = (int)c->Z.size();
+    a.Z = c->d_Z;
+    a.R = c->d_R;
+    a.bt = c->bt;
+    a.boys = c->opt.boys_mode;
+    return a;
+}
+
+int hfk_ctx_overlap_dev(hfk_ctx* c, double* S_dev)
+{
+    hfk_launch_overlap(onee_args(c), S_dev, c->stream);
+    HFK_HIP(hipGetLastError());
+    return HFK_OK;
+}
+
+int hfk_ctx_hcore_dev(hfk_ctx* c, double* H_dev)
+{
+    if (c->Z.empty()) return hfk_set_error(HFK_EINVAL, "hfk_hcore: basis has no nuclei (natom/Z/R)");
+    hfk_launch_hcore(onee_args(c), H_dev, c->stream);
+    HFK_HIP(hipGetLastError());
+    return HFK_OK;
+}
+
+extern "C" int hfk_overlap(hfk_ctx* c, double* S_flat)
+{
+    if (!c || !S_flat) return hfk_set_error(HFK_EINVAL, "hfk_overlap: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    const size_t nf = (size_t)c->n * (c->n + 1) / 2;
+    HFK_TRY(hfk_ctx_overlap_dev(c, c->d_flatA));
+    HFK_HIP(hipMemcpyAsync(S_flat, c->d_flatA, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
+    HFK_HIP(hipStreamSynchronize(c->stream));
+    return HFK_OK;
+}
+
+extern "C" int hfk_hcore(hfk_ctx* c, double* H_flat)
+{
+    if (!c || !H_flat) return hfk_set_error(HFK_EINVAL, "hfk_hcore: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    const size_t nf = (size_t)c->n * (c->n + 1) / 2;
+    HFK_TRY(hfk_ctx_hcore_dev(c, c->d_flatA));
+    HFK_HIP(hipMemcpyAsync(H_flat, c->d_flatA, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
+    HFK_HIP(hipStreamSynchronize(c->stream));
+    return HFK_OK;
+}

@@ -1,0 +1,137 @@
+// Host-side context and helpers shared by the hfk host translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "hfk.h"
+#include "hfk_internal.hpp"
+
+#define HFK_NPC ((HFK_LMAX + 1) * (HFK_LMAX + 2) / 2)   // pair classes (la >= lb)
+
+int hfk_set_error(int code, const char* fmt, ...);
+
+#define HFK_HIP(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return hfk_set_error(HFK_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                                 __LINE__);                                                             \
+    } while (0)
+
+#define HFK_TRY(expr)              \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != HFK_OK) return r_; \
+    } while (0)
+
+struct HostShell {
+    int atom;
+    double R[3];
+    int L;
+    int prim0, nprim;           // into sh_expo / sh_coef
+    int bf[HFK_MAXCOMP];        // basis function of each canonical component, or -1
+    double scale[HFK_MAXCOMP];  // coefficient of the component / base coefficient
+};
+
+struct hfk_ctx {
+    std::mutex mu;
+    int n = 0, natom = 0;
+    hfk_options opt;
+    // basis as given
+    std::vector<double> center, coef, expo, Z, R;
+    std::vector<int> lxyz, prim_off, atom_of_bf;
+    // shells
+    std::vector<HostShell> shells;
+    std::vector<double> sh_expo, sh_coef;
+    // pairs
+    int npair = 0;
+    std::vector<int> h_pair_A, h_pair_B, h_pair_npp;
+    std::vector<double> h_pair_qhi, h_pair_sch;
+    std::vector<int> h_class[HFK_NPC];
+    long long class_count[HFK_NPC] = {0};
+    int* d_class[HFK_NPC] = {nullptr};
+    // device data
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void* blas = nullptr;                 // rocblas_handle
+    std::vector<void*> allocs;
+    double *d_glx = nullptr, *d_glw = nullptr;
+    double *d_boys_ref64 = nullptr, *d_boys_acc64 = nullptr;
+    float *d_boys_ref32 = nullptr, *d_boys_acc32 = nullptr;
+    BoysTables bt;
+    double *d_center = nullptr, *d_coef = nullptr, *d_expo = nullptr, *d_Z = nullptr, *d_R = nullptr;
+    int *d_lxyz = nullptr, *d_prim_off = nullptr;
+    double *d_sh_center = nullptr, *d_sh_scale = nullptr, *d_sh_expo = nullptr, *d_sh_coef = nullptr;
+    int *d_sh_L = nullptr, *d_sh_prim0 = nullptr, *d_sh_nprim = nullptr, *d_sh_bf = nullptr;
+    int *d_pair_A = nullptr, *d_pair_B = nullptr, *d_pair_pp0 = nullptr, *d_pair_npp = nullptr;
+    double *d_pp = nullptr, *d_qcomp = nullptr, *d_pair_qhi = nullptr, *d_pair_sch = nullptr, *d_pair_diag = nullptr;
+    double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr;
+    double *d_flatA = nullptr, *d_flatB = nullptr, *d_flatC = nullptr;
+    hfk_stats stats = {};
+};
+
+template <typename T>
+int dev_alloc(T** p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * n);
+    if (e != hipSuccess) return hfk_set_error(HFK_ENOMEM, "hipMalloc(%zu bytes): %s", sizeof(T) * n, hipGetErrorString(e));
+    return HFK_OK;
+}
+
+template <typename T>
+int dev_alloc_tracked(hfk_ctx* c, T** p, size_t n)
+{
+    int rc = dev_alloc(p, n);
+    if (rc == HFK_OK) c->allocs.push_back((void*)*p);
+    return rc;
+}
+
+template <typename T>
+int dev_upload(hfk_ctx* c, T** p, const std::vector<T>& v)
+{
+    int rc = dev_alloc(p, v.size());
+    if (rc != HFK_OK) return rc;
+    c->allocs.push_back((void*)*p);
+    if (!v.empty()) {
+        hipError_t e = hipMemcpyAsync(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return hfk_set_error(HFK_EHIP, "upload: %s", hipGetErrorString(e));
+    }
+    return HFK_OK;
+}
+
+template <typename T>
+void dev_free(hfk_ctx* c, T* p)
+{
+    if (!p) return;
+    hipStreamSynchronize(c->stream);
+    for (size_t i = 0; i < c->allocs.size(); ++i)
+        if (c->allocs[i] == (void*)p) {
+            c->allocs.erase(c->allocs.begin() + i);
+            break;
+        }
+    hipFree((void*)p);
+}
+
+// dev_alloc used for context-owned buffers is tracked too
+#define dev_alloc(p, n) dev_alloc_tracked(c, p, n)
+
+int hfk_ctx_init_device(hfk_ctx* c);
+void hfk_ctx_free_device(hfk_ctx* c);
+int hfk_ctx_classify_pairs(hfk_ctx* c);
+void hfk_pair_class_l(int k, int* la, int* lb);
+QuartetArgs hfk_ctx_args(hfk_ctx* c);
+int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op);
+int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat);
+int hfk_ctx_overlap_dev(hfk_ctx* c, double* S_dev);
+int hfk_ctx_hcore_dev(hfk_ctx* c, double* H_dev);
+
+// linear algebra helpers (hfk_linalg_host.hip)
+int hfk_linalg_init(hfk_ctx* c);
+void hfk_linalg_destroy(hfk_ctx* c);
+// symmetric eigen-solve on device: A (n x n row-major, overwritten by
+// eigenvectors as ROWS: A[k][:] = k-th eigenvector), w ascending.
+int hfk_dev_syev(hfk_ctx* c, int n, double* A, double* w);

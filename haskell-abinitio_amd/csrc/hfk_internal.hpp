@@ -1,0 +1,98 @@
+// Internal interfaces between the host C-ABI layer and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hfk_device.hpp"
+
+#define HFK_MAXCOMP 6        // Cartesian components of a d shell
+#define HFK_LMAX 2           // highest shell angular momentum with generated kernels
+
+// Everything a quartet kernel needs.  Pair p: shells pair_A[p] (the one
+// with the larger angular momentum) and pair_B[p]; its primitive pairs are
+// pp[(pair_pp0[p] + i) * PP_STRIDE], i < pair_npp[p].
+struct QuartetArgs {
+    const double* pp;
+    const double* qcomp;
+    const int* pair_pp0;
+    const int* pair_npp;
+    const int* pair_A;
+    const int* pair_B;
+    const double* pair_qhi;     // max reference screen value over the pair (NaN -> +inf)
+    const double* pair_sch;     // Schwarz bound sqrt(max |(ab|ab)|), for the precision split
+    const int* bra_list;        // pair indices of the bra class
+    const int* ket_list;        // pair indices of the ket class
+    long long nbra, nket;
+    int same;                   // bra class == ket class: only j <= i
+    long long nquart;
+    const int* sh_bf;           // [nshell][HFK_MAXCOMP] basis-function index or -1
+    const double* sh_scale;     // [nshell][HFK_MAXCOMP] component coefficient scale
+    BoysTables bt;
+    double qtau;                // reference primitive screen threshold (0 = off)
+    double sch_cut;             // skip quartets with Schwarz bound below (0 = off)
+    double fp32_thr;            // precision split threshold on the Schwarz bound
+    int prec_sel;               // 0: all quartets; 1: bound >= fp32_thr; 2: bound < fp32_thr
+    int n;                      // number of basis functions
+    const double* D;            // [n][n] density (symmetric)
+    double* J;                  // [n][n] accumulator J' (symmetrised on output)
+    double* K;                  // [n][n] accumulator K'
+    double* flat;               // flat ERI output (calcIntegrals order)
+    const long long* flat_base; // [n][n] rank of (i,j,i,j)
+    double* diag;               // per-pair max |(ab|ab)| output (Schwarz setup)
+};
+
+enum { HFK_OP_JK = 0, HFK_OP_FLAT = 1, HFK_OP_DIAG = 2 };
+
+// Launch one class kernel.  classes are (la,lb,lc,ld) with la>=lb, lc>=ld,
+// (la,lb) >= (lc,ld).  prec: 0 = FP64, 1 = FP32.  boys: 0 reference, 1 accurate.
+int hfk_launch_quartets(int la, int lb, int lc, int ld, int op, int prec, int boys,
+                        const QuartetArgs& a, hipStream_t s);
+
+// Setup kernels (hfk_setup.hip)
+struct ShellDev {
+    const double* center;   // [nshell][3]
+    const int* L;
+    const int* prim0;       // first primitive in expo[]/coef[]
+    const int* nprim;
+    const double* expo;
+    const double* coef;     // base contraction coefficients
+    const double* scale;    // [nshell][HFK_MAXCOMP]
+};
+
+void hfk_launch_pair_setup(const ShellDev& sh, int npair, const int* pair_A, const int* pair_B,
+                           const int* pair_pp0, const int* pair_q0, double* pp, double* qcomp,
+                           double* pair_qhi, hipStream_t s);
+void hfk_launch_boys_tables(const double* glx, const double* glw, double* ref64, float* ref32,
+                            double* acc64, float* acc32, hipStream_t s);
+void hfk_launch_fill(double* p, double v, long long n, hipStream_t s);
+void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const double* H, double* G_flat,
+                             double* F_full, hipStream_t s);
+void hfk_launch_flat_to_full(int n, const double* flat, double* full, hipStream_t s);
+void hfk_launch_full_to_flat(int n, const double* full, double* flat, hipStream_t s);
+
+// One-electron integrals (hfk_onee.hip)
+struct OneEArgs {
+    int n;
+    const double* center;   // [n][3]
+    const int* lxyz;        // [n][3]
+    const int* prim_off;    // [n+1]
+    const double* coef;
+    const double* expo;
+    int natom;
+    const double* Z;
+    const double* R;        // [natom][3]
+    BoysTables bt;
+    int boys;
+};
+void hfk_launch_overlap(const OneEArgs& a, double* S_flat, hipStream_t s);
+void hfk_launch_hcore(const OneEArgs& a, double* H_flat, hipStream_t s);
+
+// Dense linear algebra (hfk_linalg.hip): row-major FP64, MFMA.
+// C = alpha * op(A) * op(B) + beta * C, with op = transpose when ta/tb.
+void hfk_launch_dgemm(int m, int n, int k, double alpha, const double* A, int lda, int ta,
+                      const double* B, int ldb, int tb, double beta, double* C, int ldc, hipStream_t s);
+void hfk_launch_scale_cols(int n, double* U, const double* w, int pow_mode, hipStream_t s);
+void hfk_launch_reduce(long long n, const double* x, const double* y, int mode, double* part, double* out, hipStream_t s);
+void hfk_launch_axpby(long long n, double a, const double* x, double b, double* y, hipStream_t s);
+void hfk_launch_antisym(int n, const double* T, double* E, hipStream_t s);
+void hfk_launch_guess_density(int n, int occ, double* Dflat, hipStream_t s);

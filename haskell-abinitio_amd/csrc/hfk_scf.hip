@@ -1,0 +1,435 @@
+// SCF-side entry points: density, symmetric orthogonalisation, unitary
+// transform and the DIIS-accelerated RHF driver (HartreeFock/HartreeFock.hs:
+// 70-135,195-287, HartreeFock/DIIS.hs:55-137), all FP64 on the device.  The
+// eigen-solve is rocSOLVER dsyevd; the dense products are the MFMA GEMM.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <chrono>
+#include <deque>
+#include <vector>
+
+#include "hfk.h"
+#include "hfk_host.hpp"
+
+int hfk_linalg_init(hfk_ctx* c)
+{
+    if (c->blas) return HFK_OK;
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+        return hfk_set_error(HFK_EHIP, "rocblas_create_handle failed");
+    rocblas_set_stream(h, c->stream);
+    c->blas = (void*)h;
+    return HFK_OK;
+}
+
+void hfk_linalg_destroy(hfk_ctx* c)
+{
+    if (c->blas) rocblas_destroy_handle((rocblas_handle)c->blas);
+    c->blas = nullptr;
+}
+
+// Symmetric eigen-solve (EigenValues.hs:29-31 + VectorTools.hs:27-31):
+// eigenvalues ascending; on return row k of A is the k-th eigenvector.
+int hfk_dev_syev(hfk_ctx* c, int n, double* A, double* w)
+{
+    HFK_TRY(hfk_linalg_init(c));
+    double* work = nullptr;
+    int* info = nullptr;
+    HFK_TRY(dev_alloc(&work, (size_t)n));
+    HFK_TRY(dev_alloc(&info, 1));
+    // A is symmetric, so row-major == column-major on input; dsyevd leaves
+    // eigenvectors in columns (column-major) == rows in our row-major view.
+    rocblas_status st = rocsolver_dsyevd((rocblas_handle)c->blas, rocblas_evect_original, rocblas_fill_upper, n, A, n,
+                                         w, work, info);
+    int hinfo = 0;
+    hipError_t e = hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dev_free(c, work);
+    dev_free(c, info);
+    if (st != rocblas_status_success) return hfk_set_error(HFK_EHIP, "rocsolver_dsyevd failed (%d)", (int)st);
+    if (e != hipSuccess) return hfk_set_error(HFK_EHIP, "syev info copy: %s", hipGetErrorString(e));
+    if (hinfo != 0) return hfk_set_error(HFK_ENUMERIC, "rocsolver_dsyevd info=%d", hinfo);
+    return HFK_OK;
+}
+
+namespace {
+
+struct Work {
+    hfk_ctx* c;
+    int n;
+    double *S = nullptr, *X = nullptr, *H = nullptr;          // S, X full; H flat
+    double *T1 = nullptr, *T2 = nullptr, *T3 = nullptr, *Ev = nullptr, *w = nullptr;
+    double *Ffull = nullptr, *C = nullptr;
+    double *part = nullptr, *scal = nullptr;
+};
+
+int work_alloc(hfk_ctx* c, Work& W)
+{
+    W.c = c;
+    W.n = c->n;
+    const size_t nn = (size_t)c->n * c->n, nf = (size_t)c->n * (c->n + 1) / 2;
+    HFK_TRY(dev_alloc(&W.S, nn));
+    HFK_TRY(dev_alloc(&W.X, nn));
+    HFK_TRY(dev_alloc(&W.H, nf));
+    HFK_TRY(dev_alloc(&W.T1, nn));
+    HFK_TRY(dev_alloc(&W.T2, nn));
+    HFK_TRY(dev_alloc(&W.T3, nn));
+    HFK_TRY(dev_alloc(&W.Ev, nn));
+    HFK_TRY(dev_alloc(&W.w, (size_t)c->n));
+    HFK_TRY(dev_alloc(&W.Ffull, nn));
+    HFK_TRY(dev_alloc(&W.C, nn));
+    HFK_TRY(dev_alloc(&W.part, 1024));
+    HFK_TRY(dev_alloc(&W.scal, 64));
+    return HFK_OK;
+}
+
+void work_free(Work& W)
+{
+    hfk_ctx* c = W.c;
+    double* ps[] = {W.S, W.X, W.H, W.T1, W.T2, W.T3, W.Ev, W.w, W.Ffull, W.C, W.part, W.scal};
+    for (double* p : ps) dev_free(c, p);
+}
+
+void gemm(hfk_ctx* c, int m, int n, int k, double alpha, const double* A, int lda, int ta, const double* B, int ldb,
+          int tb, double beta, double* C, int ldc)
+{
+    hfk_launch_dgemm(m, n, k, alpha, A, lda, ta, B, ldb, tb, beta, C, ldc, c->stream);
+}
+
+// X = U s^-1/2 U^T  (BasisOrthogonalization.hs:26-35); S full (n x n) in W.S
+int symm_ortho_dev(Work& W, const double* S, double* X)
+{
+    hfk_ctx* c = W.c;
+    const int n = W.n;
+    HFK_HIP(hipMemcpyAsync(W.Ev, S, sizeof(double) * n * n, hipMemcpyDeviceToDevice, c->stream));
+    HFK_TRY(hfk_dev_syev(c, n, W.Ev, W.w));
+    // rows of Ev are eigenvectors u_k; X = sum_k u_k s_k^-1/2 u_k^T = Ev^T diag(s^-1/2) Ev
+    HFK_HIP(hipMemcpyAsync(W.T1, W.Ev, sizeof(double) * n * n, hipMemcpyDeviceToDevice, c->stream));
+    hfk_launch_scale_cols(n, W.T1, W.w, 0 | 4, c->stream);   // scale row k by s_k^-1/2
+    gemm(c, n, n, n, 1.0, W.Ev, n, 1, W.T1, n, 0, 0.0, X, n);
+    HFK_HIP(hipGetLastError());
+    return HFK_OK;
+}
+
+// Fp = X^T F X  (LinearAlgebra.hs:155-159); F full
+void unitary_dev(Work& W, const double* X, const double* F, double* Fp)
+{
+    const int n = W.n;
+    gemm(W.c, n, n, n, 1.0, F, n, 0, X, n, 0, 0.0, W.T3, n);       // F X
+    gemm(W.c, n, n, n, 1.0, X, n, 1, W.T3, n, 0, 0.0, Fp, n);      // X^T (F X)
+}
+
+// D = 2 C_occ C_occ^T  (HartreeFock.hs:195-208), D full
+void density_dev(Work& W, const double* C, int occ, double* Dfull)
+{
+    const int n = W.n;
+    gemm(W.c, n, n, occ, 2.0, C, n, 0, C, n, 1, 0.0, Dfull, n);
+}
+
+// diagonalHF (HartreeFock.hs:263-273): F flat -> C (W.C), eps (W.w), D flat
+int diagonal_hf(Work& W, const double* Fflat, int occ, double* Dflat)
+{
+    hfk_ctx* c = W.c;
+    const int n = W.n;
+    hfk_launch_flat_to_full(n, Fflat, W.Ffull, c->stream);
+    unitary_dev(W, W.X, W.Ffull, W.Ev);
+    HFK_TRY(hfk_dev_syev(c, n, W.Ev, W.w));
+    gemm(c, n, n, n, 1.0, W.X, n, 0, W.Ev, n, 1, 0.0, W.C, n);   // C = X C', C' columns = eigenvectors
+    density_dev(W, W.C, occ, W.T1);
+    hfk_launch_full_to_flat(n, W.T1, Dflat, c->stream);
+    HFK_HIP(hipGetLastError());
+    return HFK_OK;
+}
+
+double read_scalar(hfk_ctx* c, const double* d)
+{
+    double v = 0.0;
+    hipMemcpyAsync(&v, d, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    hipStreamSynchronize(c->stream);
+    return v;
+}
+
+// gaussElem (NumericalTools/GaussElimination.hs:42-112): elimination without
+// pivoting; fails when a row has no entry above 1e-6 or a NaN/Inf appears.
+bool gauss_elem(std::vector<double> A, std::vector<double> b, int m, std::vector<double>& x)
+{
+    for (int k = 0; k < m; ++k) {
+        std::vector<double> A2 = A, b2 = b;
+        for (int i = k + 1; i < m; ++i) {
+            const double alpha = A[(size_t)i * m + k] / A[(size_t)k * m + k];
+            for (int j = k; j < m; ++j) A2[(size_t)i * m + j] = A[(size_t)i * m + j] - alpha * A[(size_t)k * m + j];
+            b2[i] = b[i] - alpha * b[k];
+        }
+        A = A2;
+        b = b2;
+    }
+    for (int i = 0; i < m; ++i) {
+        bool any = false;
+        for (int j = 0; j < m; ++j) any = any || fabs(A[(size_t)i * m + j]) > 1e-6;
+        if (!any) return false;
+    }
+    for (double v : A)
+        if (std::isnan(v) || std::isinf(v)) return false;
+    x.assign(m, 0.0);
+    for (int i = m - 1; i >= 0; --i) {
+        double s = 0.0;
+        for (int j = i + 1; j < m; ++j) s += x[j] * A[(size_t)i * m + j];
+        x[i] = (b[i] - s) * (1.0 / A[(size_t)i * m + i]);
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" void hfk_default_scf_options(hfk_scf_options* o)
+{
+    o->guess = HFK_GUESS_INTENDED;
+    o->max_steps = 30;      // HartreeFock.hs:87
+    o->diis_size = 5;       // HartreeFock.hs:78
+    o->diis_start = 0.1;    // DIIS.hs:58
+    o->converge = 1.0e-7;   // HartreeFock.hs:113
+    o->charge = 0.0;
+}
+
+extern "C" int hfk_density(hfk_ctx* c, const double* C, int32_t occ, double* D_flat)
+{
+    if (!c || !C || !D_flat || occ < 0 || occ > c->n) return hfk_set_error(HFK_EINVAL, "hfk_density: bad argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    Work W;
+    int rc = work_alloc(c, W);
+    const int n = c->n;
+    if (rc == HFK_OK) {
+        hipMemcpyAsync(W.C, C, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream);
+        density_dev(W, W.C, occ, W.T1);
+        hfk_launch_full_to_flat(n, W.T1, c->d_flatA, c->stream);
+        hipMemcpyAsync(D_flat, c->d_flatA, sizeof(double) * n * (n + 1) / 2, hipMemcpyDeviceToHost, c->stream);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) rc = hfk_set_error(HFK_EHIP, "hfk_density: HIP failure");
+    }
+    work_free(W);
+    return rc;
+}
+
+extern "C" int hfk_symm_ortho(hfk_ctx* c, const double* S, double* X)
+{
+    if (!c || !S || !X) return hfk_set_error(HFK_EINVAL, "hfk_symm_ortho: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    Work W;
+    int rc = work_alloc(c, W);
+    const int n = c->n;
+    if (rc == HFK_OK) {
+        hipMemcpyAsync(W.S, S, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream);
+        rc = symm_ortho_dev(W, W.S, W.X);
+        if (rc == HFK_OK) {
+            hipMemcpyAsync(X, W.X, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->stream);
+            if (hipStreamSynchronize(c->stream) != hipSuccess) rc = hfk_set_error(HFK_EHIP, "hfk_symm_ortho: HIP failure");
+        }
+    }
+    work_free(W);
+    return rc;
+}
+
+extern "C" int hfk_unitary(hfk_ctx* c, const double* X, const double* F_flat, double* Fp)
+{
+    if (!c || !X || !F_flat || !Fp) return hfk_set_error(HFK_EINVAL, "hfk_unitary: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    Work W;
+    int rc = work_alloc(c, W);
+    const int n = c->n;
+    if (rc == HFK_OK) {
+        hipMemcpyAsync(W.X, X, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream);
+        hipMemcpyAsync(c->d_flatA, F_flat, sizeof(double) * n * (n + 1) / 2, hipMemcpyHostToDevice, c->stream);
+        hfk_launch_flat_to_full(n, c->d_flatA, W.Ffull, c->stream);
+        unitary_dev(W, W.X, W.Ffull, W.T1);
+        hipMemcpyAsync(Fp, W.T1, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->stream);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) rc = hfk_set_error(HFK_EHIP, "hfk_unitary: HIP failure");
+    }
+    work_free(W);
+    return rc;
+}
+
+// scfHF + scfDIIS (HartreeFock.hs:70-123)
+extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res, double* C_out, double* eps_out,
+                       double* D_out, double* F_out)
+{
+    if (!c || !res) return hfk_set_error(HFK_EINVAL, "hfk_scf: null argument");
+    if (c->Z.empty()) return hfk_set_error(HFK_EINVAL, "hfk_scf: basis has no nuclei");
+    hfk_scf_options opt;
+    if (o) opt = *o;
+    else hfk_default_scf_options(&opt);
+    std::lock_guard<std::mutex> g(c->mu);
+    auto t0 = std::chrono::steady_clock::now();
+    double fock_s = 0.0;
+    const int n = c->n;
+    const size_t nn = (size_t)n * n, nf = (size_t)n * (n + 1) / 2;
+    double zsum = 0.0;
+    for (double z : c->Z) zsum += z;
+    const int occ = (int)floor((zsum - opt.charge) / 2.0);   // HartreeFock.hs:75-76
+    if (occ <= 0 || occ > n) return hfk_set_error(HFK_EINVAL, "hfk_scf: occupied=%d for n=%d", occ, n);
+    // nuclearRep (HartreeFock.hs:300-305)
+    double vnn = 0.0;
+    for (int a = 0; a < c->natom; ++a)
+        for (int b = a + 1; b < c->natom; ++b) {
+            double r2 = 0.0;
+            for (int x = 0; x < 3; ++x) {
+                const double d = c->R[3 * a + x] - c->R[3 * b + x];
+                r2 += d * d;
+            }
+            vnn += c->Z[a] * c->Z[b] / sqrt(r2);
+        }
+
+    Work W;
+    int rc = work_alloc(c, W);
+    double *Dflat = nullptr, *Fflat = nullptr, *Fd = nullptr, *Dfull = nullptr, *Sfull = nullptr;
+    std::vector<double*> errs, focks;   // DIIS history, newest first
+    if (rc == HFK_OK) rc = dev_alloc(&Dflat, nf);
+    if (rc == HFK_OK) rc = dev_alloc(&Fflat, nf);
+    if (rc == HFK_OK) rc = dev_alloc(&Fd, nf);
+    if (rc == HFK_OK) rc = dev_alloc(&Dfull, nn);
+    if (rc == HFK_OK) rc = dev_alloc(&Sfull, nn);
+    for (int k = 0; k < opt.diis_size && rc == HFK_OK; ++k) {
+        double *e = nullptr, *f = nullptr;
+        rc = dev_alloc(&e, nn);
+        if (rc == HFK_OK) rc = dev_alloc(&f, nf);
+        errs.push_back(e);
+        focks.push_back(f);
+    }
+    std::vector<double*> free_e(errs), free_f(focks);
+    std::deque<int> hist;   // indices into errs/focks, newest first
+    auto run = [&]() -> int {
+        HFK_TRY(hfk_ctx_hcore_dev(c, W.H));
+        HFK_TRY(hfk_ctx_overlap_dev(c, c->d_flatA));
+        hfk_launch_flat_to_full(n, c->d_flatA, Sfull, c->stream);
+        HFK_TRY(symm_ortho_dev(W, Sfull, W.X));
+        // initial density
+        if (opt.guess == HFK_GUESS_CORE) {
+            HFK_TRY(diagonal_hf(W, W.H, occ, Dflat));
+        } else {
+            hfk_launch_guess_density(n, occ, Dflat, c->stream);
+            auto f0 = std::chrono::steady_clock::now();
+            HFK_TRY(hfk_ctx_fock_device(c, W.H, Dflat, Fflat));
+            fock_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - f0).count();
+            HFK_TRY(diagonal_hf(W, Fflat, occ, Dflat));
+        }
+        bool diis_on = false;
+        for (int step = 0; step < opt.max_steps; ++step) {
+            auto f0 = std::chrono::steady_clock::now();
+            HFK_TRY(hfk_ctx_fock_device(c, W.H, Dflat, Fflat));   // fock (HartreeFock.hs:213-222)
+            fock_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - f0).count();
+            // calcErrorMtx (DIIS.hs:119-132): X^T (F D S - S D F) X
+            hfk_launch_flat_to_full(n, Fflat, W.Ffull, c->stream);
+            hfk_launch_flat_to_full(n, Dflat, Dfull, c->stream);
+            gemm(c, n, n, n, 1.0, Dfull, n, 0, Sfull, n, 0, 0.0, W.T1, n);     // D S
+            gemm(c, n, n, n, 1.0, W.Ffull, n, 0, W.T1, n, 0, 0.0, W.T2, n);   // F D S
+            hfk_launch_antisym(n, W.T2, W.T1, c->stream);                      // FDS - SDF
+            gemm(c, n, n, n, 1.0, W.T1, n, 0, W.X, n, 0, 0.0, W.T2, n);
+            // a storage slot for the current error/Fock pair (the history
+            // holds at most diis_size-1 entries, so one is always free)
+            int cur = -1;
+            for (int k = 0; k < opt.diis_size; ++k) {
+                bool used = false;
+                for (int h : hist) used = used || (h == k);
+                if (!used) { cur = k; break; }
+            }
+            if (cur < 0) return hfk_set_error(HFK_EINVAL, "hfk_scf: DIIS history overflow");
+            double* ecur = errs[cur];
+            gemm(c, n, n, n, 1.0, W.X, n, 1, W.T2, n, 0, 0.0, ecur, n);
+            HFK_HIP(hipMemcpyAsync(focks[cur], Fflat, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream));
+            hfk_launch_reduce((long long)nn, ecur, nullptr, 1, W.part, W.scal, c->stream);
+            const double maxe = read_scalar(c, W.scal);
+            if (std::isnan(maxe)) return hfk_set_error(HFK_ENUMERIC, "hfk_scf: NaN in the DIIS error at step %d", step);
+            // diisDriver (DIIS.hs:55-65)
+            const double* Fuse = Fflat;
+            auto update = [&](std::deque<int>& hs) {
+                // updateDIIS (DIIS.hs:111-114): keep at most diis_size-1 stored after adding
+                hs.push_front(cur);
+                if ((int)hs.size() >= opt.diis_size) hs.pop_back();
+            };
+            if (!diis_on) {
+                if (step > 0 && maxe < opt.diis_start) {
+                    update(hist);
+                    diis_on = true;
+                }
+            } else {
+                // diis (DIIS.hs:70-88) with the current pair in front of the history
+                std::deque<int> use(hist);
+                for (;;) {
+                    const int m = (int)use.size() + 1;
+                    std::vector<int> idx;
+                    idx.push_back(cur);
+                    for (int h : use) idx.push_back(h);
+                    std::vector<double> B((size_t)(m + 1) * (m + 1), 0.0), rhs(m + 1, 0.0);
+                    for (int i = 0; i < m; ++i)
+                        for (int j = 0; j <= i; ++j) {
+                            hfk_launch_reduce((long long)nn, errs[idx[i]], errs[idx[j]], 0, W.part, W.scal + 1,
+                                              c->stream);
+                            const double v = read_scalar(c, W.scal + 1);   // trace(e_i^T e_j)
+                            B[(size_t)i * (m + 1) + j] = v;
+                            B[(size_t)j * (m + 1) + i] = v;
+                        }
+                    for (int i = 0; i < m; ++i) {
+                        B[(size_t)i * (m + 1) + m] = -1.0;
+                        B[(size_t)m * (m + 1) + i] = -1.0;
+                    }
+                    rhs[m] = -1.0;
+                    std::vector<double> cs;
+                    if (gauss_elem(B, rhs, m + 1, cs)) {
+                        hfk_launch_fill(Fd, 0.0, (long long)nf, c->stream);
+                        for (int i = 0; i < m; ++i) hfk_launch_axpby((long long)nf, cs[i], focks[idx[i]], 1.0, Fd, c->stream);
+                        Fuse = Fd;
+                        break;
+                    }
+                    if (use.empty()) break;   // nothing left to drop: keep the plain Fock
+                    use.pop_back();           // initS: drop the oldest and retry
+                    hist = use;
+                }
+                update(hist);
+            }
+            // diagonalHF + variationalE (HartreeFock.hs:111-112, 277-287)
+            HFK_TRY(diagonal_hf(W, Fuse, occ, Dflat));
+            hfk_launch_flat_to_full(n, Dflat, Dfull, c->stream);
+            hfk_launch_flat_to_full(n, Fuse, W.T1, c->stream);
+            hfk_launch_flat_to_full(n, W.H, W.T2, c->stream);
+            hfk_launch_axpby((long long)nn, 1.0, W.T2, 1.0, W.T1, c->stream);   // H + F
+            hfk_launch_reduce((long long)nn, Dfull, W.T1, 0, W.part, W.scal + 2, c->stream);
+            const double etot = 0.5 * read_scalar(c, W.scal + 2) + vnn;
+            res->energy = etot;
+            res->e_nuc = vnn;
+            res->max_error = maxe;
+            res->steps = step + 1;
+            res->occupied = occ;
+            if (step != 0 && maxe < opt.converge) {
+                if (F_out) HFK_HIP(hipMemcpyAsync(F_out, Fuse, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
+                if (D_out) HFK_HIP(hipMemcpyAsync(D_out, Dflat, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
+                if (C_out) HFK_HIP(hipMemcpyAsync(C_out, W.C, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+                if (eps_out) HFK_HIP(hipMemcpyAsync(eps_out, W.w, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+                HFK_HIP(hipStreamSynchronize(c->stream));
+                return HFK_OK;
+            }
+        }
+        return hfk_set_error(HFK_ESCF, "SCF maxium steps exceeded");   // HartreeFock.hs:123
+    };
+    if (rc == HFK_OK) rc = run();
+    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    res->fock_seconds = fock_s;
+    for (double* p : free_e) dev_free(c, p);
+    for (double* p : free_f) dev_free(c, p);
+    dev_free(c, Dflat);
+    dev_free(c, Fflat);
+    dev_free(c, Fd);
+    dev_free(c, Dfull);
+    dev_free(c, Sfull);
+    work_free(W);
+    return rc;
+}
+
+extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_flat, double* g)
+{
+    (void)D_flat;
+    (void)W_flat;
+    (void)g;
+    if (!c) return hfk_set_error(HFK_EINVAL, "hfk_gradient: null context");
+    return hfk_set_error(HFK_EUNSUP, "hfk_gradient: not built yet");
+}
