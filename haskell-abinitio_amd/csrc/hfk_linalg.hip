@@ -14,7 +14,8 @@ constexpr int TM = 64, TN = 64, TK = 16;
 // C[m][n] = alpha * sum_k op(A)[m][k] op(B)[k][n] + beta * C[m][n], row-major.
 // 256 threads = 4 waves; each wave owns a 32x32 quarter of the 64x64 tile as
 // 2x2 MFMA 16x16 blocks.  MFMA f64 16x16x4 operand layout: lane l supplies
-// A[l%16][l/16] and B[l/16][l%16]; the accumulator holds D[4*(l/16)+r][l%16].
+// A[l%16][l/16] and B[l/16][l%16]; accumulator element r of lane l is
+// D[l/16 + 4r][l%16].
 template <int TA, int TB>
 __global__ void __launch_bounds__(256) k_dgemm(int M, int N, int K, double alpha, const double* __restrict__ A,
                                                int lda, const double* __restrict__ B, int ldb, double beta,
@@ -69,7 +70,7 @@ __global__ void __launch_bounds__(256) k_dgemm(int M, int N, int K, double alpha
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int gm = m0 + wm + i * 16 + (lane >> 4) + 4 * r;
                 const int gn = n0 + wn + j * 16 + (lane & 15);
                 if (gm < M && gn < N) {
                     double* p = C + (size_t)gm * ldc + gn;

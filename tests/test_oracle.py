@@ -1,0 +1,242 @@
+"""Pins of the CPU oracle (no GPU).  See oracle/hfk_oracle.py's header for
+what each test pins."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import hfk_oracle as O
+from hfk import basis, systems
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_gl256_matches_reference_constants():
+    g = json.load(open(os.path.join(GOLD, "gl256.json")))
+    x, w = O.gl256()
+    np.testing.assert_allclose(x, g["points256"], rtol=2e-16, atol=0)
+    np.testing.assert_allclose(w, g["weights256"], rtol=4e-15, atol=0)
+    # exactness: the symmetric rule integrates x^(2k) over [-1,1], k < 256
+    for k in (0, 1, 10, 100, 255):
+        val = 2.0 * np.sum(w * x ** (2 * k))
+        assert abs(val - 2.0 / (2 * k + 1)) < 1e-13
+
+
+def test_boys_table_f0_nodes_are_erf_closed_form():
+    t = O.boys_table()
+    for i in (1, 7, 100, 499, 500):
+        x = 0.1 * i
+        ref = 0.5 * math.sqrt(math.pi / x) * math.erf(math.sqrt(x)) if x <= 50 else 0.5 * math.sqrt(math.pi / x)
+        assert abs(t[i] - ref) <= 2e-16 * ref
+    assert t[0] == 1.0
+
+
+def test_boys_table_deviation_profile():
+    """The reference's m>=1 nodes come from a 256-point Gauss-Legendre
+    quadrature of t^(m-1/2) e^(-xt) (Boys.hs:130-136,176-188); its error
+    is largest for m=1 (integrable t^(1/2) singularity).  Independent
+    accurate values: mpmath (via tests/os_eri.boys)."""
+    import os_eri
+    t = O.boys_table()
+    worst = {}
+    for m in (1, 2, 3, 6, 10, 14):
+        errs = []
+        for i in (0, 3, 17, 150, 400, 999):
+            ex = os_eri.boys(m, 0.1 * i)
+            errs.append(abs(t[m * 1001 + i] - ex) / ex)
+        worst[m] = max(errs)
+    assert 1e-6 < worst[1] < 7e-6         # F1 nodes: ~6.9e-6 relative
+    assert worst[2] < 1e-8 and worst[3] < 1e-10
+    assert max(worst[m] for m in (6, 10, 14)) < 1e-13
+
+
+def test_boys_taylor_step_matches_formula():
+    t = O.boys_table()
+    for m, x in ((0, 0.123), (2, 7.77), (4, 99.95), (3, 0.05)):
+        k = math.floor(x / 0.1)
+        xk = k * 0.1
+        dx = x - xk
+        f = [t[(m + j) * 1001 + k] for j in range(6)]
+        ref = f[0] - f[1] * dx + 0.5 * f[2] * dx ** 2 - f[3] * dx ** 3 / 6 + f[4] * dx ** 4 / 24 - f[5] * dx ** 5 / 120
+        assert abs(O.boys_taylor(m, x) - ref) < 1e-15
+    # above x_k = 100 the asymptotic formula at the node is expanded
+    x = 150.04
+    xk = math.floor(x / 0.1) * 0.1
+    a = [O.lib().ob_boys_asympt(float(j), xk) for j in range(6)]
+    dx = x - xk
+    ref = sum((-dx) ** j / math.factorial(j) * a[j] for j in range(6))
+    assert abs(O.boys_taylor(0, x) - ref) < 1e-15 * abs(ref) + 1e-18
+
+
+def test_boys_exact_against_mamedov():
+    rows = json.load(open(os.path.join(GOLD, "mamedov.json")))
+    for m, x, eq3, eq6, ours in rows:
+        if m == 20 and x == 30:
+            eq3 = 1.37585444267909e-13   # the comment table prints e-3 (typo); Eq(6) agrees
+        v = O.boys_exact(int(m), x)
+        assert abs(v - eq3) <= 1e-12 * eq3, (m, x, v, eq3)
+
+
+def test_boys_exact_downward_recursion():
+    for T in (0.01, 1.3, 12.0, 45.0):
+        for m in range(0, 12):
+            lhs = O.boys_exact(m, T)
+            rhs = (2 * T * O.boys_exact(m + 1, T) + math.exp(-T)) / (2 * m + 1)
+            assert abs(lhs - rhs) <= 1e-14 * lhs
+
+
+def _prim_mol(centers, ls, exps):
+    atoms = []
+    for c, l, e in zip(centers, ls, exps):
+        ft = {(0, 0, 0): "S", (1, 0, 0): "Px", (0, 1, 0): "Py", (0, 0, 1): "Pz", (2, 0, 0): "Dxx",
+              (1, 1, 0): "Dxy", (1, 0, 1): "Dxz", (0, 2, 0): "Dyy", (0, 1, 1): "Dyz", (0, 0, 2): "Dzz"}[tuple(l)]
+        atoms.append(("H", tuple(c), [(ft, [(1.0, e)])]))
+    return basis.Molecule(atoms)
+
+
+def test_ssss_closed_form():
+    A, B, C, D = (0, 0, 0), (0.3, -0.2, 0.9), (1.5, 0.4, -0.7), (-0.6, 1.1, 0.2)
+    a, b, c, d = 0.8, 1.7, 0.45, 3.1
+    mol = _prim_mol([A, B, C, D], [(0, 0, 0)] * 4, [a, b, c, d])
+    v, _ = O.eri(mol, 0, 1, 2, 3, exact=True, screen=False)
+    p, q = a + b, c + d
+    P = np.array([(a * A[i] + b * B[i]) / p for i in range(3)])
+    Q = np.array([(c * C[i] + d * D[i]) / q for i in range(3)])
+    T = p * q / (p + q) * np.sum((P - Q) ** 2)
+    ref = 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * math.exp(
+        -a * b / p * np.sum((np.array(A) - B) ** 2) - c * d / q * np.sum((np.array(C) - D) ** 2)) * O.boys_exact(0, T)
+    assert abs(v - ref) < 1e-14
+
+
+@pytest.mark.parametrize("ls", [
+    ((1, 0, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0)),
+    ((0, 1, 0), (0, 0, 1), (1, 0, 0), (0, 0, 0)),
+    ((1, 0, 0), (1, 0, 0), (0, 1, 0), (0, 1, 0)),
+    ((2, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 0)),
+    ((1, 1, 0), (0, 0, 2), (0, 1, 1), (2, 0, 0)),
+])
+def test_md_oracle_vs_independent_obara_saika(ls):
+    import os_eri
+    A, B, C, D = (0.1, 0, 0), (0.3, -0.2, 0.9), (1.5, 0.4, -0.7), (-0.6, 1.1, 0.2)
+    ex = (0.8, 1.7, 0.45, 3.1)
+    mol = _prim_mol([A, B, C, D], ls, ex)
+    v, _ = O.eri(mol, 0, 1, 2, 3, exact=True, screen=False)
+    ref = os_eri.prim_eri(A, ls[0], ex[0], B, ls[1], ex[1], C, ls[2], ex[2], D, ls[3], ex[3])
+    assert abs(v - ref) < 1e-13 * max(1.0, abs(ref))
+
+
+def test_eri_permutation_symmetry_and_translation():
+    mol = systems.water("sto-3g")
+    flat, _ = O.eri_flat(mol, exact=True, screen=False)
+    n = mol.nbf
+    T = O.eri_tensor(flat, n)
+    # flat lookups are symmetric by construction; check against direct evaluation
+    for (i, j, k, l) in ((0, 1, 2, 3), (2, 5, 6, 1), (4, 4, 3, 0)):
+        v, _ = O.eri(mol, i, j, k, l, exact=True, screen=False)
+        for perm in ((i, j, k, l), (j, i, k, l), (i, j, l, k), (k, l, i, j), (l, k, j, i)):
+            w, _ = O.eri(mol, *perm, exact=True, screen=False)
+            assert abs(w - v) < 1e-14
+        assert abs(T[i, j, k, l] - v) < 1e-14
+    shifted = mol.with_coords(mol.R + np.array([0.7, -1.3, 2.1]))
+    # the CGF centres move with the atoms
+    f2, _ = O.eri_flat(systems.build_molecule(["O", "H", "H"], mol.R + np.array([0.7, -1.3, 2.1]), "sto-3g"),
+                       exact=True, screen=False)
+    assert np.max(np.abs(f2 - flat)) < 1e-12
+    del shifted
+
+
+def test_p_block_rotation_covariance():
+    """(p_i s|s s) transforms as a vector under rotation of all centres."""
+    rng = np.random.default_rng(3)
+    Rm = systems.quat_matrix(rng.standard_normal(4))
+    cen = np.array([[0.1, 0.2, 0.3], [1.0, -0.4, 0.5], [-0.7, 0.9, 0.1]])
+    ex = (0.9, 0.6, 1.3)
+
+    def block(cs):
+        vals = []
+        for l in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+            mol = _prim_mol([cs[0], cs[1], cs[2], cs[2]], [l, (0, 0, 0), (0, 0, 0), (0, 0, 0)],
+                            [ex[0], ex[1], ex[2], ex[2]])
+            vals.append(O.eri(mol, 0, 1, 2, 3, exact=True, screen=False)[0])
+        return np.array(vals)
+
+    v0 = block(cen)
+    v1 = block(cen @ Rm.T)
+    np.testing.assert_allclose(v1, Rm @ v0, atol=1e-13)
+
+
+def test_flat_layout_literal_vs_enumeration():
+    for n in range(1, 11):
+        pos = 0
+        for i in range(n):
+            for j in range(i, n):
+                for k in range(i, n):
+                    for l in range(k, n):
+                        if O.sort_keys(i, j, k, l) != [i, j, k, l]:
+                            continue
+                        assert O.flat_rank(n, i, j, k, l) == pos
+                        assert O.four_index_ref(n, i, j, k, l) == pos
+                        pos += 1
+        assert pos == (n * (n + 1) // 2) * (n * (n + 1) // 2 + 1) // 2
+
+
+def test_gmatrix_against_dense_contraction():
+    mol = systems.water("sto-3g")
+    n = mol.nbf
+    flat, _ = O.eri_flat(mol)
+    T = O.eri_tensor(flat, n)
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((n, n))
+    Df = A + A.T
+    G = O.gmatrix(n, flat, O.full_to_flat(Df))
+    Gref = np.einsum("abkl,lk->ab", T, Df) - 0.5 * np.einsum("alkb,lk->ab", T, Df)
+    np.testing.assert_allclose(O.flat_to_full(G, n), Gref, atol=1e-12)
+
+
+def test_reference_screen_is_active_for_water():
+    mol = systems.water("sto-3g")
+    flat, kept = O.eri_flat(mol)
+    flat_ns, kept_ns = O.eri_flat(mol, screen=False)
+    assert kept_ns == 7 * 0 + 32886            # 406 quartets x 81 primitive quartets
+    assert kept == 32274                       # primitive quartets passing !(q q' < 1e-8)
+    d = np.max(np.abs(flat - flat_ns))
+    assert 1e-8 < d < 1e-6
+
+
+def test_textbook_h2_and_heh():
+    """Szabo & Ostlund, STO-3G: H2 (R=1.4, zeta=1.24) and HeH+ (R=1.4632)."""
+    h2 = systems.h2_szabo()
+    S = O.overlap(h2)
+    assert abs(S[1] - 0.6593) < 5e-5
+    flat, _ = O.eri_flat(h2, exact=True, screen=False)
+    g = lambda *ix: O.lib().ob_flat_get(2, O._dp(flat), *ix)
+    assert abs(g(0, 0, 0, 0) - 0.7746) < 5e-5
+    assert abs(g(0, 0, 1, 1) - 0.5697) < 5e-5
+    assert abs(g(1, 0, 1, 0) - 0.2970) < 5e-5
+    assert abs(g(1, 0, 0, 0) - 0.4441) < 5e-5
+    r = O.scf(h2, lambda D: O.gmatrix(2, flat, D), O.hcore(h2, exact=True), S)
+    assert abs(r["energy"] - (-1.1167)) < 5e-5
+    heh = systems.heh_szabo()
+    fl2, _ = O.eri_flat(heh, exact=True, screen=False)
+    r2 = O.scf(heh, lambda D: O.gmatrix(2, fl2, D), O.hcore(heh, exact=True), O.overlap(heh))
+    assert abs(r2["energy"] - (-2.860662)) < 5e-6
+
+
+def test_water_sto3g_energy_and_guess_independence():
+    mol = systems.water("sto-3g")
+    flat, _ = O.eri_flat(mol)
+    S, H = O.overlap(mol), O.hcore(mol)
+    g = lambda D: O.gmatrix(mol.nbf, flat, D)
+    e1 = O.scf(mol, g, H, S)["energy"]
+    e2 = O.scf(mol, g, H, S, guess="core")["energy"]
+    assert abs(e1 - e2) < 1e-9
+    assert -75.0 < e1 < -74.9
+
+
+def test_gauss_elem_reference_example():
+    # GaussElimination.hs:115-119: q1 x = v1 has the solution (1, -2, 3)
+    q1 = [[4.0, -2.0, 1.0], [-2.0, 4.0, -2.0], [1.0, -2.0, 4.0]]
+    x = O.gauss_elem(q1, [11.0, -16.0, 17.0])
+    np.testing.assert_allclose(x, [1, -2, 3], atol=1e-14)
