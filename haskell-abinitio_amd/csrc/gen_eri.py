@@ -96,7 +96,7 @@ def gen_class(la, lb, lc, ld):
     name = "eri_%d%d%d%d" % (la, lb, lc, ld)
     out = []
     out.append("// (%s%s|%s%s): %d integrals, L=%d" % ("spdf"[la], "spdf"[lb], "spdf"[lc], "spdf"[ld], nbra * nket, L))
-    out.append("template <typename T, int BOYS>")
+    out.append("template <typename T, int BOYS, int MODE>")
     out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp," % name)
     out.append("    const double* __restrict__ qcomp, const BoysTables& bt, T* __restrict__ V, double qtau)")
     out.append("{")
@@ -113,14 +113,28 @@ def gen_class(la, lb, lc, ld):
     em.raw("    const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
     em.raw("    const int bqoff = (int)pb[PP_QOFF];")
     em.raw("    const T boo2p = T(pb[PP_OO2P]);")
+    if nket > 1 or nbra > 1:
+        em.raw("    double bqv[%d];" % nbra)
+        em.raw("    if constexpr (MODE == 1) {")
+        em.raw("#pragma unroll")
+        em.raw("      for (int z = 0; z < %d; ++z) bqv[z] = qcomp[bqoff + z];" % nbra)
+        em.raw("    }")
     Eb = [hermite_E(em, la, lb, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
     out.extend(em.lines)
     out.append("    for (int ik = 0; ik < nkp; ++ik) {")
     out.append("      const double* pk = ket + (size_t)ik * PP_STRIDE;")
     out.append("      const double kqlo = pk[PP_QLO], kqhi = pk[PP_QHI];")
-    out.append("      // reference screen: keep iff !(q_ab*q_cd < 1e-8), per component")
-    out.append("      if (qtau > 0.0 && bqhi * kqhi < qtau) continue;   // every component screened")
-    out.append("      const bool allin = !(qtau > 0.0) || (bqlo * kqlo >= qtau);")
+    out.append("      // reference screen: component (ab|cd) of this primitive quartet is kept iff")
+    out.append("      // !(q_ab*q_cd < tau).  MODE 0: no screen; MODE 1: the screen, branch-free.")
+    out.append("      if (MODE == 1 && bqhi * kqhi < qtau) continue;   // every component screened")
+    out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
+    if nket > 1 or nbra > 1:
+        out.append("      double kqv[%d];" % nket)
+        out.append("      if constexpr (MODE == 1) {")
+        out.append("        const int kqoff = (int)pk[PP_QOFF];")
+        out.append("#pragma unroll")
+        out.append("        for (int z = 0; z < %d; ++z) kqv[z] = qcomp[kqoff + z];" % nket)
+        out.append("      }")
     em2 = Emitter()
     em2.n = em.n
     em2.raw("      const T kq = T(pk[PP_P]);")
@@ -235,8 +249,12 @@ def gen_class(la, lb, lc, ld):
                     val = " + ".join(terms_x)
                     vi = bra_idx * nket + ket_idx
                     em2.raw("      { const T v_ = %s;" % val)
-                    em2.raw("        V[%d] += (allin || !(qcomp[bqoff + %d] * qcomp[(int)pk[PP_QOFF] + %d] < qtau)) ? v_ : T(0); }"
-                            % (vi, bra_idx, ket_idx))
+                    if nket > 1 or nbra > 1:
+                        em2.raw("        if constexpr (MODE == 0) V[%d] += v_;" % vi)
+                        em2.raw("        else V[%d] += (allin || !(bqv[%d] * kqv[%d] < qtau)) ? v_ : T(0); }"
+                                % (vi, bra_idx, ket_idx))
+                    else:
+                        em2.raw("        V[%d] += v_; }" % vi)
     out.extend(em2.lines)
     out.append("    }")
     out.append("  }")
@@ -274,14 +292,14 @@ def main():
         body.append('#include "hfk_internal.hpp"')
         body.append("namespace {")
         body.append(src)
-        body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS>")
+        body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>")
         body.append("__device__ __forceinline__ void eri_class(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp,")
         body.append("    const double* __restrict__ qcomp, const BoysTables& bt, T* __restrict__ V, double qtau)")
-        body.append("{ eri_%s<T, BOYS>(bra, nbp, ket, nkp, qcomp, bt, V, qtau); }" % tag)
+        body.append("{ eri_%s<T, BOYS, MODE>(bra, nbp, ket, nkp, qcomp, bt, V, qtau); }" % tag)
         body.append("}  // namespace")
         body.append('#include "hfk_eri_impl.hpp"')
-        body.append("int hfk_dispatch_%s(int op, int prec, int boys, const QuartetArgs& a, hipStream_t s)" % tag)
-        body.append("{ return dispatch<%d, %d, %d, %d>(op, prec, boys, a, s); }" % q)
+        body.append("int hfk_dispatch_%s(int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag)
+        body.append("{ return dispatch<%d, %d, %d, %d>(op, prec, boys, mode, a, s); }" % q)
         path = os.path.join(outdir, "eri_%s.hip" % tag)
         txt = "\n".join(body) + "\n"
         if not os.path.exists(path) or open(path).read() != txt:
@@ -289,11 +307,11 @@ def main():
                 f.write(txt)
     d = ["// Generated by gen_eri.py -- do not edit.", "#include <hip/hip_runtime.h>", '#include "hfk_internal.hpp"']
     for q in qs:
-        d.append("int hfk_dispatch_%d%d%d%d(int, int, int, const QuartetArgs&, hipStream_t);" % q)
-    d.append("int hfk_launch_quartets(int la, int lb, int lc, int ld, int op, int prec, int boys, const QuartetArgs& a, hipStream_t s)")
+        d.append("int hfk_dispatch_%d%d%d%d(int, int, int, int, const QuartetArgs&, hipStream_t);" % q)
+    d.append("int hfk_launch_quartets(int la, int lb, int lc, int ld, int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)")
     d.append("{")
     for q in qs:
-        d.append("    if (la == %d && lb == %d && lc == %d && ld == %d) return hfk_dispatch_%d%d%d%d(op, prec, boys, a, s);" % (q + q))
+        d.append("    if (la == %d && lb == %d && lc == %d && ld == %d) return hfk_dispatch_%d%d%d%d(op, prec, boys, mode, a, s);" % (q + q))
     d.append("    return -2;")
     d.append("}")
     path = os.path.join(outdir, "eri_dispatch.hip")

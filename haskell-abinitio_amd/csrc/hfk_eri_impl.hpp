@@ -1,11 +1,19 @@
 // Shell-quartet kernels: ERI evaluation (generated McMurchie-Davidson code)
-// fused with J/K digestion, flat-array export or Schwarz diagonal.  Included
-// by one generated translation unit per angular-momentum class, which
-// defines HFK_CLASS_HEADER (the generated eri_<class> function) and
-// instantiates hfk_dispatch_<class>.
+// fused with J/K digestion, flat-array export or the Schwarz diagonal.
+// Included by one generated translation unit per angular-momentum class,
+// which first defines eri_class<LA,LB,LC,LD,T,BOYS,MODE> and afterwards
+// instantiates dispatch<LA,LB,LC,LD>.
+//
+// Screening follows the reference exactly (IntegralsEvaluation.hs:305-315):
+// a primitive quartet contributes to a Cartesian component (ab|cd) iff
+// !(q_ab q_cd < tau).  MODE 1 applies that screen (branch-free selects on
+// per-component values loaded once per primitive pair); MODE 0 evaluates
+// unscreened (screen off, or the Schwarz diagonal).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <algorithm>
 
 #include "hfk_internal.hpp"
 
@@ -30,124 +38,25 @@ __device__ __forceinline__ void decode_quartet(const QuartetArgs& a, long long q
     }
 }
 
-template <int LA, int LB, int LC, int LD, typename T, int BOYS>
-__device__ __forceinline__ void eval_quartet(const QuartetArgs& a, int bp, int kp, T* V)
+template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
+__device__ __forceinline__ void eval_quartet(const QuartetArgs& a, int bp, int kp, T* V, int b0 = 0, int b1 = -1)
 {
-    const double* bra = a.pp + (size_t)a.pair_pp0[bp] * PP_STRIDE;
+    const double* bra = a.pp + (size_t)(a.pair_pp0[bp] + b0) * PP_STRIDE;
     const double* ket = a.pp + (size_t)a.pair_pp0[kp] * PP_STRIDE;
-    const int nbp = a.pair_npp[bp], nkp = a.pair_npp[kp];
-    eri_class<LA, LB, LC, LD, T, BOYS>(bra, nbp, ket, nkp, a.qcomp, a.bt, V, a.qtau);
+    const int nbp = (b1 < 0 ? a.pair_npp[bp] : b1) - b0, nkp = a.pair_npp[kp];
+    eri_class<LA, LB, LC, LD, T, BOYS, MODE>(bra, nbp, ket, nkp, a.qcomp, a.bt, V, a.qtau);
 }
 
-template <int LA, int LB, int LC, int LD, typename T>
-__device__ __forceinline__ void digest_jk(const QuartetArgs& a, int bp, int kp, const T* Vt)
+__device__ __forceinline__ bool quartet_selected(const QuartetArgs& a, int bp, int kp)
 {
-    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-    constexpr int NKET = NC * ND;
-    const int A = a.pair_A[bp], B = a.pair_B[bp], C = a.pair_A[kp], Dd = a.pair_B[kp];
-    int ia[NA], ib[NB], ic[NC], id[ND];
-    double sa[NA], sb[NB], sc[NC], sd[ND];
-#pragma unroll
-    for (int x = 0; x < NA; ++x) { ia[x] = a.sh_bf[A * HFK_MAXCOMP + x]; sa[x] = a.sh_scale[A * HFK_MAXCOMP + x]; }
-#pragma unroll
-    for (int x = 0; x < NB; ++x) { ib[x] = a.sh_bf[B * HFK_MAXCOMP + x]; sb[x] = a.sh_scale[B * HFK_MAXCOMP + x]; }
-#pragma unroll
-    for (int x = 0; x < NC; ++x) { ic[x] = a.sh_bf[C * HFK_MAXCOMP + x]; sc[x] = a.sh_scale[C * HFK_MAXCOMP + x]; }
-#pragma unroll
-    for (int x = 0; x < ND; ++x) { id[x] = a.sh_bf[Dd * HFK_MAXCOMP + x]; sd[x] = a.sh_scale[Dd * HFK_MAXCOMP + x]; }
-
-    double f = 1.0;
-    if (A == B) f *= 0.5;
-    if (C == Dd) f *= 0.5;
-    if (bp == kp) f *= 0.5;
-    const int n = a.n;
-    const double* D = a.D;
-    auto dget = [&](int p, int q) -> double { return (p < 0 || q < 0) ? 0.0 : D[(size_t)p * n + q]; };
-
-    double V[NA * NB * NC * ND];
-#pragma unroll
-    for (int x = 0; x < NA; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y)
-#pragma unroll
-            for (int z = 0; z < NC; ++z)
-#pragma unroll
-                for (int w = 0; w < ND; ++w) {
-                    const int k = ((x * NB + y) * NC + z) * ND + w;
-                    V[k] = (double)Vt[k] * (sa[x] * sb[y] * sc[z] * sd[w]);
-                }
-
-    // J'_ab += 2f sum_cd V D_cd ; J'_cd += 2f sum_ab V D_ab
-    double Dcd[NC * ND], Dab[NA * NB];
-#pragma unroll
-    for (int z = 0; z < NC; ++z)
-#pragma unroll
-        for (int w = 0; w < ND; ++w) Dcd[z * ND + w] = dget(ic[z], id[w]);
-#pragma unroll
-    for (int x = 0; x < NA; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y) Dab[x * NB + y] = dget(ia[x], ib[y]);
-#pragma unroll
-    for (int x = 0; x < NA; ++x)
-#pragma unroll
-        for (int y = 0; y < NB; ++y) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < NKET; ++k) s += V[(x * NB + y) * NKET + k] * Dcd[k];
-            if (ia[x] >= 0 && ib[y] >= 0) atomic_add_f64(&a.J[(size_t)ia[x] * n + ib[y]], 2.0 * f * s);
-        }
-#pragma unroll
-    for (int k = 0; k < NKET; ++k) {
-        double s = 0.0;
-#pragma unroll
-        for (int b = 0; b < NA * NB; ++b) s += V[b * NKET + k] * Dab[b];
-        const int z = k / ND, w = k % ND;
-        if (ic[z] >= 0 && id[w] >= 0) atomic_add_f64(&a.J[(size_t)ic[z] * n + id[w]], 2.0 * f * s);
+    if (a.qtau > 0.0 && a.pair_qhi[bp] * a.pair_qhi[kp] < a.qtau) return false;
+    if (a.sch_cut > 0.0 || a.prec_sel != 0) {
+        const double bound = a.pair_sch[bp] * a.pair_sch[kp];
+        if (bound < a.sch_cut) return false;
+        if (a.prec_sel == 1 && !(bound >= a.fp32_thr)) return false;
+        if (a.prec_sel == 2 && !(bound < a.fp32_thr)) return false;
     }
-    // K'_ac += f sum_bd V D_bd ; K'_ad += f sum_bc V D_bc
-    // K'_bc += f sum_ad V D_ad ; K'_bd += f sum_ac V D_ac
-#pragma unroll
-    for (int x = 0; x < NA; ++x) {
-#pragma unroll
-        for (int z = 0; z < NC; ++z) {
-            double s = 0.0;
-#pragma unroll
-            for (int y = 0; y < NB; ++y)
-#pragma unroll
-                for (int w = 0; w < ND; ++w) s += V[((x * NB + y) * NC + z) * ND + w] * dget(ib[y], id[w]);
-            if (ia[x] >= 0 && ic[z] >= 0) atomic_add_f64(&a.K[(size_t)ia[x] * n + ic[z]], f * s);
-        }
-#pragma unroll
-        for (int w = 0; w < ND; ++w) {
-            double s = 0.0;
-#pragma unroll
-            for (int y = 0; y < NB; ++y)
-#pragma unroll
-                for (int z = 0; z < NC; ++z) s += V[((x * NB + y) * NC + z) * ND + w] * dget(ib[y], ic[z]);
-            if (ia[x] >= 0 && id[w] >= 0) atomic_add_f64(&a.K[(size_t)ia[x] * n + id[w]], f * s);
-        }
-    }
-#pragma unroll
-    for (int y = 0; y < NB; ++y) {
-#pragma unroll
-        for (int z = 0; z < NC; ++z) {
-            double s = 0.0;
-#pragma unroll
-            for (int x = 0; x < NA; ++x)
-#pragma unroll
-                for (int w = 0; w < ND; ++w) s += V[((x * NB + y) * NC + z) * ND + w] * dget(ia[x], id[w]);
-            if (ib[y] >= 0 && ic[z] >= 0) atomic_add_f64(&a.K[(size_t)ib[y] * n + ic[z]], f * s);
-        }
-#pragma unroll
-        for (int w = 0; w < ND; ++w) {
-            double s = 0.0;
-#pragma unroll
-            for (int x = 0; x < NA; ++x)
-#pragma unroll
-                for (int z = 0; z < NC; ++z) s += V[((x * NB + y) * NC + z) * ND + w] * dget(ia[x], ic[z]);
-            if (ib[y] >= 0 && id[w] >= 0) atomic_add_f64(&a.K[(size_t)ib[y] * n + id[w]], f * s);
-        }
-    }
+    return true;
 }
 
 // Canonical position of a function quartet in calcIntegrals' flat array
@@ -163,14 +72,13 @@ __device__ __forceinline__ long long flat_pos(const QuartetArgs& a, int i, int j
     long long r = a.flat_base[(size_t)a0 * n + a1];
     if (b0 == a0) return r + (b1 - a1);
     r += n - a1;
-    // sum_{k'=a0+1}^{b0-1} (n - k')
-    const long long cnt = (long long)b0 - 1 - a0;
+    const long long cnt = (long long)b0 - 1 - a0;   // sum_{k'=a0+1}^{b0-1} (n - k')
     r += cnt * n - ((long long)(b0 - 1) * b0 / 2 - (long long)a0 * (a0 + 1) / 2);
     return r + (b1 - b0);
 }
 
-template <int LA, int LB, int LC, int LD, typename T>
-__device__ __forceinline__ void write_flat(const QuartetArgs& a, int bp, int kp, const T* Vt)
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void write_flat(const QuartetArgs& a, int bp, int kp, const double* V)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     const int A = a.pair_A[bp], B = a.pair_B[bp], C = a.pair_A[kp], Dd = a.pair_B[kp];
@@ -192,41 +100,35 @@ __device__ __forceinline__ void write_flat(const QuartetArgs& a, int bp, int kp,
                     const double sl = a.sh_scale[Dd * HFK_MAXCOMP + w];
                     if (i < 0 || j < 0 || k < 0 || l < 0) continue;
                     const int idx = ((x * NB + y) * NC + z) * ND + w;
-                    a.flat[flat_pos(a, i, j, k, l)] = (double)Vt[idx] * (si * sj * sk * sl);
+                    a.flat[flat_pos(a, i, j, k, l)] = V[idx] * (si * sj * sk * sl);
                 }
             }
         }
     }
 }
 
-template <int LA, int LB, int LC, int LD, typename T, int BOYS, int OP>
+// Thread-per-quartet kernel for the flat export and the Schwarz diagonal.
+template <int LA, int LB, int LC, int LD, int BOYS, int OP>
 __global__ void __launch_bounds__(128) k_quartets(QuartetArgs a)
 {
     constexpr int NV = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < a.nquart; q += stride) {
-        long long i, j;
         int bp, kp;
         if (OP == HFK_OP_DIAG) {
             bp = kp = a.bra_list[q];
         } else {
+            long long i, j;
             decode_quartet(a, q, i, j);
             bp = a.bra_list[i];
             kp = a.ket_list[j];
-            if (a.qtau > 0.0 && a.pair_qhi[bp] * a.pair_qhi[kp] < a.qtau) continue;
-            if (a.sch_cut > 0.0 || a.prec_sel != 0) {
-                const double bound = a.pair_sch[bp] * a.pair_sch[kp];
-                if (bound < a.sch_cut) continue;
-                if (a.prec_sel == 1 && !(bound >= a.fp32_thr)) continue;
-                if (a.prec_sel == 2 && !(bound < a.fp32_thr)) continue;
-            }
+            if (!quartet_selected(a, bp, kp)) continue;
         }
-        T V[NV];
-        eval_quartet<LA, LB, LC, LD, T, BOYS>(a, bp, kp, V);
-        if constexpr (OP == HFK_OP_JK) {
-            digest_jk<LA, LB, LC, LD, T>(a, bp, kp, V);
-        } else if constexpr (OP == HFK_OP_FLAT) {
-            write_flat<LA, LB, LC, LD, T>(a, bp, kp, V);
+        double V[NV];
+        if (a.qtau > 0.0) eval_quartet<LA, LB, LC, LD, double, BOYS, 1>(a, bp, kp, V);
+        else eval_quartet<LA, LB, LC, LD, double, BOYS, 0>(a, bp, kp, V);
+        if constexpr (OP == HFK_OP_FLAT) {
+            write_flat<LA, LB, LC, LD>(a, bp, kp, V);
         } else {
             // Schwarz diagonal: max_ab |(ab|ab)| with component scales
             constexpr int NA = ncart(LA), NB = ncart(LB), NBRA = NA * NB;
@@ -238,43 +140,256 @@ __global__ void __launch_bounds__(128) k_quartets(QuartetArgs a)
                 for (int y = 0; y < NB; ++y) {
                     const double s = a.sh_scale[A * HFK_MAXCOMP + x] * a.sh_scale[B * HFK_MAXCOMP + y];
                     const int k = x * NB + y;
-                    m = fmax(m, fabs((double)V[k * NBRA + k]) * s * s);
+                    m = fmax(m, fabs(V[k * NBRA + k]) * s * s);
                 }
             a.diag[bp] = m;
         }
     }
 }
 
-template <int LA, int LB, int LC, int LD, typename T, int BOYS, int OP>
-void launch_one(const QuartetArgs& a, hipStream_t s)
+template <int LA, int LB, int LC, int LD, int BOYS, int OP>
+void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 {
     if (a.nquart <= 0) return;
     const int block = 128;
     long long nb = (a.nquart + block - 1) / block;
-    const long long cap = 256LL * 8 * 16;   // enough waves to fill 256 CUs several times
+    const long long cap = 256LL * 8 * 16;
     if (nb > cap) nb = cap;
-    hipLaunchKernelGGL((k_quartets<LA, LB, LC, LD, T, BOYS, OP>), dim3((unsigned)nb), dim3(block), 0, s, a);
+    hipLaunchKernelGGL((k_quartets<LA, LB, LC, LD, BOYS, OP>), dim3((unsigned)nb), dim3(block), 0, s, a);
 }
 
-template <int LA, int LB, int LC, int LD>
-int dispatch(int op, int prec, int boys, const QuartetArgs& a, hipStream_t s)
+// ---------------------------------------------------------------------
+// Tiled J/K kernel: one 64-lane workgroup per task = (bra shell pair, range of
+// its primitive pairs, run of an explicit ket pair array; kets ordered by
+// primitive-pair count).  Splitting the bra primitive range of heavy pairs
+// evens out the work per lane; the digestion is linear in the integrals, so
+// partial quartets are digested independently.  The bra pair is
+// wave-uniform (its data comes through scalar loads); each lane takes ket
+// pairs of the range.  D, J', K' are (n+1) x (n+1) with a zero row/column
+// n that absent Cartesian components point to, so the digestion has no
+// branches.  Per quartet:
+//   J'_ab += 2f sum_cd V D_cd      lane registers, one wave reduction per task
+//   K'_a. , K'_b.                  rows of the two bra shells staged in LDS
+//                                  (ds_add_f64), flushed once per task
+//   J'_cd += 2f sum_ab V D_ab      one global FP64 atomic per ket function pair
+// ---------------------------------------------------------------------
+constexpr int TILE = 64;
+
+template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
+__global__ void __launch_bounds__(TILE) k_jk_tiles(QuartetArgs a)
 {
-#define HFK_OPS(T, B)                                                                          \
-    switch (op) {                                                                            \
-    case HFK_OP_JK: launch_one<LA, LB, LC, LD, T, B, HFK_OP_JK>(a, s); break;                \
-    case HFK_OP_FLAT: launch_one<LA, LB, LC, LD, T, B, HFK_OP_FLAT>(a, s); break;            \
-    case HFK_OP_DIAG: launch_one<LA, LB, LC, LD, T, B, HFK_OP_DIAG>(a, s); break;            \
-    default: return -1;                                                                      \
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NKET = NC * ND, NBRA = NA * NB;
+    extern __shared__ double krow[];   // [(NA+NB)][ld]
+    const int4 task = a.tasks[blockIdx.x];
+    const int bp = task.x;
+    const int bprim0 = task.w >> 16, bprim1 = task.w & 0xffff;   // bra primitive-pair range
+    const int ld = a.n + 1;
+    const int lane = threadIdx.x;
+    const bool lds = a.lds_rows != 0;
+    if (lds)
+        for (int t = lane; t < (NA + NB) * ld; t += TILE) krow[t] = 0.0;
+    __syncthreads();
+
+    const int A = a.pair_A[bp], B = a.pair_B[bp];
+    int ia[NA], ib[NB];
+    double sa[NA], sb[NB];
+#pragma unroll
+    for (int x = 0; x < NA; ++x) { ia[x] = a.sh_bfp[A * HFK_MAXCOMP + x]; sa[x] = a.sh_scale[A * HFK_MAXCOMP + x]; }
+#pragma unroll
+    for (int x = 0; x < NB; ++x) { ib[x] = a.sh_bfp[B * HFK_MAXCOMP + x]; sb[x] = a.sh_scale[B * HFK_MAXCOMP + x]; }
+    const double* __restrict__ D = a.D;
+    double Dab[NBRA];
+#pragma unroll
+    for (int x = 0; x < NA; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) Dab[x * NB + y] = D[(size_t)ia[x] * ld + ib[y]];
+    double jab[NBRA];
+#pragma unroll
+    for (int k = 0; k < NBRA; ++k) jab[k] = 0.0;
+
+    const int len = task.z - task.y;
+    const int rot = (int)((unsigned)task.x * 37u % (unsigned)(len > 0 ? len : 1));
+    for (int it = lane; it < len; it += TILE) {
+        int jj = it + rot;
+        if (jj >= len) jj -= len;
+        const int kp = a.ket_idx[task.y + jj];
+        if (!quartet_selected(a, bp, kp)) continue;
+        T Vt[NBRA * NKET];
+        eval_quartet<LA, LB, LC, LD, T, BOYS, MODE>(a, bp, kp, Vt, bprim0, bprim1);
+        const int C = a.pair_A[kp], Dd = a.pair_B[kp];
+        int ic[NC], id[ND];
+        double sc[NC], sd[ND];
+#pragma unroll
+        for (int x = 0; x < NC; ++x) { ic[x] = a.sh_bfp[C * HFK_MAXCOMP + x]; sc[x] = a.sh_scale[C * HFK_MAXCOMP + x]; }
+#pragma unroll
+        for (int x = 0; x < ND; ++x) { id[x] = a.sh_bfp[Dd * HFK_MAXCOMP + x]; sd[x] = a.sh_scale[Dd * HFK_MAXCOMP + x]; }
+        double f = 1.0;
+        if (A == B) f *= 0.5;
+        if (C == Dd) f *= 0.5;
+        if (bp == kp) f *= 0.5;
+        double V[NBRA * NKET];
+#pragma unroll
+        for (int x = 0; x < NA; ++x)
+#pragma unroll
+            for (int y = 0; y < NB; ++y)
+#pragma unroll
+                for (int z = 0; z < NC; ++z)
+#pragma unroll
+                    for (int w = 0; w < ND; ++w) {
+                        const int k = ((x * NB + y) * NC + z) * ND + w;
+                        V[k] = (double)Vt[k] * (f * sa[x] * sb[y] * sc[z] * sd[w]);
+                    }
+        double Dcd[NKET];
+#pragma unroll
+        for (int z = 0; z < NC; ++z)
+#pragma unroll
+            for (int w = 0; w < ND; ++w) Dcd[z * ND + w] = D[(size_t)ic[z] * ld + id[w]];
+#pragma unroll
+        for (int b = 0; b < NBRA; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < NKET; ++k) s += V[b * NKET + k] * Dcd[k];
+            jab[b] += s;
+        }
+#pragma unroll
+        for (int k = 0; k < NKET; ++k) {
+            double s = 0.0;
+#pragma unroll
+            for (int b = 0; b < NBRA; ++b) s += V[b * NKET + k] * Dab[b];
+            atomic_add_f64(&a.J[(size_t)ic[k / ND] * ld + id[k % ND]], 2.0 * s);
+        }
+        // density blocks D_bd, D_bc, D_ad, D_ac
+        double Dbd[NB * ND], Dbc[NB * NC], Dad[NA * ND], Dac[NA * NC];
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+#pragma unroll
+            for (int w = 0; w < ND; ++w) Dbd[y * ND + w] = D[(size_t)ib[y] * ld + id[w]];
+#pragma unroll
+            for (int z = 0; z < NC; ++z) Dbc[y * NC + z] = D[(size_t)ib[y] * ld + ic[z]];
+        }
+#pragma unroll
+        for (int x = 0; x < NA; ++x) {
+#pragma unroll
+            for (int w = 0; w < ND; ++w) Dad[x * ND + w] = D[(size_t)ia[x] * ld + id[w]];
+#pragma unroll
+            for (int z = 0; z < NC; ++z) Dac[x * NC + z] = D[(size_t)ia[x] * ld + ic[z]];
+        }
+        double* __restrict__ Kg = a.K;
+#pragma unroll
+        for (int x = 0; x < NA; ++x) {
+#pragma unroll
+            for (int z = 0; z < NC; ++z) {
+                double s = 0.0;
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+#pragma unroll
+                    for (int w = 0; w < ND; ++w) s += V[((x * NB + y) * NC + z) * ND + w] * Dbd[y * ND + w];
+                if (lds) atomic_add_f64(&krow[x * ld + ic[z]], s);
+                else atomic_add_f64(&Kg[(size_t)ia[x] * ld + ic[z]], s);
+            }
+#pragma unroll
+            for (int w = 0; w < ND; ++w) {
+                double s = 0.0;
+#pragma unroll
+                for (int y = 0; y < NB; ++y)
+#pragma unroll
+                    for (int z = 0; z < NC; ++z) s += V[((x * NB + y) * NC + z) * ND + w] * Dbc[y * NC + z];
+                if (lds) atomic_add_f64(&krow[x * ld + id[w]], s);
+                else atomic_add_f64(&Kg[(size_t)ia[x] * ld + id[w]], s);
+            }
+        }
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+#pragma unroll
+            for (int z = 0; z < NC; ++z) {
+                double s = 0.0;
+#pragma unroll
+                for (int x = 0; x < NA; ++x)
+#pragma unroll
+                    for (int w = 0; w < ND; ++w) s += V[((x * NB + y) * NC + z) * ND + w] * Dad[x * ND + w];
+                if (lds) atomic_add_f64(&krow[(NA + y) * ld + ic[z]], s);
+                else atomic_add_f64(&Kg[(size_t)ib[y] * ld + ic[z]], s);
+            }
+#pragma unroll
+            for (int w = 0; w < ND; ++w) {
+                double s = 0.0;
+#pragma unroll
+                for (int x = 0; x < NA; ++x)
+#pragma unroll
+                    for (int z = 0; z < NC; ++z) s += V[((x * NB + y) * NC + z) * ND + w] * Dac[x * NC + z];
+                if (lds) atomic_add_f64(&krow[(NA + y) * ld + id[w]], s);
+                else atomic_add_f64(&Kg[(size_t)ib[y] * ld + id[w]], s);
+            }
+        }
     }
-    if (prec == 0) {
-        if (boys == 0) { HFK_OPS(double, 0) } else { HFK_OPS(double, 1) }
-    } else {
-        if (op != HFK_OP_JK) return -1;
-        if (boys == 0) { launch_one<LA, LB, LC, LD, float, 0, HFK_OP_JK>(a, s); }
-        else { launch_one<LA, LB, LC, LD, float, 1, HFK_OP_JK>(a, s); }
+    // J'_ab: wave reduction, one atomic per bra function pair per task
+#pragma unroll
+    for (int b = 0; b < NBRA; ++b) {
+        double v = jab[b];
+#pragma unroll
+        for (int off = TILE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, TILE);
+        if (lane == 0 && v != 0.0) atomic_add_f64(&a.J[(size_t)ia[b / NB] * ld + ib[b % NB]], 2.0 * v);
     }
-#undef HFK_OPS
-    return 0;
+    // flush the staged K rows
+    if (lds) {
+        __syncthreads();
+        for (int t = lane; t < (NA + NB) * ld; t += TILE) {
+            const double v = krow[t];
+            if (v != 0.0) {
+                const int r = t / ld, col = t - r * ld;
+                const int row = (r < NA) ? ia[r] : ib[r - NA];
+                atomic_add_f64(&a.K[(size_t)row * ld + col], v);
+            }
+        }
+    }
+}
+
+template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
+void launch_tiles(const QuartetArgs& a, hipStream_t s)
+{
+    if (a.ntask <= 0) return;
+    constexpr int NR = ncart(LA) + ncart(LB);
+    const size_t shm = a.lds_rows ? sizeof(double) * NR * (size_t)(a.n + 1) : 0;
+    long long done = 0;
+    while (done < a.ntask) {   // grid.x limit
+        const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);
+        QuartetArgs b = a;
+        b.tasks = a.tasks + done;
+        hipLaunchKernelGGL((k_jk_tiles<LA, LB, LC, LD, T, BOYS, MODE>), dim3((unsigned)cnt), dim3(TILE), shm, s, b);
+        done += cnt;
+    }
+}
+
+// op: HFK_OP_JK (mode 0 main, 1 correction), HFK_OP_FLAT, HFK_OP_DIAG.
+template <int LA, int LB, int LC, int LD>
+int dispatch(int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)
+{
+    if (op == HFK_OP_JK) {
+#define HFK_T(T)                                                                        \
+        if (boys == 0) {                                                                  \
+            if (mode == 0) launch_tiles<LA, LB, LC, LD, T, 0, 0>(a, s);                   \
+            else launch_tiles<LA, LB, LC, LD, T, 0, 1>(a, s);                             \
+        } else {                                                                          \
+            if (mode == 0) launch_tiles<LA, LB, LC, LD, T, 1, 0>(a, s);                   \
+            else launch_tiles<LA, LB, LC, LD, T, 1, 1>(a, s);                             \
+        }
+        if (prec == 0) { HFK_T(double) } else { HFK_T(float) }
+#undef HFK_T
+        return 0;
+    }
+    if (prec != 0) return -1;
+    if (op == HFK_OP_FLAT) {
+        if (boys == 0) launch_quartets_one<LA, LB, LC, LD, 0, HFK_OP_FLAT>(a, s);
+        else launch_quartets_one<LA, LB, LC, LD, 1, HFK_OP_FLAT>(a, s);
+        return 0;
+    }
+    if (op == HFK_OP_DIAG) {
+        if (boys == 0) launch_quartets_one<LA, LB, LC, LD, 0, HFK_OP_DIAG>(a, s);
+        else launch_quartets_one<LA, LB, LC, LD, 1, HFK_OP_DIAG>(a, s);
+        return 0;
+    }
+    return -1;
 }
 
 }  // namespace

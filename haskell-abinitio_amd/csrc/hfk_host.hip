@@ -197,6 +197,7 @@ extern "C" int hfk_create(const hfk_basis* b, const hfk_options* opt, hfk_ctx** 
     if (rc == HFK_OK) rc = hfk_ctx_init_device(c);
     if (rc == HFK_OK) rc = hfk_ctx_build_pairs(c);
     if (rc == HFK_OK) rc = hfk_ctx_schwarz(c);
+    if (rc == HFK_OK) rc = hfk_ctx_plan(c);
     if (rc != HFK_OK) {
         hfk_ctx_free_device(c);
         delete c;
@@ -226,6 +227,7 @@ extern "C" int hfk_set_options(hfk_ctx* c, const hfk_options* opt)
         rc = hfk_ctx_classify_pairs(c);
         if (rc == HFK_OK) rc = hfk_ctx_schwarz(c);
     }
+    if (rc == HFK_OK) rc = hfk_ctx_plan(c);
     return rc;
 }
 
@@ -282,14 +284,19 @@ int hfk_ctx_init_device(hfk_ctx* c)
     HFK_TRY(dev_upload(c, &c->d_sh_prim0, shp0));
     HFK_TRY(dev_upload(c, &c->d_sh_nprim, shnp));
     HFK_TRY(dev_upload(c, &c->d_sh_bf, shbf));
+    std::vector<int> shbfp(shbf);
+    for (int& v : shbfp)
+        if (v < 0) v = c->n;
+    HFK_TRY(dev_upload(c, &c->d_sh_bfp, shbfp));
     HFK_TRY(dev_upload(c, &c->d_sh_scale, shs));
     HFK_TRY(dev_upload(c, &c->d_sh_expo, c->sh_expo));
     HFK_TRY(dev_upload(c, &c->d_sh_coef, c->sh_coef));
     // work matrices
-    const size_t nn = (size_t)c->n * c->n, nf = (size_t)c->n * (c->n + 1) / 2;
-    HFK_TRY(dev_alloc(&c->d_D, nn));
-    HFK_TRY(dev_alloc(&c->d_J, nn));
-    HFK_TRY(dev_alloc(&c->d_K, nn));
+    const size_t nf = (size_t)c->n * (c->n + 1) / 2;
+    const size_t np1 = (size_t)(c->n + 1) * (c->n + 1);   // padded: zero row/column n
+    HFK_TRY(dev_alloc(&c->d_D, np1));
+    HFK_TRY(dev_alloc(&c->d_J, np1));
+    HFK_TRY(dev_alloc(&c->d_K, np1));
     HFK_TRY(dev_alloc(&c->d_flatA, nf));
     HFK_TRY(dev_alloc(&c->d_flatB, nf));
     HFK_TRY(dev_alloc(&c->d_flatC, nf));
@@ -304,6 +311,8 @@ void hfk_ctx_free_device(hfk_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void* p : c->allocs) hipFree(p);
     c->allocs.clear();
+    for (hipEvent_t ev : c->events) hipEventDestroy(ev);
+    c->events.clear();
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->blas) hfk_linalg_destroy(c);
@@ -386,6 +395,13 @@ int hfk_ctx_classify_pairs(hfk_ctx* c)
         lists[pair_class(la, lb)].push_back(p);
         ++kept;
     }
+    // by primitive-pair count, then largest screen value first (the task
+    // lists of hfk_ctx_plan rely on this order)
+    for (int k = 0; k < HFK_NPC; ++k)
+        std::stable_sort(lists[k].begin(), lists[k].end(), [&](int p, int q) {
+            if (c->h_pair_npp[p] != c->h_pair_npp[q]) return c->h_pair_npp[p] < c->h_pair_npp[q];
+            return c->h_pair_qhi[p] > c->h_pair_qhi[q];
+        });
     for (int k = 0; k < HFK_NPC; ++k) {
         if (c->d_class[k]) { dev_free(c, c->d_class[k]); c->d_class[k] = nullptr; }
         c->class_count[k] = (long long)lists[k].size();
@@ -409,6 +425,7 @@ QuartetArgs hfk_ctx_args(hfk_ctx* c)
     a.pair_qhi = c->d_pair_qhi;
     a.pair_sch = c->d_pair_sch;
     a.sh_bf = c->d_sh_bf;
+    a.sh_bfp = c->d_sh_bfp;
     a.sh_scale = c->d_sh_scale;
     a.bt = c->bt;
     a.qtau = (c->opt.screen_mode == HFK_SCREEN_REFERENCE) ? c->opt.screen_tau : 0.0;
@@ -431,7 +448,7 @@ int hfk_ctx_schwarz(hfk_ctx* c)
         a.ket_list = c->d_class[k];
         a.nbra = a.nket = c->class_count[k];
         a.nquart = c->class_count[k];
-        int rc = hfk_launch_quartets(la, lb, la, lb, HFK_OP_DIAG, 0, c->opt.boys_mode, a, c->stream);
+        int rc = hfk_launch_quartets(la, lb, la, lb, HFK_OP_DIAG, 0, c->opt.boys_mode, 0, a, c->stream);
         if (rc) return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", la, lb, la, lb);
     }
     HFK_HIP(hipGetLastError());
@@ -454,52 +471,216 @@ void hfk_pair_class_l(int k, int* la, int* lb)
     *lb = k - a * (a + 1) / 2;
 }
 
-// Launch every class pair (bra class >= ket class) for one operation.
-int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
+// The launch plan: every class pair (bra class >= ket class), split into an
+// FP64 and an FP32 launch when the mixed-precision threshold is set.
+int hfk_ctx_plan(hfk_ctx* c)
 {
-    long long total = 0;
+    for (PlanEntry& e : c->plan) {
+        if (e.d_tasks) dev_free(c, e.d_tasks);
+        if (e.d_kets) dev_free(c, e.d_kets);
+    }
+    c->plan.clear();
+    const bool ref = c->opt.screen_mode == HFK_SCREEN_REFERENCE && c->opt.screen_tau > 0.0;
     for (int kb = 0; kb < HFK_NPC; ++kb) {
         if (c->class_count[kb] == 0) continue;
         for (int kk = 0; kk <= kb; ++kk) {
             if (c->class_count[kk] == 0) continue;
-            int la, lb, lc, ld;
-            hfk_pair_class_l(kb, &la, &lb);
-            hfk_pair_class_l(kk, &lc, &ld);
-            a.bra_list = c->d_class[kb];
-            a.ket_list = c->d_class[kk];
-            a.nbra = c->class_count[kb];
-            a.nket = c->class_count[kk];
-            a.same = (kb == kk);
-            a.nquart = a.same ? a.nbra * (a.nbra + 1) / 2 : a.nbra * a.nket;
-            total += a.nquart;
-            const bool split = (op == HFK_OP_JK) && c->opt.fp32_threshold > 0.0;
-            if (!split) {
-                a.prec_sel = 0;
-                a.fp32_thr = 0.0;
-                int rc = hfk_launch_quartets(la, lb, lc, ld, op, 0, c->opt.boys_mode, a, c->stream);
-                if (rc) return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", la, lb, lc, ld);
-            } else {
-                a.fp32_thr = c->opt.fp32_threshold;
-                a.prec_sel = 1;
-                int rc = hfk_launch_quartets(la, lb, lc, ld, op, 0, c->opt.boys_mode, a, c->stream);
-                a.prec_sel = 2;
-                if (!rc) rc = hfk_launch_quartets(la, lb, lc, ld, op, 1, c->opt.boys_mode, a, c->stream);
-                if (rc) return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", la, lb, lc, ld);
+            for (int prec = 0; prec < 2; ++prec) {
+                if (prec == 1 && !(c->opt.fp32_threshold > 0.0)) continue;
+                PlanEntry e = {};
+                e.kb = kb;
+                e.kk = kk;
+                hfk_pair_class_l(kb, &e.la, &e.lb);
+                hfk_pair_class_l(kk, &e.lc, &e.ld);
+                e.prec = prec;
+                e.prec_sel = (c->opt.fp32_threshold > 0.0) ? prec + 1 : 0;
+                const long long nb = c->class_count[kb], nk = c->class_count[kk];
+                e.nquart = (kb == kk) ? nb * (nb + 1) / 2 : nb * nk;
+                e.mode = ref ? 1 : 0;   // 1: kernels apply the reference screen
+                c->plan.push_back(e);
             }
         }
     }
+    // counts (shell, primitive and canonical function quartets) per main launch
+    unsigned long long* d_cnt = nullptr;
+    const size_t np = c->plan.size();
+    HFK_TRY(dev_alloc(&d_cnt, 3 * std::max<size_t>(np, 1)));
+    hfk_launch_fill((double*)d_cnt, 0.0, (long long)(3 * std::max<size_t>(np, 1)), c->stream);
+    QuartetArgs a = hfk_ctx_args(c);
+    a.sch_cut = c->opt.schwarz_cut;
+    for (size_t i = 0; i < np; ++i) {
+        PlanEntry& e = c->plan[i];
+        hfk_plan_args(c, e, a);
+        hfk_launch_count(a, ncart(e.la), ncart(e.lb), ncart(e.lc), ncart(e.ld), d_cnt + 3 * i, c->stream);
+    }
+    HFK_HIP(hipGetLastError());
+    std::vector<unsigned long long> h(3 * std::max<size_t>(np, 1));
+    HFK_HIP(hipMemcpyAsync(h.data(), d_cnt, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HFK_HIP(hipStreamSynchronize(c->stream));
+    dev_free(c, d_cnt);
+    long long fq = 0;
+    for (size_t i = 0; i < np; ++i) {
+        c->plan[i].shell_q = (long long)h[3 * i];
+        c->plan[i].prim_q = (long long)h[3 * i + 1];
+        c->plan[i].func_q = (long long)h[3 * i + 2];
+        fq += c->plan[i].func_q;
+    }
+    c->stats.func_quartets = fq;
+    // Task lists of the tiled J/K kernel.  Class lists are ordered by
+    // primitive-pair count, then by descending screen value.  For each bra
+    // pair, the kets surviving the pair screen form a prefix of each
+    // primitive-pair-count run; those prefixes are concatenated into an
+    // explicit ket array and cut into tasks of `chunk` kets, so the lanes of
+    // a workgroup are busy and mostly do equal work.
+    const int chunk = 64 * std::max(1, std::min(16, c->n / 64));
+    for (PlanEntry& e : c->plan) {
+        const std::vector<int>& bl = c->h_class[e.kb];
+        const std::vector<int>& kl = c->h_class[e.kk];
+        std::vector<double> kq(kl.size());
+        for (size_t j = 0; j < kl.size(); ++j) kq[j] = c->h_pair_qhi[kl[j]];
+        std::vector<std::pair<long long, long long>> segs;   // runs of equal primitive-pair count
+        for (size_t j = 0; j < kl.size();) {
+            size_t k = j + 1;
+            while (k < kl.size() && c->h_pair_npp[kl[k]] == c->h_pair_npp[kl[j]]) ++k;
+            segs.push_back({(long long)j, (long long)k});
+            j = k;
+        }
+        std::vector<int4> tasks;
+        std::vector<int> kets;
+        for (size_t i = 0; i < bl.size(); ++i) {
+            const double qi = c->h_pair_qhi[bl[i]];
+            // kets need qhi_j >= tau / qhi_i; relaxed so that rounding never drops a survivor
+            const double need = (!ref || std::isinf(qi)) ? -1.0
+                              : (qi > 0.0 ? c->opt.screen_tau / qi * (1.0 - 1e-12) : INFINITY);
+            const int nbp = c->h_pair_npp[bl[i]];
+            for (const auto& sg : segs) {
+                long long hi_end = sg.second;
+                if (e.kb == e.kk) hi_end = std::min<long long>(hi_end, (long long)i + 1);
+                if (hi_end <= sg.first) continue;
+                long long lo = sg.first, hi = hi_end;
+                while (lo < hi) {
+                    const long long mid = (lo + hi) / 2;
+                    if (kq[mid] >= need) lo = mid + 1; else hi = mid;
+                }
+                if (lo <= sg.first) continue;
+                const long long k0 = (long long)kets.size();
+                for (long long j = sg.first; j < lo; ++j) kets.push_back(kl[j]);
+                const long long k1 = (long long)kets.size();
+                // bra primitive chunks: a lane's share of a task is bounded to about
+                // 4096 McMurchie-Davidson operations (primitive quartets x cost)
+                const int nkp = c->h_pair_npp[kl[sg.first]];
+                const int L = e.la + e.lb + e.lc + e.ld;
+                const int per_lane = std::max(1, 64 >> (L > 4 ? 4 : L));   // 64 (ss|ss) ... 4 (pp|pp)
+                const int bstep = std::max(1, std::min(nbp, (per_lane + nkp - 1) / nkp));
+                for (int b0 = 0; b0 < nbp; b0 += bstep) {
+                    const int b1 = std::min(nbp, b0 + bstep);
+                    for (long long j0 = k0; j0 < k1; j0 += chunk)
+                        tasks.push_back(make_int4(bl[i], (int)j0, (int)std::min<long long>(k1, j0 + chunk),
+                                                  (b0 << 16) | b1));
+                }
+            }
+        }
+        if ((long long)kets.size() > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "task list too long");
+        e.ntask = (long long)tasks.size();
+        e.d_tasks = nullptr;
+        e.d_kets = nullptr;
+        if (!tasks.empty()) {
+            HFK_TRY(dev_upload(c, &e.d_tasks, tasks));
+            HFK_TRY(dev_upload(c, &e.d_kets, kets));
+        }
+    }
+    // event pool for per-launch timing
+    while (c->events.size() < 2 * np + 2) {
+        hipEvent_t ev;
+        HFK_HIP(hipEventCreate(&ev));
+        c->events.push_back(ev);
+    }
+    return HFK_OK;
+}
+
+void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
+{
+    a.tasks = e.d_tasks;
+    a.ket_idx = e.d_kets;
+    a.ntask = e.ntask;
+    {
+        const int nr = (e.la + 1) * (e.la + 2) / 2 + (e.lb + 1) * (e.lb + 2) / 2;
+        a.lds_rows = ((size_t)nr * c->n * sizeof(double) <= 64 * 1024) ? 1 : 0;
+    }
+    a.bra_list = c->d_class[e.kb];
+    a.ket_list = c->d_class[e.kk];
+    a.nbra = c->class_count[e.kb];
+    a.nket = c->class_count[e.kk];
+    a.same = (e.kb == e.kk);
+    a.nquart = e.nquart;
+    a.prec_sel = e.prec_sel;
+    a.fp32_thr = c->opt.fp32_threshold;
+}
+
+// Run the plan for one operation (J/K or flat export); per-launch HIP events.
+int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
+{
+    long long total = 0;
+    const size_t np = c->plan.size();
+    for (size_t i = 0; i < np; ++i) {
+        const PlanEntry& e = c->plan[i];
+        if (op != HFK_OP_JK && e.prec == 1) continue;   // exports are FP64
+        hfk_plan_args(c, e, a);
+        if (op != HFK_OP_JK) a.prec_sel = 0;
+        total += e.nquart;
+        HFK_HIP(hipEventRecord(c->events[2 * i], c->stream));
+        int rc = hfk_launch_quartets(e.la, e.lb, e.lc, e.ld, op, e.prec, c->opt.boys_mode, op == HFK_OP_JK ? e.mode : 0, a, c->stream);
+        if (rc) return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", e.la, e.lb, e.lc, e.ld);
+        HFK_HIP(hipEventRecord(c->events[2 * i + 1], c->stream));
+    }
     HFK_HIP(hipGetLastError());
     c->stats.quartets_launched = total;
+    c->timed_op = op;
+    return HFK_OK;
+}
+
+// After a synchronised run: per-launch milliseconds into the plan.
+int hfk_ctx_collect_times(hfk_ctx* c)
+{
+    for (size_t i = 0; i < c->plan.size(); ++i) {
+        PlanEntry& e = c->plan[i];
+        if (c->timed_op != HFK_OP_JK && e.prec == 1) { e.ms = 0.0; continue; }
+        float ms = 0.f;
+        HFK_HIP(hipEventElapsedTime(&ms, c->events[2 * i], c->events[2 * i + 1]));
+        e.ms = ms;
+    }
+    return HFK_OK;
+}
+
+extern "C" int hfk_get_class_stats(hfk_ctx* c, hfk_class_stat* out, int32_t cap, int32_t* n)
+{
+    if (!c || !n) return hfk_set_error(HFK_EINVAL, "hfk_get_class_stats: null argument");
+    std::lock_guard<std::mutex> g(c->mu);
+    *n = (int32_t)c->plan.size();
+    for (size_t i = 0; i < c->plan.size() && (int32_t)i < cap && out; ++i) {
+        const PlanEntry& e = c->plan[i];
+        hfk_class_stat& s = out[i];
+        s.la = e.la;
+        s.lb = e.lb;
+        s.lc = e.lc;
+        s.ld = e.ld;
+        s.prec = e.prec;
+        s.pad = 0;
+        s.shell_quartets = e.shell_q;
+        s.prim_quartets = e.prim_q;
+        s.func_quartets = e.func_q;
+        s.ms = e.ms;
+    }
     return HFK_OK;
 }
 
 // Device Fock build: G (or H+G) flat from a flat density, all on device.
 int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat)
 {
-    const size_t nn = (size_t)c->n * c->n;
-    hfk_launch_flat_to_full(c->n, D_dev_flat, c->d_D, c->stream);
-    hfk_launch_fill(c->d_J, 0.0, (long long)nn, c->stream);
-    hfk_launch_fill(c->d_K, 0.0, (long long)nn, c->stream);
+    const size_t np1 = (size_t)(c->n + 1) * (c->n + 1);
+    hfk_launch_flat_to_padded(c->n, D_dev_flat, c->d_D, c->stream);
+    hfk_launch_fill(c->d_J, 0.0, (long long)np1, c->stream);
+    hfk_launch_fill(c->d_K, 0.0, (long long)np1, c->stream);
     QuartetArgs a = hfk_ctx_args(c);
     a.sch_cut = c->opt.schwarz_cut;
     a.D = c->d_D;
@@ -514,6 +695,7 @@ int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_fla
     float ms = 0.f;
     HFK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->stats.last_eri_ms = ms;
+    HFK_TRY(hfk_ctx_collect_times(c));
     return HFK_OK;
 }
 

@@ -35,6 +35,20 @@ struct HostShell {
     double scale[HFK_MAXCOMP];  // coefficient of the component / base coefficient
 };
 
+struct PlanEntry {
+    int kb, kk;                 // bra / ket pair class
+    int la, lb, lc, ld;
+    int prec;                   // 0 FP64, 1 FP32
+    int prec_sel;               // QuartetArgs.prec_sel for this launch
+    long long nquart;           // launched quartet slots
+    long long shell_q, prim_q, func_q;   // counted survivors
+    double ms;                  // last timed duration
+    int mode;                   // 0 main pass, 1 screen-correction pass (shares the main tasks)
+    int4* d_tasks;              // tiled J/K tasks {bra pair, begin, end, 0} into d_kets
+    int* d_kets;                // explicit ket pair indices
+    long long ntask;
+};
+
 struct hfk_ctx {
     std::mutex mu;
     int n = 0, natom = 0;
@@ -64,12 +78,15 @@ struct hfk_ctx {
     double *d_center = nullptr, *d_coef = nullptr, *d_expo = nullptr, *d_Z = nullptr, *d_R = nullptr;
     int *d_lxyz = nullptr, *d_prim_off = nullptr;
     double *d_sh_center = nullptr, *d_sh_scale = nullptr, *d_sh_expo = nullptr, *d_sh_coef = nullptr;
-    int *d_sh_L = nullptr, *d_sh_prim0 = nullptr, *d_sh_nprim = nullptr, *d_sh_bf = nullptr;
+    int *d_sh_L = nullptr, *d_sh_prim0 = nullptr, *d_sh_nprim = nullptr, *d_sh_bf = nullptr, *d_sh_bfp = nullptr;
     int *d_pair_A = nullptr, *d_pair_B = nullptr, *d_pair_pp0 = nullptr, *d_pair_npp = nullptr;
     double *d_pp = nullptr, *d_qcomp = nullptr, *d_pair_qhi = nullptr, *d_pair_sch = nullptr, *d_pair_diag = nullptr;
     double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr;
     double *d_flatA = nullptr, *d_flatB = nullptr, *d_flatC = nullptr;
     hfk_stats stats = {};
+    std::vector<PlanEntry> plan;
+    std::vector<hipEvent_t> events;
+    int timed_op = 0;
 };
 
 template <typename T>
@@ -125,6 +142,9 @@ int hfk_ctx_classify_pairs(hfk_ctx* c);
 void hfk_pair_class_l(int k, int* la, int* lb);
 QuartetArgs hfk_ctx_args(hfk_ctx* c);
 int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op);
+int hfk_ctx_plan(hfk_ctx* c);
+void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a);
+int hfk_ctx_collect_times(hfk_ctx* c);
 int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat);
 int hfk_ctx_overlap_dev(hfk_ctx* c, double* S_dev);
 int hfk_ctx_hcore_dev(hfk_ctx* c, double* H_dev);

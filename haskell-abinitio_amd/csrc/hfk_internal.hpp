@@ -26,6 +26,7 @@ struct QuartetArgs {
     int same;                   // bra class == ket class: only j <= i
     long long nquart;
     const int* sh_bf;           // [nshell][HFK_MAXCOMP] basis-function index or -1
+    const int* sh_bfp;          // same with -1 replaced by n (the zero row of padded D/J/K)
     const double* sh_scale;     // [nshell][HFK_MAXCOMP] component coefficient scale
     BoysTables bt;
     double qtau;                // reference primitive screen threshold (0 = off)
@@ -33,19 +34,25 @@ struct QuartetArgs {
     double fp32_thr;            // precision split threshold on the Schwarz bound
     int prec_sel;               // 0: all quartets; 1: bound >= fp32_thr; 2: bound < fp32_thr
     int n;                      // number of basis functions
-    const double* D;            // [n][n] density (symmetric)
-    double* J;                  // [n][n] accumulator J' (symmetrised on output)
-    double* K;                  // [n][n] accumulator K'
+    const double* D;            // [n+1][n+1] density (symmetric), row/column n zero
+    double* J;                  // [n+1][n+1] accumulator J' (symmetrised on output)
+    double* K;                  // [n+1][n+1] accumulator K'
     double* flat;               // flat ERI output (calcIntegrals order)
     const long long* flat_base; // [n][n] rank of (i,j,i,j)
     double* diag;               // per-pair max |(ab|ab)| output (Schwarz setup)
+    // task list of the tiled J/K kernel: {bra pair, begin, end, 0} into ket_idx
+    const int4* tasks;
+    const int* ket_idx;         // explicit ket pair indices of the tasks
+    long long ntask;
+    int lds_rows;               // K rows staged in LDS (0: global atomics)
 };
 
 enum { HFK_OP_JK = 0, HFK_OP_FLAT = 1, HFK_OP_DIAG = 2 };
 
 // Launch one class kernel.  classes are (la,lb,lc,ld) with la>=lb, lc>=ld,
 // (la,lb) >= (lc,ld).  prec: 0 = FP64, 1 = FP32.  boys: 0 reference, 1 accurate.
-int hfk_launch_quartets(int la, int lb, int lc, int ld, int op, int prec, int boys,
+// mode (J/K only): 0 main pass, 1 screen-correction pass.
+int hfk_launch_quartets(int la, int lb, int lc, int ld, int op, int prec, int boys, int mode,
                         const QuartetArgs& a, hipStream_t s);
 
 // Setup kernels (hfk_setup.hip)
@@ -65,8 +72,15 @@ void hfk_launch_pair_setup(const ShellDev& sh, int npair, const int* pair_A, con
 void hfk_launch_boys_tables(const double* glx, const double* glw, double* ref64, float* ref32,
                             double* acc64, float* acc32, hipStream_t s);
 void hfk_launch_fill(double* p, double v, long long n, hipStream_t s);
+void hfk_launch_count(const QuartetArgs& a, int na, int nb, int nc, int nd, unsigned long long* out, hipStream_t s);
+// Quartets of the task list that hold a partially screened primitive quartet:
+// out==nullptr counts them into *count; otherwise appends {bra, ket} pairs.
+void hfk_launch_find_mixed(const QuartetArgs& a, unsigned long long* count, int2* out, hipStream_t s);
+// J, K padded (leading dimension n+1)
 void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const double* H, double* G_flat,
                              double* F_full, hipStream_t s);
+// flat -> padded (n+1)x(n+1) full, row/column n zero
+void hfk_launch_flat_to_padded(int n, const double* flat, double* full, hipStream_t s);
 void hfk_launch_flat_to_full(int n, const double* flat, double* full, hipStream_t s);
 void hfk_launch_full_to_flat(int n, const double* full, double* flat, hipStream_t s);
 

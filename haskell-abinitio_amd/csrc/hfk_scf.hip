@@ -186,7 +186,7 @@ bool gauss_elem(std::vector<double> A, std::vector<double> b, int m, std::vector
 
 extern "C" void hfk_default_scf_options(hfk_scf_options* o)
 {
-    o->guess = HFK_GUESS_INTENDED;
+    o->guess = HFK_GUESS_CORE;
     o->max_steps = 30;      // HartreeFock.hs:87
     o->diis_size = 5;       // HartreeFock.hs:78
     o->diis_start = 0.1;    // DIIS.hs:58

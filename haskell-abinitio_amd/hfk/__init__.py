@@ -166,13 +166,17 @@ class Context:
         check(_lib.lib().hfk_hcore(self._h, _ptr(H)))
         return H
 
-    def scfHF(self, charge=None, guess=HFK_GUESS_INTENDED, max_steps=30, diis_size=5, converge=1e-7):
+    def scfHF(self, charge=None, guess=HFK_GUESS_CORE, max_steps=30, diis_size=5, converge=1e-7, diis_start=0.1):
+        """scfHF.  Defaults are the reference's (30 steps, 5 DIIS vectors,
+        DIIS switched on once max|e| < 0.1, converged at max|e| < 1e-7);
+        diis_start=float('inf') switches DIIS on from the second step."""
         o = _lib.HfkScfOptions()
         _lib.lib().hfk_default_scf_options(C.byref(o))
         o.guess = guess
         o.max_steps = max_steps
         o.diis_size = diis_size
         o.converge = converge
+        o.diis_start = diis_start
         o.charge = self.mol.charge if charge is None else float(charge)
         r = _lib.HfkScfResult()
         n = self.n
@@ -189,6 +193,15 @@ class Context:
         g = np.zeros((self.mol.natom, 3))
         check(_lib.lib().hfk_gradient(self._h, _ptr(D), _ptr(W), _ptr(g)))
         return g
+
+    def class_stats(self):
+        """Per-launch records of the last Fock build: class, precision,
+        surviving shell/primitive/function quartets, kernel ms."""
+        n = C.c_int32(0)
+        _lib.lib().hfk_get_class_stats(self._h, None, 0, C.byref(n))
+        arr = (_lib.HfkClassStat * max(1, n.value))()
+        check(_lib.lib().hfk_get_class_stats(self._h, arr, n.value, C.byref(n)))
+        return [{k: getattr(arr[i], k) for k, _ in arr[i]._fields_ if k != "pad"} for i in range(n.value)]
 
     def stats(self):
         s = _lib.HfkStats()

@@ -18,7 +18,7 @@ EXPORTS = (
     "hfk_default_options", "hfk_create", "hfk_destroy", "hfk_set_options", "hfk_eri_flat",
     "hfk_fock_G", "hfk_fock", "hfk_fock_dev", "hfk_density", "hfk_symm_ortho", "hfk_unitary",
     "hfk_overlap", "hfk_hcore", "hfk_default_scf_options", "hfk_scf", "hfk_gradient",
-    "hfk_get_stats", "hfk_last_error",
+    "hfk_get_stats", "hfk_get_class_stats", "hfk_last_error",
 )
 
 
@@ -62,6 +62,14 @@ class HfkStats(C.Structure):
     ]
 
 
+class HfkClassStat(C.Structure):
+    _fields_ = [
+        ("la", C.c_int32), ("lb", C.c_int32), ("lc", C.c_int32), ("ld", C.c_int32),
+        ("prec", C.c_int32), ("pad", C.c_int32), ("shell_quartets", C.c_int64),
+        ("prim_quartets", C.c_int64), ("func_quartets", C.c_int64), ("ms", C.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -94,6 +102,7 @@ def lib():
     L.hfk_scf.argtypes = [vp, C.POINTER(HfkScfOptions), C.POINTER(HfkScfResult), dp, dp, dp, dp]
     L.hfk_gradient.argtypes = [vp, dp, dp, dp]
     L.hfk_get_stats.argtypes = [vp, C.POINTER(HfkStats)]
+    L.hfk_get_class_stats.argtypes = [vp, C.POINTER(HfkClassStat), C.c_int32, C.POINTER(C.c_int32)]
     L.hfk_last_error.argtypes = [C.c_char_p, C.c_size_t]
     for name in EXPORTS:
         f = getattr(L, name)

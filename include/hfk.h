@@ -172,6 +172,21 @@ typedef struct hfk_stats {
 
 int hfk_get_stats(hfk_ctx *ctx, hfk_stats *st);
 
+/* One record per kernel launch of a Fock build: angular-momentum class,
+ * precision, counted survivors and the duration of its last run (HIP events
+ * on the context's stream).  New (measurement; no reference counterpart). */
+typedef struct hfk_class_stat {
+    int32_t la, lb, lc, ld;
+    int32_t prec;                 /* 0 FP64, 1 FP32                                     */
+    int32_t pad;
+    int64_t shell_quartets;       /* shell quartets passing the pair screens            */
+    int64_t prim_quartets;        /* primitive quartets not screened as a whole         */
+    int64_t func_quartets;        /* canonical function quartets they hold              */
+    double ms;                    /* kernel time of the last Fock build / export        */
+} hfk_class_stat;
+
+int hfk_get_class_stats(hfk_ctx *ctx, hfk_class_stat *out, int32_t cap, int32_t *n);
+
 int hfk_last_error(char *buf, size_t len);
 
 #ifdef __cplusplus

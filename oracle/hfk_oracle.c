@@ -743,3 +743,12 @@ void ob_hcore(const ob_basis *bs, const double *tab, int exact, int natom, const
         }
     }
 }
+
+/* A list of function quartets (4 indices each), evaluated with the
+ * reference algorithm (contracted4Centers) in parallel: the CPU baseline. */
+void ob_eri_list(const ob_basis *bs, const double *tab, int exact, int screen, long nq, const int *q, double *out)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long t = 0; t < nq; t++)
+        out[t] = ob_eri(bs, tab, exact, screen, q[4 * t], q[4 * t + 1], q[4 * t + 2], q[4 * t + 3], NULL);
+}

@@ -88,6 +88,7 @@ def lib():
         L.ob_index_dim2_to_flat.restype = C.c_long
         L.ob_flat_get.argtypes = [C.c_long, dp, C.c_long, C.c_long, C.c_long, C.c_long]
         L.ob_flat_get.restype = C.c_double
+        L.ob_eri_list.argtypes = [bp, dp, C.c_int, C.c_int, C.c_long, C.POINTER(C.c_int), dp]
         L.ob_overlap.argtypes = [bp, dp]
         L.ob_hcore.argtypes = [bp, dp, C.c_int, C.c_int, dp, dp, dp]
         _lib = L
@@ -161,6 +162,27 @@ def eri_flat(mol, exact=False, screen=True):
     kept = C.c_long(0)
     lib().ob_eri_flat(C.byref(ob.b), _dp(boys_table()), int(exact), int(screen), _dp(out), C.byref(kept))
     return out, kept.value
+
+
+def eri_list(mol, quartets, exact=False, screen=True):
+    """contracted4Centers over a list of canonical quartets (OpenMP)."""
+    ob = OracleBasis(mol)
+    q = np.ascontiguousarray(quartets, dtype=np.int32).reshape(-1, 4)
+    out = np.zeros(q.shape[0])
+    lib().ob_eri_list(C.byref(ob.b), _dp(boys_table()), int(exact), int(screen), q.shape[0],
+                      q.ctypes.data_as(C.POINTER(C.c_int)), _dp(out))
+    return out
+
+
+def random_canonical_quartets(n, count, seed=0):
+    """Uniform sample of canonical quartets i<=j, k<=l, (i,j)<=(k,l)."""
+    rng = np.random.default_rng(seed)
+    iu, ju = np.triu_indices(n)
+    m = iu.size
+    a = rng.integers(0, m, size=count)
+    b = rng.integers(0, m, size=count)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    return np.stack([iu[lo], ju[lo], iu[hi], ju[hi]], axis=1).astype(np.int32)
 
 
 def flat_rank(n, i, j, k, l):
@@ -268,7 +290,7 @@ def gauss_elem(A, b):
     return x
 
 
-def scf(mol, fock_G, H_flat, S_flat, guess="intended", max_steps=30, ndiis=5, converge=1e-7,
+def scf(mol, fock_G, H_flat, S_flat, guess="core", max_steps=30, ndiis=5, converge=1e-7,
         diis_start=0.1):
     """scfHF/scfDIIS.  fock_G(D_flat) -> G_flat.  Returns dict(energy, steps, C, eps, D, F)."""
     n = mol.nbf

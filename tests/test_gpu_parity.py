@@ -71,3 +71,36 @@ def test_water_dimer_631g_eri_flat(require_gpu):
         got = ctx.calcIntegrals()
     err = np.max(np.abs(got - ref))
     assert err < 1e-12, err
+
+
+@pytest.fixture(scope="module")
+def water8():
+    mol = systems.water_cluster(8, "6-31g")
+    flat, _ = O.eri_flat(mol)
+    return mol, flat
+
+
+def test_water8_gmatrix(require_gpu, water8):
+    _say("case: (H2O)8 6-31g gmatrix")
+    mol, flat = water8
+    n = mol.nbf
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((n, n)) * 0.1
+    D = O.full_to_flat(A + A.T)
+    ref = O.gmatrix(n, flat, D)
+    with hfk.Context(mol) as ctx:
+        got = ctx.calcGmatrix(D)
+        got2 = ctx.calcGmatrix(D)
+    assert np.max(np.abs(got - ref)) < 1e-11 * max(1.0, np.max(np.abs(ref)))
+    assert np.max(np.abs(got2 - got)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_water_dimer_scf(require_gpu):
+    _say("case: (H2O)2 6-31g scf")
+    mol = systems.water_cluster(2, "6-31g")
+    flat, _ = O.eri_flat(mol)
+    ref = O.scf(mol, lambda D: O.gmatrix(mol.nbf, flat, D), O.hcore(mol), O.overlap(mol))
+    with hfk.Context(mol) as ctx:
+        r = ctx.scfHF()
+    assert abs(r.getEnergy - ref["energy"]) < 1e-8
+    assert r.steps == ref["steps"]
