@@ -102,47 +102,50 @@ def gen_class(la, lb, lc, ld):
     out.append("{")
     out.append("#pragma unroll")
     out.append("  for (int z = 0; z < %d; ++z) V[z] = T(0);" % (nbra * nket))
-    out.append("  for (int ib = 0; ib < nbp; ++ib) {")
-    out.append("    const double* pb = bra + (size_t)ib * PP_STRIDE;")
+    # ket primitive pairs outer (per-lane data, loaded once each); bra
+    # primitive pairs inner (wave-uniform: scalar loads)
+    out.append("  for (int ik = 0; ik < nkp; ++ik) {")
+    out.append("    const double* pk = ket + (size_t)ik * PP_STRIDE;")
     em = Emitter()
-    em.raw("    const T bp = T(pb[PP_P]);")
-    em.raw("    const T bPx = T(pb[PP_PX]), bPy = T(pb[PP_PY]), bPz = T(pb[PP_PZ]);")
-    em.raw("    const T bAx = T(pb[PP_AX]), bAy = T(pb[PP_AY]), bAz = T(pb[PP_AZ]);")
-    em.raw("    const T bBx = T(pb[PP_BX]), bBy = T(pb[PP_BY]), bBz = T(pb[PP_BZ]);")
-    em.raw("    const T bK = T(pb[PP_K]);")
-    em.raw("    const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
-    em.raw("    const int bqoff = (int)pb[PP_QOFF];")
-    em.raw("    const T boo2p = T(pb[PP_OO2P]);")
+    em.raw("    const double kqlo = pk[PP_QLO], kqhi = pk[PP_QHI];")
+    em.raw("    const T kq = T(pk[PP_P]);")
+    em.raw("    const T kQx = T(pk[PP_PX]), kQy = T(pk[PP_PY]), kQz = T(pk[PP_PZ]);")
+    em.raw("    const T kCx = T(pk[PP_AX]), kCy = T(pk[PP_AY]), kCz = T(pk[PP_AZ]);")
+    em.raw("    const T kDx = T(pk[PP_BX]), kDy = T(pk[PP_BY]), kDz = T(pk[PP_BZ]);")
+    em.raw("    const T kK = T(pk[PP_K]);")
+    em.raw("    const T koo2q = T(pk[PP_OO2P]);")
     if nket > 1 or nbra > 1:
-        em.raw("    double bqv[%d];" % nbra)
+        em.raw("    double kqv[%d];" % nket)
         em.raw("    if constexpr (MODE == 1) {")
+        em.raw("      const int kqoff = (int)pk[PP_QOFF];")
         em.raw("#pragma unroll")
-        em.raw("      for (int z = 0; z < %d; ++z) bqv[z] = qcomp[bqoff + z];" % nbra)
+        em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = qcomp[kqoff + z];" % nket)
         em.raw("    }")
-    Eb = [hermite_E(em, la, lb, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
+    Ek = [hermite_E(em, lc, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
     out.extend(em.lines)
-    out.append("    for (int ik = 0; ik < nkp; ++ik) {")
-    out.append("      const double* pk = ket + (size_t)ik * PP_STRIDE;")
-    out.append("      const double kqlo = pk[PP_QLO], kqhi = pk[PP_QHI];")
+    out.append("    for (int ib = 0; ib < nbp; ++ib) {")
+    out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
+    out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
     out.append("      // reference screen: component (ab|cd) of this primitive quartet is kept iff")
     out.append("      // !(q_ab*q_cd < tau).  MODE 0: no screen; MODE 1: the screen, branch-free.")
     out.append("      if (MODE == 1 && bqhi * kqhi < qtau) continue;   // every component screened")
     out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
-    if nket > 1 or nbra > 1:
-        out.append("      double kqv[%d];" % nket)
-        out.append("      if constexpr (MODE == 1) {")
-        out.append("        const int kqoff = (int)pk[PP_QOFF];")
-        out.append("#pragma unroll")
-        out.append("        for (int z = 0; z < %d; ++z) kqv[z] = qcomp[kqoff + z];" % nket)
-        out.append("      }")
     em2 = Emitter()
     em2.n = em.n
-    em2.raw("      const T kq = T(pk[PP_P]);")
-    em2.raw("      const T kQx = T(pk[PP_PX]), kQy = T(pk[PP_PY]), kQz = T(pk[PP_PZ]);")
-    em2.raw("      const T kCx = T(pk[PP_AX]), kCy = T(pk[PP_AY]), kCz = T(pk[PP_AZ]);")
-    em2.raw("      const T kDx = T(pk[PP_BX]), kDy = T(pk[PP_BY]), kDz = T(pk[PP_BZ]);")
-    em2.raw("      const T kK = T(pk[PP_K]);")
-    em2.raw("      const T koo2q = T(pk[PP_OO2P]);")
+    em2.raw("      const T bp = T(pb[PP_P]);")
+    em2.raw("      const T bPx = T(pb[PP_PX]), bPy = T(pb[PP_PY]), bPz = T(pb[PP_PZ]);")
+    em2.raw("      const T bAx = T(pb[PP_AX]), bAy = T(pb[PP_AY]), bAz = T(pb[PP_AZ]);")
+    em2.raw("      const T bBx = T(pb[PP_BX]), bBy = T(pb[PP_BY]), bBz = T(pb[PP_BZ]);")
+    em2.raw("      const T bK = T(pb[PP_K]);")
+    em2.raw("      const T boo2p = T(pb[PP_OO2P]);")
+    if nket > 1 or nbra > 1:
+        em2.raw("      double bqv[%d];" % nbra)
+        em2.raw("      if constexpr (MODE == 1) {")
+        em2.raw("        const int bqoff = (int)pb[PP_QOFF];")
+        em2.raw("#pragma unroll")
+        em2.raw("        for (int z = 0; z < %d; ++z) bqv[z] = qcomp[bqoff + z];" % nbra)
+        em2.raw("      }")
+    Eb = [hermite_E(em2, la, lb, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
     em2.raw("      const T pq = bp + kq;")
     em2.raw("      const T alpha = bp * kq / pq;")
     em2.raw("      const T Xpq = bPx - kQx, Ypq = bPy - kQy, Zpq = bPz - kQz;")
@@ -191,8 +194,6 @@ def gen_class(la, lb, lc, ld):
         R[key] = em2.tmp(" + ".join(terms), "r")
         return R[key]
 
-    # ket E (with (-1)^tau folded in later), per axis
-    Ek = [hermite_E(em2, lc, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
     lab = la + lb
     # hermite tuv list for the bra side
     btuv = [(t, u, v) for t in range(lab + 1) for u in range(lab + 1 - t) for v in range(lab + 1 - t - u)]

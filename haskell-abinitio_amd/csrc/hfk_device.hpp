@@ -22,18 +22,23 @@ enum {
 #define HFK_TWO_PI_2_5 34.986836655249725   // 2 pi^(5/2)
 #define HFK_PI 3.141592653589793
 
-// Reference Boys grid (NumericalTools/TableBoys.hs:58-63): m = 0..14, x = 0.1 i, i = 0..1000
+// Reference Boys grid (NumericalTools/TableBoys.hs:58-63): m = 0..14, x = 0.1 i, i = 0..1000.
+// Stored node-major: the BOYS_REF_STRIDE values F_0..F_14 (+pad) of one node
+// share one 128-byte line, so a Taylor step reads one or two lines.
 #define BOYS_REF_M 15
 #define BOYS_NX 1001
-// Accurate grid: m = 0..ACC_M-1, x = 0.1 i, i = 0..ACC_NX-1 (x <= 50); above, asymptotic + recursion
+#define BOYS_REF_STRIDE 16
+// Accurate grid: m = 0..ACC_M-1, x = 0.1 i, i = 0..ACC_NX-1 (x <= 50), node-major;
+// above, asymptotic + recursion
 #define BOYS_ACC_M 26
 #define BOYS_ACC_NX 501
+#define BOYS_ACC_STRIDE 32
 #define BOYS_ACC_XMAX 50.0
 
 struct BoysTables {
-    const double* ref64;   // [BOYS_REF_M][BOYS_NX]
+    const double* ref64;   // [BOYS_NX][BOYS_REF_STRIDE]
     const float* ref32;
-    const double* acc64;   // [BOYS_ACC_M][BOYS_ACC_NX]
+    const double* acc64;   // [BOYS_ACC_NX][BOYS_ACC_STRIDE]
     const float* acc32;
     double asym[24];       // a_m = facOdd(floor(2m-1)) / 2^(m+1), with the reference's Int64 wrap
 };
@@ -74,9 +79,9 @@ __device__ __forceinline__ void boys_eval(T x, const BoysTables& bt, T* F)
         } else {
             int idx = (int)fl;
             idx = idx < 0 ? 0 : (idx > BOYS_NX - 1 ? BOYS_NX - 1 : idx);   // NaN/negative guard
-            const T* tab = boys_ref_tab<T>(bt) + idx;
+            const T* tab = boys_ref_tab<T>(bt) + (size_t)idx * BOYS_REF_STRIDE;
 #pragma unroll
-            for (int m = 0; m < L + 6; ++m) f[m] = tab[m * BOYS_NX];
+            for (int m = 0; m < L + 6; ++m) f[m] = tab[m];
         }
         const T d2 = dx * dx, d3 = d2 * dx, d4 = d2 * d2, d5 = d4 * dx;
 #pragma unroll
@@ -89,13 +94,13 @@ __device__ __forceinline__ void boys_eval(T x, const BoysTables& bt, T* F)
             const T dx = x - fl * T(0.1);
             int idx = (int)fl;
             idx = idx < 0 ? 0 : (idx > BOYS_ACC_NX - 1 ? BOYS_ACC_NX - 1 : idx);
-            const T* tab = boys_acc_tab<T>(bt) + idx;
+            const T* tab = boys_acc_tab<T>(bt) + (size_t)idx * BOYS_ACC_STRIDE;
             const T mdx = -dx;
 #pragma unroll
             for (int m = 0; m <= L; ++m) {
-                T acc = tab[(m + 7) * BOYS_ACC_NX];
+                T acc = tab[m + 7];
 #pragma unroll
-                for (int k = 7; k >= 1; --k) acc = tab[(m + k - 1) * BOYS_ACC_NX] + (mdx / T(k)) * acc;
+                for (int k = 7; k >= 1; --k) acc = tab[m + k - 1] + (mdx / T(k)) * acc;
                 F[m] = acc;
             }
         } else {

@@ -38,12 +38,16 @@ __device__ __forceinline__ void decode_quartet(const QuartetArgs& a, long long q
     }
 }
 
+// Primitive-pair sub-ranges [b0,b1) of the bra and [k0,k1) of the ket
+// (b1/k1 < 0: to the end).
 template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
-__device__ __forceinline__ void eval_quartet(const QuartetArgs& a, int bp, int kp, T* V, int b0 = 0, int b1 = -1)
+__device__ __forceinline__ void eval_quartet(const QuartetArgs& a, int bp, int kp, T* V, int b0 = 0, int b1 = -1,
+                                             int k0 = 0, int k1 = -1)
 {
     const double* bra = a.pp + (size_t)(a.pair_pp0[bp] + b0) * PP_STRIDE;
-    const double* ket = a.pp + (size_t)a.pair_pp0[kp] * PP_STRIDE;
-    const int nbp = (b1 < 0 ? a.pair_npp[bp] : b1) - b0, nkp = a.pair_npp[kp];
+    const double* ket = a.pp + (size_t)(a.pair_pp0[kp] + k0) * PP_STRIDE;
+    const int nbp = (b1 < 0 ? a.pair_npp[bp] : b1) - b0;
+    const int nkp = (k1 < 0 ? a.pair_npp[kp] : k1) - k0;
     eri_class<LA, LB, LC, LD, T, BOYS, MODE>(bra, nbp, ket, nkp, a.qcomp, a.bt, V, a.qtau);
 }
 
@@ -159,11 +163,13 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------
-// Tiled J/K kernel: one 64-lane workgroup per task = (bra shell pair, range of
-// its primitive pairs, run of an explicit ket pair array; kets ordered by
-// primitive-pair count).  Splitting the bra primitive range of heavy pairs
-// evens out the work per lane; the digestion is linear in the integrals, so
-// partial quartets are digested independently.  The bra pair is
+// Tiled J/K kernel: one workgroup per task = (bra shell pair, run of an
+// explicit ket pair array with equal primitive-pair counts, primitive-pair
+// ranges of bra and ket).  Each lane loops over its ket primitive pairs
+// (vector loads, once each) and, inside, over the bra primitive pairs
+// (wave-uniform, scalar loads).  Splitting heavy quartets into primitive
+// ranges evens out the work per lane; the digestion is linear in the
+// integrals, so partial quartets are digested independently.  The bra pair is
 // wave-uniform (its data comes through scalar loads); each lane takes ket
 // pairs of the range.  D, J', K' are (n+1) x (n+1) with a zero row/column
 // n that absent Cartesian components point to, so the digestion has no
@@ -173,22 +179,26 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 //                                  (ds_add_f64), flushed once per task
 //   J'_cd += 2f sum_ab V D_ab      one global FP64 atomic per ket function pair
 // ---------------------------------------------------------------------
-constexpr int TILE = 64;
+// Workgroups are 64 lanes, or 256 lanes (4 waves sharing one set of staged K
+// rows) when the rows are long enough to limit occupancy through LDS.
+constexpr int WAVE = 64;
 
 template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
-__global__ void __launch_bounds__(TILE) k_jk_tiles(QuartetArgs a)
+__global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NKET = NC * ND, NBRA = NA * NB;
     extern __shared__ double krow[];   // [(NA+NB)][ld]
     const int4 task = a.tasks[blockIdx.x];
     const int bp = task.x;
-    const int bprim0 = task.w >> 16, bprim1 = task.w & 0xffff;   // bra primitive-pair range
+    // primitive-pair ranges: bra [b0,b1), ket [k0,k1), 8 bits each
+    const int bprim0 = (task.w >> 24) & 0xff, bprim1 = (task.w >> 16) & 0xff;
+    const int kprim0 = (task.w >> 8) & 0xff, kprim1 = task.w & 0xff;
     const int ld = a.n + 1;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & (WAVE - 1);
     const bool lds = a.lds_rows != 0;
     if (lds)
-        for (int t = lane; t < (NA + NB) * ld; t += TILE) krow[t] = 0.0;
+        for (int t = tid; t < (NA + NB) * ld; t += nthr) krow[t] = 0.0;
     __syncthreads();
 
     const int A = a.pair_A[bp], B = a.pair_B[bp];
@@ -210,13 +220,13 @@ __global__ void __launch_bounds__(TILE) k_jk_tiles(QuartetArgs a)
 
     const int len = task.z - task.y;
     const int rot = (int)((unsigned)task.x * 37u % (unsigned)(len > 0 ? len : 1));
-    for (int it = lane; it < len; it += TILE) {
+    for (int it = tid; it < len; it += nthr) {
         int jj = it + rot;
         if (jj >= len) jj -= len;
         const int kp = a.ket_idx[task.y + jj];
         if (!quartet_selected(a, bp, kp)) continue;
         T Vt[NBRA * NKET];
-        eval_quartet<LA, LB, LC, LD, T, BOYS, MODE>(a, bp, kp, Vt, bprim0, bprim1);
+        eval_quartet<LA, LB, LC, LD, T, BOYS, MODE>(a, bp, kp, Vt, bprim0, bprim1, kprim0, kprim1);
         const int C = a.pair_A[kp], Dd = a.pair_B[kp];
         int ic[NC], id[ND];
         double sc[NC], sd[ND];
@@ -323,18 +333,18 @@ __global__ void __launch_bounds__(TILE) k_jk_tiles(QuartetArgs a)
             }
         }
     }
-    // J'_ab: wave reduction, one atomic per bra function pair per task
+    // J'_ab: wave reduction, one atomic per bra function pair per wave
 #pragma unroll
     for (int b = 0; b < NBRA; ++b) {
         double v = jab[b];
 #pragma unroll
-        for (int off = TILE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, TILE);
+        for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
         if (lane == 0 && v != 0.0) atomic_add_f64(&a.J[(size_t)ia[b / NB] * ld + ib[b % NB]], 2.0 * v);
     }
     // flush the staged K rows
     if (lds) {
         __syncthreads();
-        for (int t = lane; t < (NA + NB) * ld; t += TILE) {
+        for (int t = tid; t < (NA + NB) * ld; t += nthr) {
             const double v = krow[t];
             if (v != 0.0) {
                 const int r = t / ld, col = t - r * ld;
@@ -356,7 +366,7 @@ void launch_tiles(const QuartetArgs& a, hipStream_t s)
         const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);
         QuartetArgs b = a;
         b.tasks = a.tasks + done;
-        hipLaunchKernelGGL((k_jk_tiles<LA, LB, LC, LD, T, BOYS, MODE>), dim3((unsigned)cnt), dim3(TILE), shm, s, b);
+        hipLaunchKernelGGL((k_jk_tiles<LA, LB, LC, LD, T, BOYS, MODE>), dim3((unsigned)cnt), dim3(a.block), shm, s, b);
         done += cnt;
     }
 }

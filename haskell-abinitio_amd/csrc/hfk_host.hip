@@ -239,10 +239,12 @@ int hfk_ctx_init_device(hfk_ctx* c)
     gl256(glx.data(), glw.data());
     HFK_TRY(dev_upload(c, &c->d_glx, glx));
     HFK_TRY(dev_upload(c, &c->d_glw, glw));
-    HFK_TRY(dev_alloc(&c->d_boys_ref64, (size_t)BOYS_REF_M * BOYS_NX));
-    HFK_TRY(dev_alloc(&c->d_boys_ref32, (size_t)BOYS_REF_M * BOYS_NX));
-    HFK_TRY(dev_alloc(&c->d_boys_acc64, (size_t)BOYS_ACC_M * BOYS_ACC_NX));
-    HFK_TRY(dev_alloc(&c->d_boys_acc32, (size_t)BOYS_ACC_M * BOYS_ACC_NX));
+    HFK_TRY(dev_alloc(&c->d_boys_ref64, (size_t)BOYS_REF_STRIDE * BOYS_NX));
+    HFK_TRY(dev_alloc(&c->d_boys_ref32, (size_t)BOYS_REF_STRIDE * BOYS_NX));
+    HFK_TRY(dev_alloc(&c->d_boys_acc64, (size_t)BOYS_ACC_STRIDE * BOYS_ACC_NX));
+    HFK_TRY(dev_alloc(&c->d_boys_acc32, (size_t)BOYS_ACC_STRIDE * BOYS_ACC_NX));
+    hfk_launch_fill(c->d_boys_ref64, 0.0, (long long)BOYS_REF_STRIDE * BOYS_NX, c->stream);
+    hfk_launch_fill(c->d_boys_acc64, 0.0, (long long)BOYS_ACC_STRIDE * BOYS_ACC_NX, c->stream);
     hfk_launch_boys_tables(c->d_glx, c->d_glw, c->d_boys_ref64, c->d_boys_ref32, c->d_boys_acc64, c->d_boys_acc32,
                            c->stream);
     HFK_HIP(hipGetLastError());
@@ -532,7 +534,7 @@ int hfk_ctx_plan(hfk_ctx* c)
     // primitive-pair-count run; those prefixes are concatenated into an
     // explicit ket array and cut into tasks of `chunk` kets, so the lanes of
     // a workgroup are busy and mostly do equal work.
-    const int chunk = 64 * std::max(1, std::min(16, c->n / 64));
+    const int chunk = c->n >= 200 ? 256 : 64;   // one ket per lane of the workgroup
     for (PlanEntry& e : c->plan) {
         const std::vector<int>& bl = c->h_class[e.kb];
         const std::vector<int>& kl = c->h_class[e.kk];
@@ -566,17 +568,28 @@ int hfk_ctx_plan(hfk_ctx* c)
                 const long long k0 = (long long)kets.size();
                 for (long long j = sg.first; j < lo; ++j) kets.push_back(kl[j]);
                 const long long k1 = (long long)kets.size();
-                // bra primitive chunks: a lane's share of a task is bounded to about
-                // 4096 McMurchie-Davidson operations (primitive quartets x cost)
+                // Primitive ranges: a lane's share of a task is about per_lane primitive
+                // quartets, sized from the launch's total so that small launches still
+                // fill the GPU (~2.6e5 lanes in flight) and large ones keep quartets whole
+                // (fewer partial digestions).  Bra primitives are wave-uniform (scalar
+                // loads), so a lane keeps as many of them as fit and the ket range is split.
                 const int nkp = c->h_pair_npp[kl[sg.first]];
                 const int L = e.la + e.lb + e.lc + e.ld;
-                const int per_lane = std::max(1, 64 >> (L > 4 ? 4 : L));   // 64 (ss|ss) ... 4 (pp|pp)
-                const int bstep = std::max(1, std::min(nbp, (per_lane + nkp - 1) / nkp));
+                const long long floor_l = std::max(4, 32 >> L);   // cheap classes: amortise the digestion
+                const int per_lane = (int)std::max(floor_l, std::min(128LL, e.prim_q / 262144));
+                int bstep = nbp, kstep = nkp;
+                if (nbp * nkp > per_lane) {
+                    bstep = std::max(1, std::min(nbp, per_lane));
+                    kstep = std::max(1, std::min(nkp, per_lane / bstep));
+                }
                 for (int b0 = 0; b0 < nbp; b0 += bstep) {
                     const int b1 = std::min(nbp, b0 + bstep);
-                    for (long long j0 = k0; j0 < k1; j0 += chunk)
-                        tasks.push_back(make_int4(bl[i], (int)j0, (int)std::min<long long>(k1, j0 + chunk),
-                                                  (b0 << 16) | b1));
+                    for (int q0 = 0; q0 < nkp; q0 += kstep) {
+                        const int q1 = std::min(nkp, q0 + kstep);
+                        const int w = (b0 << 24) | (b1 << 16) | (q0 << 8) | q1;
+                        for (long long j0 = k0; j0 < k1; j0 += chunk)
+                            tasks.push_back(make_int4(bl[i], (int)j0, (int)std::min<long long>(k1, j0 + chunk), w));
+                    }
                 }
             }
         }
@@ -603,6 +616,7 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
     a.tasks = e.d_tasks;
     a.ket_idx = e.d_kets;
     a.ntask = e.ntask;
+    a.block = c->n >= 200 ? 256 : 64;
     {
         const int nr = (e.la + 1) * (e.la + 2) / 2 + (e.lb + 1) * (e.lb + 2) / 2;
         a.lds_rows = ((size_t)nr * c->n * sizeof(double) <= 64 * 1024) ? 1 : 0;

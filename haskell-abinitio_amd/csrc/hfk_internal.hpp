@@ -45,6 +45,7 @@ struct QuartetArgs {
     const int* ket_idx;         // explicit ket pair indices of the tasks
     long long ntask;
     int lds_rows;               // K rows staged in LDS (0: global atomics)
+    int block;                  // workgroup size of the tiled kernel (64 or 256)
 };
 
 enum { HFK_OP_JK = 0, HFK_OP_FLAT = 1, HFK_OP_DIAG = 2 };

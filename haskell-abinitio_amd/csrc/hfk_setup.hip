@@ -59,14 +59,14 @@ __global__ void k_boys_tables(const double* glx, const double* glw, double* ref6
     if (t < nref) {
         const int m = t / BOYS_NX, i = t % BOYS_NX;
         const double v = boys_ref_node(m, 0.1 * (double)i, glx, glw);
-        ref64[t] = v;
-        ref32[t] = (float)v;
+        ref64[(size_t)i * BOYS_REF_STRIDE + m] = v;
+        ref32[(size_t)i * BOYS_REF_STRIDE + m] = (float)v;
     } else if (t < nref + BOYS_ACC_M * BOYS_ACC_NX) {
         const int u = t - nref;
         const int m = u / BOYS_ACC_NX, i = u % BOYS_ACC_NX;
         const double v = boys_acc_node(m, 0.1 * (double)i);
-        acc64[u] = v;
-        acc32[u] = (float)v;
+        acc64[(size_t)i * BOYS_ACC_STRIDE + m] = v;
+        acc32[(size_t)i * BOYS_ACC_STRIDE + m] = (float)v;
     }
 }
 
