@@ -63,6 +63,42 @@ def cpu_baseline(mol, target_s=12.0):
                       "contracted4Centers, reference screen + Boys table, OpenMP)" % (count, m * (m + 1) // 2, dt)}
 
 
+def mixed_vs_fp64(workload, thr):
+    """SCF wall time FP64 vs mixed (FP32 for shell quartets whose Schwarz
+    bound is below thr) on the same molecule; |dE|, orbital-energy error and
+    one Fock build each."""
+    import numpy as np
+    import hfk
+    from hfk import systems
+    nwat, desc = WORKLOADS[workload]
+    mol = systems.water_cluster(nwat, "6-31g")
+    res = {"workload": desc, "fp32_threshold": thr}
+    runs = {}
+    for name, opts in (("fp64", hfk.options()), ("mixed", hfk.options(fp32_threshold=thr))):
+        t0 = time.perf_counter()
+        ctx = hfk.Context(mol, opts)
+        t_ctx = time.perf_counter() - t0
+        r = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60)
+        ms = []
+        for _ in range(3):
+            ctx.calcGmatrix(r.getDensity)
+            ms.append(ctx.stats()["last_eri_ms"])
+        cs = ctx.class_stats()
+        ctx.close()
+        runs[name] = r
+        res[name] = {"energy": r.getEnergy, "steps": r.steps, "scf_s": r.seconds, "wall_s": r.seconds + t_ctx,
+                     "fock_ms": min(ms),
+                     "fp32_shell_quartets": sum(c["shell_quartets"] for c in cs if c["prec"] == 1),
+                     "fp64_shell_quartets": sum(c["shell_quartets"] for c in cs if c["prec"] == 0)}
+        log("bench: %s %s E=%.10f steps=%d scf %.2fs fock %.2fms" % (workload, name, r.getEnergy, r.steps,
+                                                                       r.seconds, min(ms)))
+    res["dE"] = runs["mixed"].getEnergy - runs["fp64"].getEnergy
+    res["max_dEps"] = float(np.max(np.abs(runs["mixed"].getOrbE - runs["fp64"].getOrbE)))
+    res["fock_speedup"] = res["fp64"]["fock_ms"] / res["mixed"]["fock_ms"]
+    res["scf_speedup"] = res["fp64"]["scf_s"] / res["mixed"]["scf_s"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +107,9 @@ def main():
     ap.add_argument("--workload", default="h2o8", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--mixed-workload", default="h2o32", choices=sorted(WORKLOADS))
+    ap.add_argument("--mixed-thr", type=float, default=1e-4)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
@@ -178,6 +217,8 @@ def main():
         "scf": scf,
     }
     ctx.close()
+    if not args.no_mixed:
+        out["mixed_vs_fp64"] = mixed_vs_fp64(args.mixed_workload, args.mixed_thr)
     if not args.no_cpu:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(mol)
