@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-workload", default="h2o32", choices=sorted(WORKLOADS))
     ap.add_argument("--mixed-thr", type=float, default=1e-4)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="before timing, run one device copy of this size (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
@@ -148,6 +150,13 @@ def main():
         rng = np.random.default_rng(0)
         A = rng.standard_normal((n, n)) * 0.01
         D = (A + A.T)[np.triu_indices(n)]
+    if args.calibrate_copy_mib > 0:
+        src = torch.ones(args.calibrate_copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     nf = n * (n + 1) // 2
     dD = torch.tensor(D, dtype=torch.float64, device="cuda")
     dG = torch.empty(nf, dtype=torch.float64, device="cuda")
@@ -190,6 +199,12 @@ def main():
                  and ("fp32" if c["prec"] else "fp64") == dom["prec"]][0]
     achieved = dom_flops / (dom["ms"] * 1e-3) / 1e12
     peak = PEAK_FP32 if dom["prec"] == "fp32" else PEAK_FP64
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # FETCH_SIZE/WRITE_SIZE summary of the same workload (calibrated).
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "jk_tiles_%s_traffic.json" % args.workload)
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath))["traffic"].get("%s %s" % (dom["class"], dom["prec"]))
     out = {
         "metric": METRIC,
         "value": value,
@@ -211,7 +226,7 @@ def main():
         "alg_tflops_all_classes": total_flops / (eri_avg * 1e-3) / 1e12,
         "classes": classes,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic,
                      "kernel": "k_quartets %s %s (ERI + fused J/K)" % (dom["class"], dom["prec"]),
                      "note": "FP64 vector-ALU kernel; peak is MI355X FP64 (vector = matrix figure)"},
         "scf": scf,
