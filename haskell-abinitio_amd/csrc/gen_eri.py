@@ -263,6 +263,227 @@ def gen_class(la, lb, lc, ld):
     return "\n".join(out), (na, nb, nc, nd)
 
 
+def gen_grad_class(la, lb, lc, ld):
+    """Derivative-integral contraction for the nuclear gradient of class
+    (la lb|lc ld): per primitive quartet, with the two-particle density
+    G[ab][cd] of the shell quartet,
+        gA_k += sum_abcd G (d/dA_k a, b|c d),  dA_k g_a = 2 alpha g_{a+1_k} - a_k g_{a-1_k}
+    and likewise B (exponent beta) and C (exponent gamma); D follows from
+    translational invariance.  McMurchie-Davidson: the shifted functions
+    enter through Hermite coefficients E^{i+-1,j}, R_tuv runs to L+1."""
+    L = la + lb + lc + ld
+    ca, cb, cc, cd = comps(la), comps(lb), comps(lc), comps(ld)
+    na, nb, nc, nd = len(ca), len(cb), len(cc), len(cd)
+    nbra, nket = na * nb, nc * nd
+    name = "grad_%d%d%d%d" % (la, lb, lc, ld)
+    out = []
+    out.append("template <typename T, int BOYS, int MODE>")
+    out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp," % name)
+    out.append("    const double* __restrict__ qcomp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
+    out.append("{")
+    out.append("#pragma unroll")
+    out.append("  for (int z = 0; z < 9; ++z) g[z] = T(0);")
+    out.append("  for (int ik = 0; ik < nkp; ++ik) {")
+    out.append("    const double* pk = ket + (size_t)ik * PP_STRIDE;")
+    em = Emitter()
+    em.raw("    const double kqlo = pk[PP_QLO], kqhi = pk[PP_QHI];")
+    em.raw("    const T kq = T(pk[PP_P]);")
+    em.raw("    const T kQx = T(pk[PP_PX]), kQy = T(pk[PP_PY]), kQz = T(pk[PP_PZ]);")
+    em.raw("    const T kCx = T(pk[PP_AX]), kCy = T(pk[PP_AY]), kCz = T(pk[PP_AZ]);")
+    em.raw("    const T kDx = T(pk[PP_BX]), kDy = T(pk[PP_BY]), kDz = T(pk[PP_BZ]);")
+    em.raw("    const T kK = T(pk[PP_K]);")
+    em.raw("    const T koo2q = T(pk[PP_OO2P]);")
+    em.raw("    const T kgam2 = T(2) * T(pk[PP_EA]);")
+    if nket > 1 or nbra > 1:
+        em.raw("    double kqv[%d];" % nket)
+        em.raw("    if constexpr (MODE == 1) {")
+        em.raw("      const int kqoff = (int)pk[PP_QOFF];")
+        em.raw("#pragma unroll")
+        em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = qcomp[kqoff + z];" % nket)
+        em.raw("    }")
+    Ek = [hermite_E(em, lc + 1, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
+    out.extend(em.lines)
+    out.append("    for (int ib = 0; ib < nbp; ++ib) {")
+    out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
+    out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
+    out.append("      if (MODE == 1 && bqhi * kqhi < qtau) continue;")
+    out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
+    em2 = Emitter()
+    em2.n = em.n
+    em2.raw("      const T bp = T(pb[PP_P]);")
+    em2.raw("      const T bPx = T(pb[PP_PX]), bPy = T(pb[PP_PY]), bPz = T(pb[PP_PZ]);")
+    em2.raw("      const T bAx = T(pb[PP_AX]), bAy = T(pb[PP_AY]), bAz = T(pb[PP_AZ]);")
+    em2.raw("      const T bBx = T(pb[PP_BX]), bBy = T(pb[PP_BY]), bBz = T(pb[PP_BZ]);")
+    em2.raw("      const T bK = T(pb[PP_K]);")
+    em2.raw("      const T boo2p = T(pb[PP_OO2P]);")
+    em2.raw("      const T balp2 = T(2) * T(pb[PP_EA]);")
+    em2.raw("      const T bbet2 = T(2) * (T(pb[PP_P]) - T(pb[PP_EA]));")
+    if nket > 1 or nbra > 1:
+        em2.raw("      double bqv[%d];" % nbra)
+        em2.raw("      if constexpr (MODE == 1) {")
+        em2.raw("        const int bqoff = (int)pb[PP_QOFF];")
+        em2.raw("#pragma unroll")
+        em2.raw("        for (int z = 0; z < %d; ++z) bqv[z] = qcomp[bqoff + z];" % nbra)
+        em2.raw("      }")
+    Eb = [hermite_E(em2, la + 1, lb + 1, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
+    em2.raw("      const T pq = bp + kq;")
+    em2.raw("      const T alpha = bp * kq / pq;")
+    em2.raw("      const T Xpq = bPx - kQx, Ypq = bPy - kQy, Zpq = bPz - kQz;")
+    em2.raw("      const T Targ = alpha * ((Xpq * Xpq + Ypq * Ypq) + Zpq * Zpq);")
+    em2.raw("      T F[%d];" % (L + 2))
+    em2.raw("      boys_eval<T, %d, BOYS>(Targ, bt, F);" % (L + 1))
+    em2.raw("      const T pref = T(HFK_TWO_PI_2_5) * bK * kK / sqrt_t(pq);")
+    em2.raw("      const T m2a = T(-2) * alpha;")
+    # masked density per component quartet
+    Gm = {}
+    for ab in range(nbra):
+        for cdi in range(nket):
+            k = ab * nket + cdi
+            if nket > 1 or nbra > 1:
+                Gm[(ab, cdi)] = em2.tmp("(MODE == 0 || allin || !(bqv[%d] * kqv[%d] < qtau)) ? G[%d] : T(0)" % (ab, cdi, k), "gm")
+            else:
+                Gm[(ab, cdi)] = "G[%d]" % k
+    R = {}
+    pw = "pref"
+    for n in range(0, L + 2):
+        if n == 0:
+            R[(0, 0, 0, 0)] = em2.tmp("pref * F[0]", "r")
+        else:
+            pw = em2.tmp("%s * m2a" % pw, "w")
+            R[(n, 0, 0, 0)] = em2.tmp("%s * F[%d]" % (pw, n), "r")
+
+    def getR(n, t, u, v):
+        if t < 0 or u < 0 or v < 0:
+            return None
+        key = (n, t, u, v)
+        if key in R:
+            return R[key]
+        if t >= 1:
+            a = getR(n + 1, t - 2, u, v); b = getR(n + 1, t - 1, u, v); X = "Xpq"; c = t - 1
+        elif u >= 1:
+            a = getR(n + 1, t, u - 2, v); b = getR(n + 1, t, u - 1, v); X = "Ypq"; c = u - 1
+        else:
+            a = getR(n + 1, t, u, v - 2); b = getR(n + 1, t, u, v - 1); X = "Zpq"; c = v - 1
+        terms = []
+        if a is not None and c != 0:
+            terms.append("T(%d)*%s" % (c, a))
+        terms.append("%s*%s" % (X, b))
+        R[key] = em2.tmp(" + ".join(terms), "r")
+        return R[key]
+
+    def prod_sum(factors_by_axis, inner):
+        """sum over (t,u,v) of fx[t] fy[u] fz[v] inner(t,u,v); factors are lists of (index, expr)."""
+        tx = []
+        for t, ex in factors_by_axis[0]:
+            ty = []
+            for u, ey in factors_by_axis[1]:
+                tz = []
+                for v, ez in factors_by_axis[2]:
+                    val = inner(t, u, v)
+                    if val is None:
+                        continue
+                    tz.append(val if ez == "T(1)" else "%s*%s" % (ez, val))
+                if tz:
+                    s = " + ".join(tz)
+                    ty.append("(%s)" % s if ey == "T(1)" else "%s*(%s)" % (ey, s))
+            if ty:
+                s = " + ".join(ty)
+                tx.append("(%s)" % s if ex == "T(1)" else "%s*(%s)" % (ex, s))
+        return " + ".join(tx) if tx else None
+
+    def Efac(E, i, j, sign=False):
+        return [(t, E[(i, j, t)]) for t in range(i + j + 1)]
+
+    def dEfac(E, i, j, w, shift_first):
+        """2w E^{i+1,j} - i E^{i-1,j} (shift_first) or 2w E^{i,j+1} - j E^{i,j-1}."""
+        outl = []
+        if shift_first:
+            up, dn, m = (i + 1, j), (i - 1, j), i
+        else:
+            up, dn, m = (i, j + 1), (i, j - 1), j
+        for t in range(up[0] + up[1] + 1):
+            terms = ["%s*%s" % (w, E[(up[0], up[1], t)]) if E[(up[0], up[1], t)] != "T(1)" else w]
+            if m > 0 and t <= dn[0] + dn[1]:
+                e = E[(dn[0], dn[1], t)]
+                terms.append("T(%d)*%s" % (m, e) if e != "T(1)" else "T(%d)" % m)
+                expr = "(%s - %s)" % (terms[0], terms[1])
+            else:
+                expr = terms[0]
+            outl.append((t, em2.tmp(expr, "de")))
+        return outl
+
+    lab, lcd = la + lb, lc + ld
+    # W^{cd}_{tuv} = sum (-1)^{tau+nu+phi} E^{cd} R_{t+tau,...}, t+u+v <= lab+1
+    btuv = [(t, u, v) for t in range(lab + 2) for u in range(lab + 2 - t) for v in range(lab + 2 - t - u)]
+    W = {}
+    for ic, c in enumerate(cc):
+        for id_, d in enumerate(cd):
+            kidx = ic * nd + id_
+            for (t, u, v) in btuv:
+                fac = [[(tau, Ek[x][(c[x], d[x], tau)]) for tau in range(c[x] + d[x] + 1)] for x in range(3)]
+                def inner(tau, nu, phi, t=t, u=u, v=v):
+                    r = getR(0, t + tau, u + nu, v + phi)
+                    return ("-%s" % r) if (tau + nu + phi) % 2 else r
+                val = prod_sum(fac, inner)
+                W[(kidx, t, u, v)] = em2.tmp(val.replace("+ -", "- "), "W")
+    # Y^{ab}_{tuv} = sum_cd Gm[ab][cd] W^{cd}_tuv, then bra derivatives
+    for ia, a in enumerate(ca):
+        for ib_, b in enumerate(cb):
+            bidx = ia * nb + ib_
+            Y = {}
+            for (t, u, v) in btuv:
+                if t > a[0] + b[0] + 1 or u > a[1] + b[1] + 1 or v > a[2] + b[2] + 1:
+                    continue
+                terms = ["%s*%s" % (Gm[(bidx, k)], W[(k, t, u, v)]) for k in range(nket)]
+                Y[(t, u, v)] = em2.tmp(" + ".join(terms), "Y")
+            for k in range(3):   # direction
+                for which, w, base in ((0, "balp2", 0), (1, "bbet2", 3)):
+                    fac = []
+                    for x in range(3):
+                        if x == k:
+                            fac.append(dEfac(Eb[x], a[x], b[x], w, which == 0))
+                        else:
+                            fac.append(Efac(Eb[x], a[x], b[x]))
+                    val = prod_sum(fac, lambda t, u, v: Y.get((t, u, v)))
+                    if val:
+                        em2.raw("      g[%d] += %s;" % (base + k, val))
+    # ket derivatives: X^{ab}_{tnp} = sum_tuv E^{ab}_tuv R_{t+tau,...}, tau+nu+phi <= lcd+1
+    ktuv = [(t, u, v) for t in range(lcd + 2) for u in range(lcd + 2 - t) for v in range(lcd + 2 - t - u)]
+    Xv = {}
+    for ia, a in enumerate(ca):
+        for ib_, b in enumerate(cb):
+            bidx = ia * nb + ib_
+            for (tau, nu, phi) in ktuv:
+                fac = [Efac(Eb[x], a[x], b[x]) for x in range(3)]
+                val = prod_sum(fac, lambda t, u, v, tau=tau, nu=nu, phi=phi: getR(0, t + tau, u + nu, v + phi))
+                Xv[(bidx, tau, nu, phi)] = em2.tmp(val, "X")
+    for ic, c in enumerate(cc):
+        for id_, d in enumerate(cd):
+            kidx = ic * nd + id_
+            Z = {}
+            for (tau, nu, phi) in ktuv:
+                if tau > c[0] + d[0] + 1 or nu > c[1] + d[1] + 1 or phi > c[2] + d[2] + 1:
+                    continue
+                terms = ["%s*%s" % (Gm[(bb, kidx)], Xv[(bb, tau, nu, phi)]) for bb in range(nbra)]
+                z = em2.tmp(" + ".join(terms), "Z")
+                Z[(tau, nu, phi)] = ("-%s" % z) if (tau + nu + phi) % 2 else z
+            for k in range(3):
+                fac = []
+                for x in range(3):
+                    if x == k:
+                        fac.append(dEfac(Ek[x], c[x], d[x], "kgam2", True))
+                    else:
+                        fac.append(Efac(Ek[x], c[x], d[x]))
+                val = prod_sum(fac, lambda t, u, v: Z.get((t, u, v)))
+                if val:
+                    em2.raw("      g[%d] += %s;" % (6 + k, val.replace("+ -", "- ")))
+    out.extend(em2.lines)
+    out.append("    }")
+    out.append("  }")
+    out.append("}")
+    return "\n".join(out)
+
+
 def pair_classes(lmax):
     return [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
 
@@ -297,8 +518,16 @@ def main():
         body.append("__device__ __forceinline__ void eri_class(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp,")
         body.append("    const double* __restrict__ qcomp, const BoysTables& bt, T* __restrict__ V, double qtau)")
         body.append("{ eri_%s<T, BOYS, MODE>(bra, nbp, ket, nkp, qcomp, bt, V, qtau); }" % tag)
+        body.append(gen_grad_class(*q))
+        body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>")
+        body.append("__device__ __forceinline__ void grad_class(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp,")
+        body.append("    const double* __restrict__ qcomp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
+        body.append("{ grad_%s<T, BOYS, MODE>(bra, nbp, ket, nkp, qcomp, bt, G, qtau, g); }" % tag)
         body.append("}  // namespace")
         body.append('#include "hfk_eri_impl.hpp"')
+        body.append('#include "hfk_grad_impl.hpp"')
+        body.append("int hfk_dispatch_grad_%s(int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag)
+        body.append("{ return dispatch_grad<%d, %d, %d, %d>(boys, mode, a, s); }" % q)
         body.append("int hfk_dispatch_%s(int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag)
         body.append("{ return dispatch<%d, %d, %d, %d>(op, prec, boys, mode, a, s); }" % q)
         path = os.path.join(outdir, "eri_%s.hip" % tag)
@@ -309,6 +538,14 @@ def main():
     d = ["// Generated by gen_eri.py -- do not edit.", "#include <hip/hip_runtime.h>", '#include "hfk_internal.hpp"']
     for q in qs:
         d.append("int hfk_dispatch_%d%d%d%d(int, int, int, int, const QuartetArgs&, hipStream_t);" % q)
+    for q in qs:
+        d.append("int hfk_dispatch_grad_%d%d%d%d(int, int, const QuartetArgs&, hipStream_t);" % q)
+    d.append("int hfk_launch_grad(int la, int lb, int lc, int ld, int boys, int mode, const QuartetArgs& a, hipStream_t s)")
+    d.append("{")
+    for q in qs:
+        d.append("    if (la == %d && lb == %d && lc == %d && ld == %d) return hfk_dispatch_grad_%d%d%d%d(boys, mode, a, s);" % (q + q))
+    d.append("    return -2;")
+    d.append("}")
     d.append("int hfk_launch_quartets(int la, int lb, int lc, int ld, int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)")
     d.append("{")
     for q in qs:

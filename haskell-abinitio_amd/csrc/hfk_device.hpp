@@ -15,7 +15,7 @@ enum {
     PP_QHI,         // max over components                             (NaN -> +inf)
     PP_QOFF,        // offset of this pair's per-component screen values in qcomp[]
     PP_OO2P,        // 1 / (2p)
-    PP_PAD,
+    PP_EA,          // exponent of the first shell's primitive (gradient)
     PP_STRIDE = 16
 };
 

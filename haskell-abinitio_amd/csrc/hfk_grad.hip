@@ -1,0 +1,119 @@
+// RHF nuclear gradient (host side).  The reference's
+// HartreeFock/Derivatives.hs only sketches derivative integrals and does
+// not compile, so this entry point has no reference counterpart; it
+// differentiates the energy the rest of the library computes:
+//   E = 1/2 sum D (H + F) + V_nn   (HartreeFock.hs:70-123, variationalE)
+//   dE/dR = sum D dH/dR + 1/2 sum Gamma d(ij|kl)/dR - sum W dS/dR + dV_nn/dR
+// with W = 1/2 D F D (= 2 sum_occ eps_i C_i C_i^T at convergence).
+#include <math.h>
+
+#include <vector>
+
+#include "hfk_host.hpp"
+
+static int grad_upload_maps(hfk_ctx* c)
+{
+    if (c->d_sh_atom) return HFK_OK;
+    const int natom = (int)c->Z.size();
+    std::vector<int> sa(c->shells.size());
+    for (size_t s = 0; s < c->shells.size(); ++s) {
+        sa[s] = c->shells[s].atom;
+        if (sa[s] < 0 || sa[s] >= natom)
+            return hfk_set_error(HFK_EINVAL, "hfk_gradient: shell %zu has atom %d (natom %d)", s, sa[s], natom);
+    }
+    for (int a : c->atom_of_bf)
+        if (a < 0 || a >= natom) return hfk_set_error(HFK_EINVAL, "hfk_gradient: atom_of_bf out of range");
+    HFK_TRY(dev_upload(c, &c->d_sh_atom, sa));
+    HFK_TRY(dev_upload(c, &c->d_atom_of_bf, c->atom_of_bf));
+    return HFK_OK;
+}
+
+// W = 1/2 D F D with F = H + G(D); D, W flat on device.
+static int grad_energy_weighted(hfk_ctx* c, const double* Dflat, double* Wflat)
+{
+    const int n = c->n;
+    const size_t nf = (size_t)n * (n + 1) / 2, nn = (size_t)n * n;
+    double *H = nullptr, *F = nullptr, *Df = nullptr, *Ff = nullptr, *T = nullptr;
+    int rc = dev_alloc(&H, nf);
+    if (rc == HFK_OK) rc = dev_alloc(&F, nf);
+    if (rc == HFK_OK) rc = dev_alloc(&Df, nn);
+    if (rc == HFK_OK) rc = dev_alloc(&Ff, nn);
+    if (rc == HFK_OK) rc = dev_alloc(&T, nn);
+    if (rc == HFK_OK) rc = hfk_ctx_hcore_dev(c, H);
+    if (rc == HFK_OK) rc = hfk_ctx_fock_device(c, H, Dflat, F);
+    if (rc == HFK_OK) {
+        hfk_launch_flat_to_full(n, Dflat, Df, c->stream);
+        hfk_launch_flat_to_full(n, F, Ff, c->stream);
+        hfk_launch_dgemm(n, n, n, 1.0, Ff, n, 0, Df, n, 0, 0.0, T, n, c->stream);    // F D
+        hfk_launch_dgemm(n, n, n, 0.5, Df, n, 0, T, n, 0, 0.0, Ff, n, c->stream);   // 1/2 D F D
+        hfk_launch_full_to_flat(n, Ff, Wflat, c->stream);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = hfk_set_error(HFK_EHIP, "hfk_gradient: energy-weighted density failed");
+    }
+    dev_free(c, T);
+    dev_free(c, Ff);
+    dev_free(c, Df);
+    dev_free(c, F);
+    dev_free(c, H);
+    return rc;
+}
+
+extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_flat, double* g)
+{
+    if (!c || !D_flat || !g) return hfk_set_error(HFK_EINVAL, "hfk_gradient: null argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const int n = c->n, natom = (int)c->Z.size();
+    if (natom == 0 || (int)c->atom_of_bf.size() != n)
+        return hfk_set_error(HFK_EINVAL, "hfk_gradient: basis needs natom, Z, R and atom_of_bf");
+    if ((size_t)3 * natom * sizeof(double) > 64 * 1024)
+        return hfk_set_error(HFK_EUNSUP, "hfk_gradient: %d atoms exceed the per-workgroup accumulator", natom);
+    HFK_TRY(grad_upload_maps(c));
+    const size_t nf = (size_t)n * (n + 1) / 2;
+    HFK_HIP(hipMemcpyAsync(c->d_flatA, D_flat, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+    if (W_flat) {
+        HFK_HIP(hipMemcpyAsync(c->d_flatB, W_flat, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+    } else {
+        HFK_TRY(grad_energy_weighted(c, c->d_flatA, c->d_flatB));
+    }
+    double* d_grad = nullptr;
+    HFK_TRY(dev_alloc(&d_grad, (size_t)3 * natom));
+    hfk_launch_fill(d_grad, 0.0, 3LL * natom, c->stream);
+    hfk_launch_flat_to_padded(n, c->d_flatA, c->d_D, c->stream);
+    QuartetArgs a = hfk_ctx_args(c);
+    a.sch_cut = c->opt.schwarz_cut;
+    a.D = c->d_D;
+    a.grad = d_grad;
+    a.sh_atom = c->d_sh_atom;
+    a.natom = natom;
+    a.grad_lds = 1;
+    int rc = HFK_OK;
+    for (const PlanEntry& e : c->plan) {
+        hfk_plan_args(c, e, a);   // FP32 entries select their quartets; all run in FP64 here
+        if (hfk_launch_grad(e.la, e.lb, e.lc, e.ld, c->opt.boys_mode, e.mode, a, c->stream)) {
+            rc = hfk_set_error(HFK_EUNSUP, "no gradient kernel for class (%d%d|%d%d)", e.la, e.lb, e.lc, e.ld);
+            break;
+        }
+    }
+    if (rc == HFK_OK) {
+        hfk_launch_grad_onee(hfk_onee_args(c), c->d_atom_of_bf, c->d_flatA, c->d_flatB, d_grad, c->stream);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(g, d_grad, sizeof(double) * 3 * natom, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = hfk_set_error(HFK_EHIP, "hfk_gradient: kernel or copy-out failed");
+    }
+    dev_free(c, d_grad);
+    if (rc != HFK_OK) return rc;
+    // nuclear repulsion: d/dR_A sum_{B<C} Z_B Z_C / |R_B - R_C|
+    for (int A = 0; A < natom; ++A)
+        for (int B = 0; B < natom; ++B) {
+            if (A == B) continue;
+            double d[3], r2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                d[k] = c->R[3 * A + k] - c->R[3 * B + k];
+                r2 += d[k] * d[k];
+            }
+            const double r = sqrt(r2), f = -c->Z[A] * c->Z[B] / (r2 * r);
+            for (int k = 0; k < 3; ++k) g[3 * A + k] += f * d[k];
+        }
+    return HFK_OK;
+}

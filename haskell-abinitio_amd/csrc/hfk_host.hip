@@ -789,7 +789,7 @@ extern "C" int hfk_get_stats(hfk_ctx* c, hfk_stats* st)
     return HFK_OK;
 }
 
-static OneEArgs onee_args(hfk_ctx* c)
+OneEArgs hfk_onee_args(hfk_ctx* c)
 {
     OneEArgs a;
     a.n = c->n;
@@ -808,7 +808,7 @@ static OneEArgs onee_args(hfk_ctx* c)
 
 int hfk_ctx_overlap_dev(hfk_ctx* c, double* S_dev)
 {
-    hfk_launch_overlap(onee_args(c), S_dev, c->stream);
+    hfk_launch_overlap(hfk_onee_args(c), S_dev, c->stream);
     HFK_HIP(hipGetLastError());
     return HFK_OK;
 }
@@ -816,7 +816,7 @@ int hfk_ctx_overlap_dev(hfk_ctx* c, double* S_dev)
 int hfk_ctx_hcore_dev(hfk_ctx* c, double* H_dev)
 {
     if (c->Z.empty()) return hfk_set_error(HFK_EINVAL, "hfk_hcore: basis has no nuclei (natom/Z/R)");
-    hfk_launch_hcore(onee_args(c), H_dev, c->stream);
+    hfk_launch_hcore(hfk_onee_args(c), H_dev, c->stream);
     HFK_HIP(hipGetLastError());
     return HFK_OK;
 }

@@ -83,6 +83,7 @@ struct hfk_ctx {
     double *d_pp = nullptr, *d_qcomp = nullptr, *d_pair_qhi = nullptr, *d_pair_sch = nullptr, *d_pair_diag = nullptr;
     double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr;
     double *d_flatA = nullptr, *d_flatB = nullptr, *d_flatC = nullptr;
+    int *d_sh_atom = nullptr, *d_atom_of_bf = nullptr;   // gradient (uploaded on first use)
     hfk_stats stats = {};
     std::vector<PlanEntry> plan;
     std::vector<hipEvent_t> events;
@@ -148,6 +149,7 @@ int hfk_ctx_collect_times(hfk_ctx* c);
 int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat);
 int hfk_ctx_overlap_dev(hfk_ctx* c, double* S_dev);
 int hfk_ctx_hcore_dev(hfk_ctx* c, double* H_dev);
+OneEArgs hfk_onee_args(hfk_ctx* c);
 
 // linear algebra helpers (hfk_linalg_host.hip)
 int hfk_linalg_init(hfk_ctx* c);

@@ -46,6 +46,11 @@ struct QuartetArgs {
     long long ntask;
     int lds_rows;               // K rows staged in LDS (0: global atomics)
     int block;                  // workgroup size of the tiled kernel (64 or 256)
+    // nuclear gradient
+    const int* sh_atom;         // [nshell] atom of each shell
+    double* grad;               // [natom][3] accumulator
+    int natom;
+    int grad_lds;               // per-workgroup atom accumulator in LDS (0: global atomics)
 };
 
 enum { HFK_OP_JK = 0, HFK_OP_FLAT = 1, HFK_OP_DIAG = 2 };
@@ -55,6 +60,10 @@ enum { HFK_OP_JK = 0, HFK_OP_FLAT = 1, HFK_OP_DIAG = 2 };
 // mode (J/K only): 0 main pass, 1 screen-correction pass.
 int hfk_launch_quartets(int la, int lb, int lc, int ld, int op, int prec, int boys, int mode,
                         const QuartetArgs& a, hipStream_t s);
+
+// Two-electron gradient contribution of one class over a plan entry's task list
+// (FP64), accumulated into a.grad.
+int hfk_launch_grad(int la, int lb, int lc, int ld, int boys, int mode, const QuartetArgs& a, hipStream_t s);
 
 // Setup kernels (hfk_setup.hip)
 struct ShellDev {
@@ -101,6 +110,9 @@ struct OneEArgs {
 };
 void hfk_launch_overlap(const OneEArgs& a, double* S_flat, hipStream_t s);
 void hfk_launch_hcore(const OneEArgs& a, double* H_flat, hipStream_t s);
+// One-electron gradient: sum D_ij dH_ij/dR - W_ij dS_ij/dR (flat D, W) into grad[natom][3].
+void hfk_launch_grad_onee(const OneEArgs& a, const int* atom_of_bf, const double* D_flat, const double* W_flat,
+                          double* grad, hipStream_t s);
 
 // Dense linear algebra (hfk_linalg.hip): row-major FP64, MFMA.
 // C = alpha * op(A) * op(B) + beta * C, with op = transpose when ta/tb.

@@ -424,12 +424,3 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
     work_free(W);
     return rc;
 }
-
-extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_flat, double* g)
-{
-    (void)D_flat;
-    (void)W_flat;
-    (void)g;
-    if (!c) return hfk_set_error(HFK_EINVAL, "hfk_gradient: null context");
-    return hfk_set_error(HFK_EUNSUP, "hfk_gradient: not built yet");
-}

@@ -178,7 +178,7 @@ __global__ void k_pair_setup(ShellDev sh, int npair, const int* pair_A, const in
             r[PP_OO2P] = 1.0 / (2.0 * pe);
             const int qoff = pair_q0[p] + k * na * nb;
             r[PP_QOFF] = (double)qoff;
-            r[PP_PAD] = 0.0;
+            r[PP_EA] = ea;
             double qlo = INFINITY, qhi = 0.0;
             for (int a = 0; a < na; ++a) {
                 int ea3[3];

@@ -187,11 +187,17 @@ class Context:
         check(_lib.lib().hfk_scf(self._h, C.byref(o), C.byref(r), _ptr(Cm), _ptr(eps), _ptr(D), _ptr(F)))
         return HFData(F, Cm, D, eps, r.energy, r.steps, r.max_error, r.seconds, r.fock_seconds)
 
-    def gradient(self, D_flat, W_flat):
+    def gradient(self, D_flat, W_flat=None):
+        """Nuclear gradient dE/dR (natom x 3, Eh/bohr) at density D_flat.
+        W_flat: energy-weighted density; None computes 1/2 D F D, which is
+        W at SCF convergence."""
         D = np.ascontiguousarray(D_flat, dtype=np.float64)
-        W = np.ascontiguousarray(W_flat, dtype=np.float64)
         g = np.zeros((self.mol.natom, 3))
-        check(_lib.lib().hfk_gradient(self._h, _ptr(D), _ptr(W), _ptr(g)))
+        if W_flat is None:
+            check(_lib.lib().hfk_gradient(self._h, _ptr(D), None, _ptr(g)))
+        else:
+            W = np.ascontiguousarray(W_flat, dtype=np.float64)
+            check(_lib.lib().hfk_gradient(self._h, _ptr(D), _ptr(W), _ptr(g)))
         return g
 
     def class_stats(self):
