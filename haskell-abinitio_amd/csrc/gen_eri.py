@@ -507,6 +507,21 @@ def main():
     qs = quartet_classes(lmax)
     for q in qs:
         tag = "%d%d%d%d" % q
+        if max(q) >= 2:
+            # classes with d shells: cooperative wave kernel (hfk_coop_impl.hpp)
+            body = ["// Generated by gen_eri.py -- do not edit.",
+                    "#include <hip/hip_runtime.h>",
+                    '#include "hfk_coop_impl.hpp"',
+                    "int hfk_dispatch_%s(int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag,
+                    "{ return dispatch_coop<%d, %d, %d, %d>(op, prec, boys, mode, a, s); }" % q,
+                    "int hfk_dispatch_grad_%s(int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag,
+                    "{ return dispatch_coop<%d, %d, %d, %d>(HFK_OP_GRAD, 0, boys, mode, a, s); }" % q]
+            path = os.path.join(outdir, "eri_%s.hip" % tag)
+            txt = "\n".join(body) + "\n"
+            if not os.path.exists(path) or open(path).read() != txt:
+                with open(path, "w") as f:
+                    f.write(txt)
+            continue
         src, _ = gen_class(*q)
         body = []
         body.append("// Generated by gen_eri.py -- do not edit.")

@@ -15,28 +15,9 @@
 
 #include <algorithm>
 
-#include "hfk_internal.hpp"
+#include "hfk_quartet_common.hpp"
 
 namespace {
-
-constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
-
-// Decode linear quartet index -> (bra list position i, ket list position j).
-__device__ __forceinline__ void decode_quartet(const QuartetArgs& a, long long q, long long& i, long long& j)
-{
-    if (a.same) {
-        // q = i(i+1)/2 + j, 0 <= j <= i
-        double x = sqrt(8.0 * (double)q + 1.0);
-        long long ii = (long long)((x - 1.0) * 0.5);
-        while (ii * (ii + 1) / 2 > q) --ii;
-        while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
-        i = ii;
-        j = q - ii * (ii + 1) / 2;
-    } else {
-        i = q / a.nket;
-        j = q - i * a.nket;
-    }
-}
 
 // Primitive-pair sub-ranges [b0,b1) of the bra and [k0,k1) of the ket
 // (b1/k1 < 0: to the end).
@@ -49,36 +30,6 @@ __device__ __forceinline__ void eval_quartet(const QuartetArgs& a, int bp, int k
     const int nbp = (b1 < 0 ? a.pair_npp[bp] : b1) - b0;
     const int nkp = (k1 < 0 ? a.pair_npp[kp] : k1) - k0;
     eri_class<LA, LB, LC, LD, T, BOYS, MODE>(bra, nbp, ket, nkp, a.qcomp, a.bt, V, a.qtau);
-}
-
-__device__ __forceinline__ bool quartet_selected(const QuartetArgs& a, int bp, int kp)
-{
-    if (a.qtau > 0.0 && a.pair_qhi[bp] * a.pair_qhi[kp] < a.qtau) return false;
-    if (a.sch_cut > 0.0 || a.prec_sel != 0) {
-        const double bound = a.pair_sch[bp] * a.pair_sch[kp];
-        if (bound < a.sch_cut) return false;
-        if (a.prec_sel == 1 && !(bound >= a.fp32_thr)) return false;
-        if (a.prec_sel == 2 && !(bound < a.fp32_thr)) return false;
-    }
-    return true;
-}
-
-// Canonical position of a function quartet in calcIntegrals' flat array
-// (HartreeFock.hs:153-181): sortKeys, then the enumeration rank.
-__device__ __forceinline__ long long flat_pos(const QuartetArgs& a, int i, int j, int k, int l)
-{
-    int a0 = min(i, j), a1 = max(i, j), b0 = min(k, l), b1 = max(k, l);
-    if (!(a0 < b0 || (a0 == b0 && a1 <= b1))) {
-        int t0 = a0, t1 = a1;
-        a0 = b0; a1 = b1; b0 = t0; b1 = t1;
-    }
-    const long long n = a.n;
-    long long r = a.flat_base[(size_t)a0 * n + a1];
-    if (b0 == a0) return r + (b1 - a1);
-    r += n - a1;
-    const long long cnt = (long long)b0 - 1 - a0;   // sum_{k'=a0+1}^{b0-1} (n - k')
-    r += cnt * n - ((long long)(b0 - 1) * b0 / 2 - (long long)a0 * (a0 + 1) / 2);
-    return r + (b1 - b0);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -181,8 +132,6 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 // ---------------------------------------------------------------------
 // Workgroups are 64 lanes, or 256 lanes (4 waves sharing one set of staged K
 // rows) when the rows are long enough to limit occupancy through LDS.
-constexpr int WAVE = 64;
-
 template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
 __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
 {

@@ -487,15 +487,22 @@ int hfk_ctx_plan(hfk_ctx* c)
         if (c->class_count[kb] == 0) continue;
         for (int kk = 0; kk <= kb; ++kk) {
             if (c->class_count[kk] == 0) continue;
+            int la, lb, lc, ld;
+            hfk_pair_class_l(kb, &la, &lb);
+            hfk_pair_class_l(kk, &lc, &ld);
+            const bool coop = hfk_class_coop(la, lb, lc, ld);   // FP64 only
+            const bool split = c->opt.fp32_threshold > 0.0 && !coop;
             for (int prec = 0; prec < 2; ++prec) {
-                if (prec == 1 && !(c->opt.fp32_threshold > 0.0)) continue;
+                if (prec == 1 && !split) continue;
                 PlanEntry e = {};
                 e.kb = kb;
                 e.kk = kk;
-                hfk_pair_class_l(kb, &e.la, &e.lb);
-                hfk_pair_class_l(kk, &e.lc, &e.ld);
+                e.la = la;
+                e.lb = lb;
+                e.lc = lc;
+                e.ld = ld;
                 e.prec = prec;
-                e.prec_sel = (c->opt.fp32_threshold > 0.0) ? prec + 1 : 0;
+                e.prec_sel = split ? prec + 1 : 0;
                 const long long nb = c->class_count[kb], nk = c->class_count[kk];
                 e.nquart = (kb == kk) ? nb * (nb + 1) / 2 : nb * nk;
                 e.mode = ref ? 1 : 0;   // 1: kernels apply the reference screen
@@ -534,8 +541,11 @@ int hfk_ctx_plan(hfk_ctx* c)
     // primitive-pair-count run; those prefixes are concatenated into an
     // explicit ket array and cut into tasks of `chunk` kets, so the lanes of
     // a workgroup are busy and mostly do equal work.
-    const int chunk = c->n >= 200 ? 256 : 64;   // one ket per lane of the workgroup
     for (PlanEntry& e : c->plan) {
+        const bool coop = hfk_class_coop(e.la, e.lb, e.lc, e.ld);
+        // per-lane kernel: one ket per lane of the workgroup; cooperative kernel:
+        // 64 / (na nb) kets at a time, four rounds per task
+        const int chunk = coop ? 4 * (64 / (ncart(e.la) * ncart(e.lb))) : (c->n >= 200 ? 256 : 64);
         const std::vector<int>& bl = c->h_class[e.kb];
         const std::vector<int>& kl = c->h_class[e.kk];
         std::vector<double> kq(kl.size());
@@ -578,7 +588,7 @@ int hfk_ctx_plan(hfk_ctx* c)
                 const long long floor_l = std::max(4, 32 >> L);   // cheap classes: amortise the digestion
                 const int per_lane = (int)std::max(floor_l, std::min(128LL, e.prim_q / 262144));
                 int bstep = nbp, kstep = nkp;
-                if (nbp * nkp > per_lane) {
+                if (!coop && nbp * nkp > per_lane) {
                     bstep = std::max(1, std::min(nbp, per_lane));
                     kstep = std::max(1, std::min(nkp, per_lane / bstep));
                 }
@@ -616,10 +626,12 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
     a.tasks = e.d_tasks;
     a.ket_idx = e.d_kets;
     a.ntask = e.ntask;
-    a.block = c->n >= 200 ? 256 : 64;
+    a.block = (c->n >= 200 && !hfk_class_coop(e.la, e.lb, e.lc, e.ld)) ? 256 : 64;
     {
         const int nr = (e.la + 1) * (e.la + 2) / 2 + (e.lb + 1) * (e.lb + 2) / 2;
-        a.lds_rows = ((size_t)nr * c->n * sizeof(double) <= 64 * 1024) ? 1 : 0;
+        // the cooperative kernel also holds ~12 KB of static LDS
+        const size_t cap = hfk_class_coop(e.la, e.lb, e.lc, e.ld) ? 48 * 1024 : 64 * 1024;
+        a.lds_rows = ((size_t)nr * (c->n + 1) * sizeof(double) <= cap) ? 1 : 0;
     }
     a.bra_list = c->d_class[e.kb];
     a.ket_list = c->d_class[e.kk];

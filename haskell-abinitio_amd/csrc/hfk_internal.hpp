@@ -53,7 +53,11 @@ struct QuartetArgs {
     int grad_lds;               // per-workgroup atom accumulator in LDS (0: global atomics)
 };
 
-enum { HFK_OP_JK = 0, HFK_OP_FLAT = 1, HFK_OP_DIAG = 2 };
+enum { HFK_OP_JK = 0, HFK_OP_FLAT = 1, HFK_OP_DIAG = 2, HFK_OP_GRAD = 3 };
+
+// Classes with a d shell run on the cooperative wave kernel (64-lane
+// workgroups, FP64, whole quartets per task).
+inline bool hfk_class_coop(int la, int lb, int lc, int ld) { return la >= 2 || lb >= 2 || lc >= 2 || ld >= 2; }
 
 // Launch one class kernel.  classes are (la,lb,lc,ld) with la>=lb, lc>=ld,
 // (la,lb) >= (lc,ld).  prec: 0 = FP64, 1 = FP32.  boys: 0 reference, 1 accurate.
