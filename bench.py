@@ -99,6 +99,44 @@ def mixed_vs_fp64(workload, thr):
     return res
 
 
+def gradient_bench(workload, reps=3):
+    """Nuclear gradient wall time (hfk_gradient: derivative ERIs, one-electron
+    and Pulay terms, W = 1/2 DFD) at the converged FP64 density, and the same
+    with mixed-precision SCF density vs FP64."""
+    import numpy as np
+    import hfk
+    from hfk import systems
+    if workload == "c12h26":
+        mol = systems.alkane(12, "6-31g*")
+        desc = "C12H26 RHF/6-31G* (Cartesian d), 38 atoms, 232 basis functions"
+    else:
+        nwat, desc = WORKLOADS[workload]
+        mol = systems.water_cluster(nwat, "6-31g")
+    res = {"workload": desc, "natom": mol.natom, "nbf": mol.nbf}
+    grads = {}
+    for name, opts in (("fp64", hfk.options()), ("mixed", hfk.options(fp32_threshold=1e-4))):
+        t0 = time.perf_counter()
+        with hfk.Context(mol, opts) as ctx:
+            t_ctx = time.perf_counter() - t0
+            r = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60)
+            ms = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                g = ctx.gradient(r.getDensity)
+                ms.append((time.perf_counter() - t) * 1e3)
+            ctx.calcGmatrix(r.getDensity)
+            fock_ms = ctx.stats()["last_eri_ms"]
+        grads[name] = (r, g)
+        res[name] = {"energy": r.getEnergy, "steps": r.steps, "scf_wall_s": r.seconds + t_ctx,
+                     "fock_ms": fock_ms, "gradient_ms": min(ms), "max_abs_grad": float(np.max(np.abs(g))),
+                     "translation_residual": float(np.max(np.abs(g.sum(axis=0))))}
+        log("bench: gradient %s %s E=%.10f scf %.2fs grad %.1fms" % (workload, name, r.getEnergy,
+                                                                     r.seconds + t_ctx, min(ms)))
+    res["dE"] = grads["mixed"][0].getEnergy - grads["fp64"][0].getEnergy
+    res["max_dgrad"] = float(np.max(np.abs(grads["mixed"][1] - grads["fp64"][1])))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +148,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-workload", default="h2o32", choices=sorted(WORKLOADS))
     ap.add_argument("--mixed-thr", type=float, default=1e-4)
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--grad-workloads", default="h2o16,c12h26",
+                    help="comma list of gradient/SCF workloads (h2oN or c12h26)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before timing, run one device copy of this size (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -227,13 +268,15 @@ def main():
         "classes": classes,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "kernel": "k_quartets %s %s (ERI + fused J/K)" % (dom["class"], dom["prec"]),
+                     "kernel": "k_jk_tiles %s %s (ERI + fused J/K)" % (dom["class"], dom["prec"]),
                      "note": "FP64 vector-ALU kernel; peak is MI355X FP64 (vector = matrix figure)"},
         "scf": scf,
     }
     ctx.close()
     if not args.no_mixed:
         out["mixed_vs_fp64"] = mixed_vs_fp64(args.mixed_workload, args.mixed_thr)
+    if not args.no_grad:
+        out["gradient"] = [gradient_bench(w) for w in args.grad_workloads.split(",") if w]
     if not args.no_cpu:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(mol)

@@ -484,6 +484,13 @@ def gen_grad_class(la, lb, lc, ld):
     return "\n".join(out)
 
 
+# Classes holding more contracted integrals per quartet than this run on
+# the cooperative wave kernel (per-lane registers cannot hold them); the
+# host's hfk_class_coop() uses the same bounds.
+COOP_NV_JK = 54
+COOP_NV_GRAD = 18
+
+
 def pair_classes(lmax):
     return [(a, b) for a in range(lmax + 1) for b in range(a + 1)]
 
@@ -507,15 +514,18 @@ def main():
     qs = quartet_classes(lmax)
     for q in qs:
         tag = "%d%d%d%d" % q
-        if max(q) >= 2:
-            # classes with d shells: cooperative wave kernel (hfk_coop_impl.hpp)
-            body = ["// Generated by gen_eri.py -- do not edit.",
-                    "#include <hip/hip_runtime.h>",
-                    '#include "hfk_coop_impl.hpp"',
-                    "int hfk_dispatch_%s(int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag,
-                    "{ return dispatch_coop<%d, %d, %d, %d>(op, prec, boys, mode, a, s); }" % q,
-                    "int hfk_dispatch_grad_%s(int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag,
-                    "{ return dispatch_coop<%d, %d, %d, %d>(HFK_OP_GRAD, 0, boys, mode, a, s); }" % q]
+        nv = 1
+        for l in q:
+            nv *= (l + 1) * (l + 2) // 2
+        jk_coop = nv > COOP_NV_JK          # cooperative wave kernel (hfk_coop_impl.hpp)
+        grad_coop = max(q) >= 2 and nv > COOP_NV_GRAD
+        body = ["// Generated by gen_eri.py -- do not edit.", "#include <hip/hip_runtime.h>"]
+        if jk_coop:
+            body += ['#include "hfk_coop_impl.hpp"',
+                     "int hfk_dispatch_%s(int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag,
+                     "{ return dispatch_coop<%d, %d, %d, %d>(op, prec, boys, mode, a, s); }" % q,
+                     "int hfk_dispatch_grad_%s(int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag,
+                     "{ return dispatch_coop<%d, %d, %d, %d>(HFK_OP_GRAD, 0, boys, mode, a, s); }" % q]
             path = os.path.join(outdir, "eri_%s.hip" % tag)
             txt = "\n".join(body) + "\n"
             if not os.path.exists(path) or open(path).read() != txt:
@@ -523,9 +533,6 @@ def main():
                     f.write(txt)
             continue
         src, _ = gen_class(*q)
-        body = []
-        body.append("// Generated by gen_eri.py -- do not edit.")
-        body.append("#include <hip/hip_runtime.h>")
         body.append('#include "hfk_internal.hpp"')
         body.append("namespace {")
         body.append(src)
@@ -533,16 +540,22 @@ def main():
         body.append("__device__ __forceinline__ void eri_class(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp,")
         body.append("    const double* __restrict__ qcomp, const BoysTables& bt, T* __restrict__ V, double qtau)")
         body.append("{ eri_%s<T, BOYS, MODE>(bra, nbp, ket, nkp, qcomp, bt, V, qtau); }" % tag)
-        body.append(gen_grad_class(*q))
-        body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>")
-        body.append("__device__ __forceinline__ void grad_class(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp,")
-        body.append("    const double* __restrict__ qcomp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
-        body.append("{ grad_%s<T, BOYS, MODE>(bra, nbp, ket, nkp, qcomp, bt, G, qtau, g); }" % tag)
+        if not grad_coop:
+            body.append(gen_grad_class(*q))
+            body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>")
+            body.append("__device__ __forceinline__ void grad_class(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp,")
+            body.append("    const double* __restrict__ qcomp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
+            body.append("{ grad_%s<T, BOYS, MODE>(bra, nbp, ket, nkp, qcomp, bt, G, qtau, g); }" % tag)
         body.append("}  // namespace")
         body.append('#include "hfk_eri_impl.hpp"')
-        body.append('#include "hfk_grad_impl.hpp"')
-        body.append("int hfk_dispatch_grad_%s(int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag)
-        body.append("{ return dispatch_grad<%d, %d, %d, %d>(boys, mode, a, s); }" % q)
+        if grad_coop:
+            body.append('#include "hfk_coop_impl.hpp"')
+            body.append("int hfk_dispatch_grad_%s(int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag)
+            body.append("{ return dispatch_coop<%d, %d, %d, %d>(HFK_OP_GRAD, 0, boys, mode, a, s); }" % q)
+        else:
+            body.append('#include "hfk_grad_impl.hpp"')
+            body.append("int hfk_dispatch_grad_%s(int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag)
+            body.append("{ return dispatch_grad<%d, %d, %d, %d>(boys, mode, a, s); }" % q)
         body.append("int hfk_dispatch_%s(int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag)
         body.append("{ return dispatch<%d, %d, %d, %d>(op, prec, boys, mode, a, s); }" % q)
         path = os.path.join(outdir, "eri_%s.hip" % tag)

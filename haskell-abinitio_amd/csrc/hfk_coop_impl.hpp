@@ -60,32 +60,70 @@ __host__ __device__ constexpr int cart_e(int l, int k, int axis)
 
 // E^{ij}_t for i <= IM, j <= JM at [(i*(JM+1)+j)*(IM+JM+1)+t]; Helgaker's
 // recursion with E^{00}_0 = 1 (the Gaussian product factor is in PP_K).
+// Built in registers (fully unrolled), then stored.
 template <int IM, int JM>
-__device__ void herm_table(double PA, double PB, double oo2p, double* E)
+__device__ __forceinline__ void herm_table(double PA, double PB, double oo2p, double* E)
 {
-    constexpr int TS = IM + JM + 1;
+    constexpr int TS = IM + JM + 1, SZ = (IM + 1) * (JM + 1) * TS;
+    double e[SZ];
+#pragma unroll
     for (int i = 0; i <= IM; ++i)
-        for (int j = 0; j <= JM; ++j) {
-            double* e = E + (i * (JM + 1) + j) * TS;
-            if (i == 0 && j == 0) {
-                e[0] = 1.0;
-                for (int t = 1; t < TS; ++t) e[t] = 0.0;
-                continue;
-            }
-            const bool up_i = i > 0;
-            const double* s = up_i ? E + ((i - 1) * (JM + 1) + j) * TS : E + (i * (JM + 1) + j - 1) * TS;
-            const double X = up_i ? PA : PB;
-            const int smax = i + j - 1;
+#pragma unroll
+        for (int j = 0; j <= JM; ++j)
+#pragma unroll
             for (int t = 0; t < TS; ++t) {
                 double v = 0.0;
-                if (t <= smax + 1) {
-                    if (t >= 1) v += oo2p * s[t - 1];
-                    if (t <= smax) v += X * s[t];
-                    if (t + 1 <= smax) v += (double)(t + 1) * s[t + 1];
+                if (i == 0 && j == 0) {
+                    v = (t == 0) ? 1.0 : 0.0;
+                } else {
+                    const int si = i > 0 ? i - 1 : i, sj = i > 0 ? j : j - 1;
+                    const double X = i > 0 ? PA : PB;
+                    const int smax = si + sj;
+                    const double* src = e + (si * (JM + 1) + sj) * TS;
+                    if (t <= smax + 1) {
+                        if (t >= 1) v += oo2p * src[t - 1];
+                        if (t <= smax) v += X * src[t];
+                        if (t + 1 <= smax) v += (double)(t + 1) * src[t + 1];
+                    }
                 }
-                e[t] = v;
+                e[(i * (JM + 1) + j) * TS + t] = v;
             }
-        }
+#pragma unroll
+    for (int k = 0; k < SZ; ++k) E[k] = e[k];
+}
+
+// R_tuv (t+u+v <= LR) in registers from the scaled Boys values
+// F[n] = pref (-2 alpha)^n F_n(T): R^n_{t..} = (t-1) R^{n+1}_{t-2..} + X R^{n+1}_{t-1..}
+// (then u, then v), IntegralsEvaluation.hs:430-449.
+template <int LR>
+__device__ __forceinline__ void herm_R_regs(const double* F, double X, double Y, double Z, double* R)
+{
+    double cur[nherm(LR)];
+    cur[0] = F[LR];
+#pragma unroll
+    for (int n = LR - 1; n >= 0; --n) {
+        double nxt[nherm(LR)];
+        const int top = LR - n;
+#pragma unroll
+        for (int t = 0; t <= LR; ++t)
+#pragma unroll
+            for (int u = 0; u <= LR - t; ++u)
+#pragma unroll
+                for (int v = 0; v <= LR - t - u; ++v) {
+                    if (t + u + v > top) continue;
+                    double val;
+                    if (t > 0) val = (t >= 2 ? (double)(t - 1) * cur[hidx(t - 2, u, v)] : 0.0) + X * cur[hidx(t - 1, u, v)];
+                    else if (u > 0) val = (u >= 2 ? (double)(u - 1) * cur[hidx(t, u - 2, v)] : 0.0) + Y * cur[hidx(t, u - 1, v)];
+                    else if (v > 0) val = (v >= 2 ? (double)(v - 1) * cur[hidx(t, u, v - 2)] : 0.0) + Z * cur[hidx(t, u, v - 1)];
+                    else val = F[n];
+                    nxt[hidx(t, u, v)] = val;
+                }
+#pragma unroll
+        for (int e = 0; e < nherm(LR); ++e)
+            if (e < nherm(top)) cur[e] = nxt[e];
+    }
+#pragma unroll
+    for (int e = 0; e < nherm(LR); ++e) R[e] = cur[e];
 }
 
 template <int LA, int LB, int LC, int LD, int OP>
@@ -105,6 +143,7 @@ struct CoopShape {
     static constexpr int EC_SZ = EC_I * EC_J * EC_T;
     static constexpr int NBP_RAW = 768 / (3 * EB_SZ);
     static constexpr int NBP = NBP_RAW < 1 ? 1 : (NBP_RAW > 16 ? 16 : NBP_RAW);   // bra primitive pairs staged at once
+    static constexpr bool REG = LR <= 6;   // R_tuv in lane registers (else built level by level in LDS)
 };
 
 // LDS of one workgroup
@@ -119,17 +158,40 @@ struct CoopLds {
     double jcd[SH::S][SH::NKET];        // J'_cd partial sums
 };
 
+// Stage the Hermite tables and scalars of bra primitive pairs [bc0, bc0+nbc).
+template <class SH>
+__device__ __forceinline__ void coop_stage_bra(CoopLds<SH>& sm, const double* bra0, int bc0, int nbc)
+{
+    const int lane = threadIdx.x;
+    if (lane < 3 * nbc) {
+        const int ip = lane / 3, ax = lane - 3 * ip;
+        const double* r = bra0 + (size_t)(bc0 + ip) * PP_STRIDE;
+        herm_table<SH::EB_I - 1, SH::EB_J - 1>(r[PP_AX + ax], r[PP_BX + ax], r[PP_OO2P], &sm.eb[ip][ax][0]);
+    }
+    if (lane < nbc) {
+        const double* r = bra0 + (size_t)(bc0 + lane) * PP_STRIDE;
+        sm.bs[lane][0] = r[PP_PX];
+        sm.bs[lane][1] = r[PP_PY];
+        sm.bs[lane][2] = r[PP_PZ];
+        sm.bs[lane][3] = r[PP_P];
+        sm.bs[lane][4] = r[PP_K];
+        sm.bs[lane][5] = r[PP_EA];
+        sm.bs[lane][6] = r[PP_QOFF];
+    }
+}
+
 // One group of quartets (bra pair bp, this slot's ket kp or -1) over the
 // primitive ranges [b0,b1) x [k0,k1).  OP != GRAD: V[cd] of the lane's bra
 // component.  GRAD: ga, gb, gc (d/dA, d/dB, d/dC) of sum_cd Gw[cd] (ab|cd).
+// staged: the bra tables of [b0,b1) (at most NBP pairs) are already in LDS.
 template <int LA, int LB, int LC, int LD, int BOYS, int MODE, int OP>
 __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopShape<LA, LB, LC, LD, OP>>& sm, int bp,
-                                           int kp, int b0, int b1, int k0, int k1, double* V, const double* Gw,
-                                           double* ga, double* gb, double* gc)
+                                           int kp, int b0, int b1, int k0, int k1, bool staged, double* V,
+                                           const double* Gw, double* ga, double* gb, double* gc)
 {
     using SH = CoopShape<LA, LB, LC, LD, OP>;
     constexpr int NB = SH::NB, ND = SH::ND, NBRA = SH::NBRA, NKET = SH::NKET;
-    constexpr int LR = SH::LR, G = SH::G;
+    constexpr int LR = SH::LR, G = SH::G, LAB = SH::LAB, LCD = SH::LCD;
     const int lane = threadIdx.x;
     const int slot = lane / NBRA, li = lane - slot * NBRA;
     const bool active = slot < SH::S;
@@ -167,21 +229,9 @@ __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopSha
     const double* ket0 = has ? a.pp + (size_t)a.pair_pp0[kp] * PP_STRIDE : nullptr;
     for (int bc0 = b0; bc0 < b1; bc0 += SH::NBP) {
         const int nbc = min(SH::NBP, b1 - bc0);
-        __syncthreads();
-        if (lane < 3 * nbc) {
-            const int ip = lane / 3, ax = lane - 3 * ip;
-            const double* r = bra0 + (size_t)(bc0 + ip) * PP_STRIDE;
-            herm_table<SH::EB_I - 1, SH::EB_J - 1>(r[PP_AX + ax], r[PP_BX + ax], r[PP_OO2P], &sm.eb[ip][ax][0]);
-        }
-        if (lane < nbc) {
-            const double* r = bra0 + (size_t)(bc0 + lane) * PP_STRIDE;
-            sm.bs[lane][0] = r[PP_PX];
-            sm.bs[lane][1] = r[PP_PY];
-            sm.bs[lane][2] = r[PP_PZ];
-            sm.bs[lane][3] = r[PP_P];
-            sm.bs[lane][4] = r[PP_K];
-            sm.bs[lane][5] = r[PP_EA];
-            sm.bs[lane][6] = r[PP_QOFF];
+        if (!staged) {
+            __syncthreads();
+            coop_stage_bra(sm, bra0, bc0, nbc);
         }
         for (int ik = k0; ik < k1; ++ik) {
             __syncthreads();
@@ -204,6 +254,184 @@ __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopSha
             }
             __syncthreads();
             for (int ib = 0; ib < nbc; ++ib) {
+                if constexpr (SH::REG) {
+                    if (has) {
+                        const double* Eb = &sm.eb[ib][0][0];
+                        const double* Ec = &sm.ec[slot][0][0];
+                        auto EB = [&](int axis, int i, int j, int t) -> double {
+                            return Eb[axis * SH::EB_SZ + (i * SH::EB_J + j) * SH::EB_T + t];
+                        };
+                        auto EC = [&](int axis, int i, int j, int t) -> double {
+                            return Ec[axis * SH::EC_SZ + (i * SH::EC_J + j) * SH::EC_T + t];
+                        };
+                        double R[SH::NH];
+                        {
+                            double F[LR + 1];
+                            const double p = sm.bs[ib][3], q = sm.ks[slot][3];
+                            const double pq = p + q, alpha = p * q / pq;
+                            const double X = sm.bs[ib][0] - sm.ks[slot][0];
+                            const double Y = sm.bs[ib][1] - sm.ks[slot][1];
+                            const double Z = sm.bs[ib][2] - sm.ks[slot][2];
+                            boys_eval<double, LR, BOYS>(alpha * ((X * X + Y * Y) + Z * Z), a.bt, F);
+                            double w = HFK_TWO_PI_2_5 * sm.bs[ib][4] * sm.ks[slot][4] / sqrt(pq);
+#pragma unroll
+                            for (int n = 0; n <= LR; ++n) {
+                                F[n] *= w;
+                                w *= -2.0 * alpha;
+                            }
+                            herm_R_regs<LR>(F, X, Y, Z, R);
+                        }
+                        // the lane's bra component, dense over t+u+v <= LAB (+1): zero beyond a+b
+                        constexpr int LE = LAB + 1 + G;
+                        double e1[3][LE];
+#pragma unroll
+                        for (int x = 0; x < 3; ++x)
+#pragma unroll
+                            for (int t = 0; t < LE; ++t) e1[x][t] = (t < SH::EB_T) ? EB(x, ea[x], eb[x], t) : 0.0;
+                        // X_{tau nu phi} = sum_tuv E^{ab}_tuv R_{t+tau,u+nu,v+phi}, tau+nu+phi <= LCD+G
+                        double Xv[SH::NX];
+#pragma unroll
+                        for (int e = 0; e < SH::NX; ++e) Xv[e] = 0.0;
+#pragma unroll
+                        for (int t = 0; t <= LAB; ++t)
+#pragma unroll
+                            for (int u = 0; u <= LAB - t; ++u)
+#pragma unroll
+                                for (int v = 0; v <= LAB - t - u; ++v) {
+                                    const double c = (e1[0][t] * e1[1][u]) * e1[2][v];
+#pragma unroll
+                                    for (int tau = 0; tau <= LCD + G; ++tau)
+#pragma unroll
+                                        for (int nu = 0; nu <= LCD + G - tau; ++nu)
+#pragma unroll
+                                            for (int ph = 0; ph <= LCD + G - tau - nu; ++ph)
+                                                Xv[hidx(tau, nu, ph)] += c * R[hidx(t + tau, u + nu, v + ph)];
+                                }
+                        double bq = 0.0;
+                        if constexpr (MODE == 1) bq = a.qcomp[(int)sm.bs[ib][6] + ab];
+                        if constexpr (OP != HFK_OP_GRAD) {
+#pragma unroll
+                            for (int cd = 0; cd < NKET; ++cd) {
+                                const int c = cd / ND, d = cd - (cd / ND) * ND;
+                                const int cx = cart_e(LC, c, 0), cy = cart_e(LC, c, 1), cz = cart_e(LC, c, 2);
+                                const int dx = cart_e(LD, d, 0), dy = cart_e(LD, d, 1), dz = cart_e(LD, d, 2);
+                                double s = 0.0;
+#pragma unroll
+                                for (int tau = 0; tau <= cx + dx; ++tau)
+#pragma unroll
+                                    for (int nu = 0; nu <= cy + dy; ++nu) {
+                                        const double exy = EC(0, cx, dx, tau) * EC(1, cy, dy, nu);
+#pragma unroll
+                                        for (int ph = 0; ph <= cz + dz; ++ph) {
+                                            const double term = exy * EC(2, cz, dz, ph) * Xv[hidx(tau, nu, ph)];
+                                            s += ((tau + nu + ph) & 1) ? -term : term;
+                                        }
+                                    }
+                                if constexpr (MODE == 1) V[cd] += !(bq * sm.kq[slot][cd] < a.qtau) ? s : 0.0;
+                                else V[cd] += s;
+                            }
+                        } else {
+                            double Zv[SH::NZ];
+#pragma unroll
+                            for (int e = 0; e < SH::NZ; ++e) Zv[e] = 0.0;
+                            const double gam2 = 2.0 * sm.ks[slot][5];
+#pragma unroll
+                            for (int cd = 0; cd < NKET; ++cd) {
+                                const int c = cd / ND, d = cd - (cd / ND) * ND;
+                                const int cc[3] = {cart_e(LC, c, 0), cart_e(LC, c, 1), cart_e(LC, c, 2)};
+                                const int dd[3] = {cart_e(LD, d, 0), cart_e(LD, d, 1), cart_e(LD, d, 2)};
+                                double w = Gw[cd];
+                                if constexpr (MODE == 1) w = !(bq * sm.kq[slot][cd] < a.qtau) ? w : 0.0;
+#pragma unroll
+                                for (int tau = 0; tau <= cc[0] + dd[0]; ++tau)
+#pragma unroll
+                                    for (int nu = 0; nu <= cc[1] + dd[1]; ++nu)
+#pragma unroll
+                                        for (int ph = 0; ph <= cc[2] + dd[2]; ++ph) {
+                                            const double e = EC(0, cc[0], dd[0], tau) * EC(1, cc[1], dd[1], nu) * EC(2, cc[2], dd[2], ph);
+                                            Zv[hidx(tau, nu, ph)] += ((tau + nu + ph) & 1) ? -w * e : w * e;
+                                        }
+#pragma unroll
+                                for (int k = 0; k < 3; ++k) {
+                                    double sacc = 0.0;
+#pragma unroll
+                                    for (int tau = 0; tau <= cc[0] + dd[0] + (k == 0); ++tau)
+#pragma unroll
+                                        for (int nu = 0; nu <= cc[1] + dd[1] + (k == 1); ++nu)
+#pragma unroll
+                                            for (int ph = 0; ph <= cc[2] + dd[2] + (k == 2); ++ph) {
+                                                const int idx[3] = {tau, nu, ph};
+                                                double f = 1.0;
+#pragma unroll
+                                                for (int x = 0; x < 3; ++x) {
+                                                    if (x == k) {
+                                                        double dv = gam2 * EC(x, cc[x] + 1, dd[x], idx[x]);
+                                                        if (cc[x] > 0 && idx[x] <= cc[x] - 1 + dd[x])
+                                                            dv -= (double)cc[x] * EC(x, cc[x] - 1, dd[x], idx[x]);
+                                                        f *= dv;
+                                                    } else {
+                                                        f *= EC(x, cc[x], dd[x], idx[x]);
+                                                    }
+                                                }
+                                                const double term = f * Xv[hidx(tau, nu, ph)];
+                                                sacc += ((tau + nu + ph) & 1) ? -term : term;
+                                            }
+                                    gc[k] += w * sacc;
+                                }
+                            }
+                            // Y_tuv = sum_{tau nu phi} Z R_{t+tau,...}, t+u+v <= LAB+1
+                            double Yv[nherm(LAB + 1)];
+#pragma unroll
+                            for (int t = 0; t <= LAB + 1; ++t)
+#pragma unroll
+                                for (int u = 0; u <= LAB + 1 - t; ++u)
+#pragma unroll
+                                    for (int v = 0; v <= LAB + 1 - t - u; ++v) {
+                                        double y = 0.0;
+#pragma unroll
+                                        for (int tau = 0; tau <= LCD; ++tau)
+#pragma unroll
+                                            for (int nu = 0; nu <= LCD - tau; ++nu)
+#pragma unroll
+                                                for (int ph = 0; ph <= LCD - tau - nu; ++ph)
+                                                    y += Zv[hidx(tau, nu, ph)] * R[hidx(t + tau, u + nu, v + ph)];
+                                        Yv[hidx(t, u, v)] = y;
+                                    }
+                            // 1-D derivative coefficients of the lane's component
+                            const double alp2 = 2.0 * sm.bs[ib][5];
+                            const double bet2 = 2.0 * (sm.bs[ib][3] - sm.bs[ib][5]);
+                            double dA1[3][LE], dB1[3][LE];
+#pragma unroll
+                            for (int x = 0; x < 3; ++x) {
+                                const int i = ea[x], j = eb[x];
+#pragma unroll
+                                for (int t = 0; t < LE; ++t) {
+                                    double va = alp2 * EB(x, i + 1, j, t);
+                                    if (i > 0) va -= (double)i * EB(x, i - 1, j, t);
+                                    double vb = bet2 * EB(x, i, j + 1, t);
+                                    if (j > 0) vb -= (double)j * EB(x, i, j - 1, t);
+                                    dA1[x][t] = va;
+                                    dB1[x][t] = vb;
+                                }
+                            }
+#pragma unroll
+                            for (int t = 0; t <= LAB + 1; ++t)
+#pragma unroll
+                                for (int u = 0; u <= LAB + 1 - t; ++u)
+#pragma unroll
+                                    for (int v = 0; v <= LAB + 1 - t - u; ++v) {
+                                        const double y = Yv[hidx(t, u, v)];
+                                        ga[0] += dA1[0][t] * e1[1][u] * e1[2][v] * y;
+                                        ga[1] += e1[0][t] * dA1[1][u] * e1[2][v] * y;
+                                        ga[2] += e1[0][t] * e1[1][u] * dA1[2][v] * y;
+                                        gb[0] += dB1[0][t] * e1[1][u] * e1[2][v] * y;
+                                        gb[1] += e1[0][t] * dB1[1][u] * e1[2][v] * y;
+                                        gb[2] += e1[0][t] * e1[1][u] * dB1[2][v] * y;
+                                    }
+                        }
+                    }
+                } else {
+                {
                 // Boys values and R^n_000, then R_tuv level by level
                 double F[LR + 1];
                 {
@@ -398,7 +626,9 @@ __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopSha
                     }
                 }
                 }   // has
-                __syncthreads();   // R and the staged tables are rewritten next
+                }
+                }
+                if constexpr (!SH::REG) __syncthreads();   // R and the staged tables are rewritten next
             }
         }
     }
@@ -432,7 +662,7 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
                 if (!quartet_selected(a, bp, kp)) continue;   // uniform over the workgroup
             }
             coop_group<LA, LB, LC, LD, BOYS, MODE, OP>(a, sm, bp, active ? kp : -1, 0, a.pair_npp[bp], 0,
-                                                       a.pair_npp[kp], V, nullptr, nullptr, nullptr, nullptr);
+                                                       a.pair_npp[kp], false, V, nullptr, nullptr, nullptr, nullptr);
             const int A = a.pair_A[bp], B = a.pair_B[bp], C = a.pair_A[kp], Dd = a.pair_B[kp];
             if constexpr (OP == HFK_OP_FLAT) {
                 if (active) {
@@ -478,6 +708,9 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
         if (lds)
             for (int t = lane; t < ndyn; t += 64) dyn[t] = 0.0;
         for (int t = lane; t < SH::S * NKET; t += 64) (&sm.jcd[0][0])[t] = 0.0;
+        // bra tables once per task when its primitive range fits the staging area
+        const bool staged = (b1 - b0) <= SH::NBP;
+        if (staged) coop_stage_bra(sm, a.pp + (size_t)a.pair_pp0[bp] * PP_STRIDE, b0, b1 - b0);
         __syncthreads();
         double jab = 0.0;
         double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
@@ -513,8 +746,8 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
             }
             if constexpr (OP == HFK_OP_JK) {
                 double V[NKET];
-                coop_group<LA, LB, LC, LD, BOYS, MODE, OP>(a, sm, bp, kp, b0, b1, k0, k1, V, nullptr, nullptr,
-                                                           nullptr, nullptr);
+                coop_group<LA, LB, LC, LD, BOYS, MODE, OP>(a, sm, bp, kp, b0, b1, k0, k1, staged, V, nullptr,
+                                                           nullptr, nullptr, nullptr);
                 if (kp >= 0) {
                     double* __restrict__ K = lds ? dyn : a.K;
                     const size_t rowa = lds ? (size_t)ca * ld : (size_t)ia * ld;
@@ -571,7 +804,8 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
                     Gw[cd] = (4.0 * f * sab * sc[c] * sd[d]) * (dab * dcd - 0.25 * (dac * dbd + dad * dbc));
                 }
                 double ga[3], gb[3], gc[3];
-                coop_group<LA, LB, LC, LD, BOYS, MODE, OP>(a, sm, bp, kp, b0, b1, k0, k1, nullptr, Gw, ga, gb, gc);
+                coop_group<LA, LB, LC, LD, BOYS, MODE, OP>(a, sm, bp, kp, b0, b1, k0, k1, staged, nullptr, Gw, ga, gb,
+                                                           gc);
                 if (kp >= 0) {
                     double* acc = lds ? dyn : a.grad;
                     const int atC = a.sh_atom[C], atD = a.sh_atom[Dd];
@@ -629,13 +863,20 @@ void launch_coop(const QuartetArgs& a, hipStream_t s)
     constexpr int NA = ncart(LA), NB = ncart(LB);
     if constexpr (OP == HFK_OP_JK || OP == HFK_OP_GRAD) {
         if (a.ntask <= 0) return;
+        using SH = CoopShape<LA, LB, LC, LD, OP>;
+        QuartetArgs base = a;
         size_t shm = 0;
         if (OP == HFK_OP_JK && a.lds_rows) shm = sizeof(double) * (NA + NB) * (size_t)(a.n + 1);
         if (OP == HFK_OP_GRAD && a.grad_lds) shm = sizeof(double) * 3 * (size_t)a.natom;
+        if (shm + sizeof(CoopLds<SH>) > 64 * 1024) {   // stage nothing: global atomics
+            shm = 0;
+            base.lds_rows = 0;
+            base.grad_lds = 0;
+        }
         long long done = 0;
         while (done < a.ntask) {
             const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);
-            QuartetArgs b = a;
+            QuartetArgs b = base;
             b.tasks = a.tasks + done;
             hipLaunchKernelGGL((k_coop<LA, LB, LC, LD, BOYS, MODE, OP>), dim3((unsigned)cnt), dim3(64), shm, s, b);
             done += cnt;

@@ -55,9 +55,14 @@ struct QuartetArgs {
 
 enum { HFK_OP_JK = 0, HFK_OP_FLAT = 1, HFK_OP_DIAG = 2, HFK_OP_GRAD = 3 };
 
-// Classes with a d shell run on the cooperative wave kernel (64-lane
-// workgroups, FP64, whole quartets per task).
-inline bool hfk_class_coop(int la, int lb, int lc, int ld) { return la >= 2 || lb >= 2 || lc >= 2 || ld >= 2; }
+// Classes with more than 54 contracted integrals per quartet (d shells with
+// p or d partners) run J/K on the cooperative wave kernel (64-lane
+// workgroups, FP64, whole quartets per task); gen_eri.py uses the same bound.
+inline bool hfk_class_coop(int la, int lb, int lc, int ld)
+{
+    const int nv = (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2) * ((lc + 1) * (lc + 2) / 2) * ((ld + 1) * (ld + 2) / 2);
+    return nv > 54;
+}
 
 // Launch one class kernel.  classes are (la,lb,lc,ld) with la>=lb, lc>=ld,
 // (la,lb) >= (lc,ld).  prec: 0 = FP64, 1 = FP32.  boys: 0 reference, 1 accurate.
