@@ -96,9 +96,12 @@ def gen_class(la, lb, lc, ld):
     name = "eri_%d%d%d%d" % (la, lb, lc, ld)
     out = []
     out.append("// (%s%s|%s%s): %d integrals, L=%d" % ("spdf"[la], "spdf"[lb], "spdf"[lc], "spdf"[ld], nbra * nket, L))
+    out.append("// bra: nbp primitive-pair records (LDS or global); bq: their per-component")
+    out.append("// screen values, bq[ib * %d + ab] (MODE 1)." % nbra)
     out.append("template <typename T, int BOYS, int MODE>")
-    out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp," % name)
-    out.append("    const double* __restrict__ qcomp, const BoysTables& bt, T* __restrict__ V, double qtau)")
+    out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, const double* __restrict__ bq, int nbp," % name)
+    out.append("    const double* __restrict__ ket, int nkp, const double* __restrict__ qcomp, const BoysTables& bt,")
+    out.append("    T* __restrict__ V, double qtau)")
     out.append("{")
     out.append("#pragma unroll")
     out.append("  for (int z = 0; z < %d; ++z) V[z] = T(0);" % (nbra * nket))
@@ -129,11 +132,11 @@ def gen_class(la, lb, lc, ld):
     out.append("      // reference screen: component (ab|cd) of this primitive quartet is kept iff")
     out.append("      // !(q_ab*q_cd < tau).  MODE 0: no screen; MODE 1: the screen, branch-free.")
     out.append("      if (MODE == 1 && bqhi * kqhi < qtau) continue;   // every component screened")
+    out.append("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
+    out.append("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
     out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
     em2 = Emitter()
     em2.n = em.n
-    em2.raw("      const T bp = T(pb[PP_P]);")
-    em2.raw("      const T bPx = T(pb[PP_PX]), bPy = T(pb[PP_PY]), bPz = T(pb[PP_PZ]);")
     em2.raw("      const T bAx = T(pb[PP_AX]), bAy = T(pb[PP_AY]), bAz = T(pb[PP_AZ]);")
     em2.raw("      const T bBx = T(pb[PP_BX]), bBy = T(pb[PP_BY]), bBz = T(pb[PP_BZ]);")
     em2.raw("      const T bK = T(pb[PP_K]);")
@@ -141,18 +144,13 @@ def gen_class(la, lb, lc, ld):
     if nket > 1 or nbra > 1:
         em2.raw("      double bqv[%d];" % nbra)
         em2.raw("      if constexpr (MODE == 1) {")
-        em2.raw("        const int bqoff = (int)pb[PP_QOFF];")
         em2.raw("#pragma unroll")
-        em2.raw("        for (int z = 0; z < %d; ++z) bqv[z] = qcomp[bqoff + z];" % nbra)
+        em2.raw("        for (int z = 0; z < %d; ++z) bqv[z] = bq[ib * %d + z];" % (nbra, nbra))
         em2.raw("      }")
     Eb = [hermite_E(em2, la, lb, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
-    em2.raw("      const T pq = bp + kq;")
-    em2.raw("      const T alpha = bp * kq / pq;")
-    em2.raw("      const T Xpq = bPx - kQx, Ypq = bPy - kQy, Zpq = bPz - kQz;")
-    em2.raw("      const T Targ = alpha * ((Xpq * Xpq + Ypq * Ypq) + Zpq * Zpq);")
     em2.raw("      T F[%d];" % (L + 1))
-    em2.raw("      boys_eval<T, %d, BOYS>(Targ, bt, F);" % L)
-    em2.raw("      const T pref = T(HFK_TWO_PI_2_5) * bK * kK / sqrt_t(pq);")
+    em2.raw("      boys_fast<T, %d, BOYS>(Targ, bt, F);" % L)
+    em2.raw("      const T pref = T(HFK_TWO_PI_2_5) * bK * kK * rs;")
     # base R^n_000 = pref * (-2 alpha)^n * F_n
     em2.raw("      const T m2a = T(-2) * alpha;")
     R = {}
@@ -484,9 +482,9 @@ def gen_grad_class(la, lb, lc, ld):
     return "\n".join(out)
 
 
-# Classes holding more contracted integrals per quartet than this run on
-# the cooperative wave kernel (per-lane registers cannot hold them); the
-# host's hfk_class_coop() uses the same bounds.
+# Classes with a d shell holding more contracted integrals per quartet than
+# this run on the cooperative wave kernel (per-lane registers cannot hold
+# them); the host's hfk_class_coop() uses the same bounds.
 COOP_NV_JK = 54
 COOP_NV_GRAD = 18
 
@@ -517,7 +515,7 @@ def main():
         nv = 1
         for l in q:
             nv *= (l + 1) * (l + 2) // 2
-        jk_coop = nv > COOP_NV_JK          # cooperative wave kernel (hfk_coop_impl.hpp)
+        jk_coop = max(q) >= 2 and nv > COOP_NV_JK   # cooperative wave kernel (hfk_coop_impl.hpp)
         grad_coop = max(q) >= 2 and nv > COOP_NV_GRAD
         body = ["// Generated by gen_eri.py -- do not edit.", "#include <hip/hip_runtime.h>"]
         if jk_coop:
@@ -537,9 +535,10 @@ def main():
         body.append("namespace {")
         body.append(src)
         body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>")
-        body.append("__device__ __forceinline__ void eri_class(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp,")
-        body.append("    const double* __restrict__ qcomp, const BoysTables& bt, T* __restrict__ V, double qtau)")
-        body.append("{ eri_%s<T, BOYS, MODE>(bra, nbp, ket, nkp, qcomp, bt, V, qtau); }" % tag)
+        body.append("__device__ __forceinline__ void eri_class(const double* __restrict__ bra, const double* __restrict__ bq, int nbp,")
+        body.append("    const double* __restrict__ ket, int nkp, const double* __restrict__ qcomp, const BoysTables& bt,")
+        body.append("    T* __restrict__ V, double qtau)")
+        body.append("{ eri_%s<T, BOYS, MODE>(bra, bq, nbp, ket, nkp, qcomp, bt, V, qtau); }" % tag)
         if not grad_coop:
             body.append(gen_grad_class(*q))
             body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>")

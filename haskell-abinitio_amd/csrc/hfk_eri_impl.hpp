@@ -20,7 +20,7 @@
 namespace {
 
 // Primitive-pair sub-ranges [b0,b1) of the bra and [k0,k1) of the ket
-// (b1/k1 < 0: to the end).
+// (b1/k1 < 0: to the end); bra records read from global memory.
 template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
 __device__ __forceinline__ void eval_quartet(const QuartetArgs& a, int bp, int kp, T* V, int b0 = 0, int b1 = -1,
                                              int k0 = 0, int k1 = -1)
@@ -29,7 +29,8 @@ __device__ __forceinline__ void eval_quartet(const QuartetArgs& a, int bp, int k
     const double* ket = a.pp + (size_t)(a.pair_pp0[kp] + k0) * PP_STRIDE;
     const int nbp = (b1 < 0 ? a.pair_npp[bp] : b1) - b0;
     const int nkp = (k1 < 0 ? a.pair_npp[kp] : k1) - k0;
-    eri_class<LA, LB, LC, LD, T, BOYS, MODE>(bra, nbp, ket, nkp, a.qcomp, a.bt, V, a.qtau);
+    const double* bq = a.qcomp + (MODE == 1 ? (long long)bra[PP_QOFF] : 0LL);
+    eri_class<LA, LB, LC, LD, T, BOYS, MODE>(bra, bq, nbp, ket, nkp, a.qcomp, a.bt, V, a.qtau);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -137,7 +138,7 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NKET = NC * ND, NBRA = NA * NB;
-    extern __shared__ double krow[];   // [(NA+NB)][ld]
+    extern __shared__ double krow[];   // [(NA+NB)][ld] when lds_rows, then the staged bra
     const int4 task = a.tasks[blockIdx.x];
     const int bp = task.x;
     // primitive-pair ranges: bra [b0,b1), ket [k0,k1), 8 bits each
@@ -148,6 +149,19 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
     const bool lds = a.lds_rows != 0;
     if (lds)
         for (int t = tid; t < (NA + NB) * ld; t += nthr) krow[t] = 0.0;
+    // the task's bra primitive-pair records and their per-component screen
+    // values, read by every lane in every ket iteration: staged once in LDS
+    const int nbs = bprim1 - bprim0;
+    double* sbra = krow + (lds ? (NA + NB) * ld : 0);
+    double* sbq = sbra + nbs * PP_STRIDE;
+    {
+        const double* g = a.pp + (size_t)(a.pair_pp0[bp] + bprim0) * PP_STRIDE;
+        for (int t = tid; t < nbs * PP_STRIDE; t += nthr) sbra[t] = g[t];
+        if constexpr (MODE == 1) {
+            const double* q = a.qcomp + (long long)g[PP_QOFF];
+            for (int t = tid; t < nbs * NBRA; t += nthr) sbq[t] = q[t];
+        }
+    }
     __syncthreads();
 
     const int A = a.pair_A[bp], B = a.pair_B[bp];
@@ -175,7 +189,8 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
         const int kp = a.ket_idx[task.y + jj];
         if (!quartet_selected(a, bp, kp)) continue;
         T Vt[NBRA * NKET];
-        eval_quartet<LA, LB, LC, LD, T, BOYS, MODE>(a, bp, kp, Vt, bprim0, bprim1, kprim0, kprim1);
+        eri_class<LA, LB, LC, LD, T, BOYS, MODE>(sbra, sbq, nbs, a.pp + (size_t)(a.pair_pp0[kp] + kprim0) * PP_STRIDE,
+                                                 kprim1 - kprim0, a.qcomp, a.bt, Vt, a.qtau);
         const int C = a.pair_A[kp], Dd = a.pair_B[kp];
         int ic[NC], id[ND];
         double sc[NC], sd[ND];
@@ -309,7 +324,7 @@ void launch_tiles(const QuartetArgs& a, hipStream_t s)
 {
     if (a.ntask <= 0) return;
     constexpr int NR = ncart(LA) + ncart(LB);
-    const size_t shm = a.lds_rows ? sizeof(double) * NR * (size_t)(a.n + 1) : 0;
+    const size_t shm = (a.lds_rows ? sizeof(double) * NR * (size_t)(a.n + 1) : 0) + hfk_bra_stage_bytes(LA, LB, a.bra_stage);
     long long done = 0;
     while (done < a.ntask) {   // grid.x limit
         const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);

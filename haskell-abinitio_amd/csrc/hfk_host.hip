@@ -592,6 +592,7 @@ int hfk_ctx_plan(hfk_ctx* c)
                     bstep = std::max(1, std::min(nbp, per_lane));
                     kstep = std::max(1, std::min(nkp, per_lane / bstep));
                 }
+                if (!coop) bstep = std::min(bstep, HFK_BRA_STAGE);   // staged in LDS by the kernel
                 for (int b0 = 0; b0 < nbp; b0 += bstep) {
                     const int b1 = std::min(nbp, b0 + bstep);
                     for (int q0 = 0; q0 < nkp; q0 += kstep) {
@@ -599,6 +600,7 @@ int hfk_ctx_plan(hfk_ctx* c)
                         const int w = (b0 << 24) | (b1 << 16) | (q0 << 8) | q1;
                         for (long long j0 = k0; j0 < k1; j0 += chunk)
                             tasks.push_back(make_int4(bl[i], (int)j0, (int)std::min<long long>(k1, j0 + chunk), w));
+                        e.bra_stage = std::max(e.bra_stage, b1 - b0);
                     }
                 }
             }
@@ -626,11 +628,13 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
     a.tasks = e.d_tasks;
     a.ket_idx = e.d_kets;
     a.ntask = e.ntask;
+    a.bra_stage = e.bra_stage;
     a.block = (c->n >= 200 && !hfk_class_coop(e.la, e.lb, e.lc, e.ld)) ? 256 : 64;
     {
         const int nr = (e.la + 1) * (e.la + 2) / 2 + (e.lb + 1) * (e.lb + 2) / 2;
         // the cooperative kernel also holds ~12 KB of static LDS
-        const size_t cap = hfk_class_coop(e.la, e.lb, e.lc, e.ld) ? 48 * 1024 : 64 * 1024;
+        const size_t cap = hfk_class_coop(e.la, e.lb, e.lc, e.ld) ? 48 * 1024
+                                                                  : 64 * 1024 - hfk_bra_stage_bytes(e.la, e.lb, e.bra_stage);
         a.lds_rows = ((size_t)nr * (c->n + 1) * sizeof(double) <= cap) ? 1 : 0;
     }
     a.bra_list = c->d_class[e.kb];

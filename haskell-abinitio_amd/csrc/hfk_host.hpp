@@ -47,6 +47,7 @@ struct PlanEntry {
     int4* d_tasks;              // tiled J/K tasks {bra pair, begin, end, 0} into d_kets
     int* d_kets;                // explicit ket pair indices
     long long ntask;
+    int bra_stage;              // longest bra primitive-pair range of the tasks
 };
 
 struct hfk_ctx {

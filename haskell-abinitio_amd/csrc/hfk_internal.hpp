@@ -46,6 +46,7 @@ struct QuartetArgs {
     long long ntask;
     int lds_rows;               // K rows staged in LDS (0: global atomics)
     int block;                  // workgroup size of the tiled kernel (64 or 256)
+    int bra_stage;              // longest bra primitive-pair range of the tasks (LDS staging)
     // nuclear gradient
     const int* sh_atom;         // [nshell] atom of each shell
     double* grad;               // [natom][3] accumulator
@@ -55,13 +56,22 @@ struct QuartetArgs {
 
 enum { HFK_OP_JK = 0, HFK_OP_FLAT = 1, HFK_OP_DIAG = 2, HFK_OP_GRAD = 3 };
 
-// Classes with more than 54 contracted integrals per quartet (d shells with
-// p or d partners) run J/K on the cooperative wave kernel (64-lane
+// Classes with a d shell and more than 54 contracted integrals per quartet
+// run J/K on the cooperative wave kernel (64-lane
 // workgroups, FP64, whole quartets per task); gen_eri.py uses the same bound.
 inline bool hfk_class_coop(int la, int lb, int lc, int ld)
 {
     const int nv = (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2) * ((lc + 1) * (lc + 2) / 2) * ((ld + 1) * (ld + 2) / 2);
-    return nv > 54;
+    return (la >= 2 || lb >= 2 || lc >= 2 || ld >= 2) && nv > 54;
+}
+
+// The per-lane J/K kernel stages the task's bra primitive-pair records and
+// their per-component screen values in LDS: at most HFK_BRA_STAGE primitive
+// pairs per task (the planner splits longer bra ranges).
+#define HFK_BRA_STAGE 64
+inline size_t hfk_bra_stage_bytes(int la, int lb, int nprim)
+{
+    return sizeof(double) * nprim * (16 + (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2));
 }
 
 // Launch one class kernel.  classes are (la,lb,lc,ld) with la>=lb, lc>=ld,
