@@ -788,7 +788,9 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
                 if (kp >= 0) {
                     for (int z = li; z < NKET; z += NBRA) {
                         const double v = sm.jcd[slot][z];
-                        if (v != 0.0) atomic_add_f64(&a.J[(size_t)ic[z / ND] * ld + id[z % ND]], v);
+                        if (v != 0.0)
+                            atomic_add_f64(&a.Jc[(size_t)(blockIdx.x % (unsigned)a.jcopies) * ld * ld +
+                                                 (size_t)ic[z / ND] * ld + id[z % ND]], v);
                         sm.jcd[slot][z] = 0.0;
                     }
                 }

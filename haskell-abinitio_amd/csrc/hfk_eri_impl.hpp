@@ -181,6 +181,9 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
 #pragma unroll
     for (int k = 0; k < NBRA; ++k) jab[k] = 0.0;
 
+    // ket-side J'_cd: every task of the launch adds into the same few ket
+    // pairs, so workgroups spread those atomics over private copies of J'
+    double* __restrict__ Jc = a.Jc + (size_t)(blockIdx.x % (unsigned)a.jcopies) * ld * ld;
     const int len = task.z - task.y;
     const int rot = (int)((unsigned)task.x * 37u % (unsigned)(len > 0 ? len : 1));
     for (int it = tid; it < len; it += nthr) {
@@ -231,7 +234,7 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
             double s = 0.0;
 #pragma unroll
             for (int b = 0; b < NBRA; ++b) s += V[b * NKET + k] * Dab[b];
-            atomic_add_f64(&a.J[(size_t)ic[k / ND] * ld + id[k % ND]], 2.0 * s);
+            atomic_add_f64(&Jc[(size_t)ic[k / ND] * ld + id[k % ND]], 2.0 * s);
         }
         // density blocks D_bd, D_bc, D_ad, D_ac
         double Dbd[NB * ND], Dbc[NB * NC], Dad[NA * ND], Dac[NA * NC];

@@ -299,6 +299,9 @@ int hfk_ctx_init_device(hfk_ctx* c)
     HFK_TRY(dev_alloc(&c->d_D, np1));
     HFK_TRY(dev_alloc(&c->d_J, np1));
     HFK_TRY(dev_alloc(&c->d_K, np1));
+    // copies of J' for the ket-side atomics: up to 64, about 32 MB in all
+    c->jcopies = (int)std::max<size_t>(1, std::min<size_t>(64, ((size_t)32 << 20) / (np1 * sizeof(double))));
+    HFK_TRY(dev_alloc(&c->d_Jc, np1 * (size_t)c->jcopies));
     HFK_TRY(dev_alloc(&c->d_flatA, nf));
     HFK_TRY(dev_alloc(&c->d_flatB, nf));
     HFK_TRY(dev_alloc(&c->d_flatC, nf));
@@ -477,6 +480,7 @@ void hfk_pair_class_l(int k, int* la, int* lb)
 // FP64 and an FP32 launch when the mixed-precision threshold is set.
 int hfk_ctx_plan(hfk_ctx* c)
 {
+    if (const char* v = getenv("HFK_KETS_PER_LANE")) c->kets_per_lane = std::max(1, atoi(v));
     for (PlanEntry& e : c->plan) {
         if (e.d_tasks) dev_free(c, e.d_tasks);
         if (e.d_kets) dev_free(c, e.d_kets);
@@ -545,7 +549,7 @@ int hfk_ctx_plan(hfk_ctx* c)
         const bool coop = hfk_class_coop(e.la, e.lb, e.lc, e.ld);
         // per-lane kernel: one ket per lane of the workgroup; cooperative kernel:
         // 64 / (na nb) kets at a time, four rounds per task
-        const int chunk = coop ? 4 * (64 / (ncart(e.la) * ncart(e.lb))) : (c->n >= 200 ? 256 : 64);
+        const int chunk = coop ? 4 * (64 / (ncart(e.la) * ncart(e.lb))) : (c->n >= 200 ? 256 : 64) * c->kets_per_lane;
         const std::vector<int>& bl = c->h_class[e.kb];
         const std::vector<int>& kl = c->h_class[e.kk];
         std::vector<double> kq(kl.size());
@@ -650,6 +654,8 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
 // Run the plan for one operation (J/K or flat export); per-launch HIP events.
 int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
 {
+    if (op == HFK_OP_JK && (!a.Jc || a.jcopies < 1 || !a.J || !a.K || !a.D))
+        return hfk_set_error(HFK_EINVAL, "J/K launch without its accumulators");
     long long total = 0;
     const size_t np = c->plan.size();
     for (size_t i = 0; i < np; ++i) {
@@ -711,13 +717,18 @@ int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_fla
     hfk_launch_flat_to_padded(c->n, D_dev_flat, c->d_D, c->stream);
     hfk_launch_fill(c->d_J, 0.0, (long long)np1, c->stream);
     hfk_launch_fill(c->d_K, 0.0, (long long)np1, c->stream);
+    // timed: the J' copies (cleared, then summed) belong to the digestion
+    HFK_HIP(hipEventRecord(c->ev0, c->stream));
+    hfk_launch_fill(c->d_Jc, 0.0, (long long)(np1 * (size_t)c->jcopies), c->stream);
     QuartetArgs a = hfk_ctx_args(c);
     a.sch_cut = c->opt.schwarz_cut;
     a.D = c->d_D;
     a.J = c->d_J;
     a.K = c->d_K;
-    HFK_HIP(hipEventRecord(c->ev0, c->stream));
+    a.Jc = c->d_Jc;
+    a.jcopies = c->jcopies;
     HFK_TRY(hfk_ctx_run_quartets(c, a, HFK_OP_JK));
+    hfk_launch_sum_copies(c->d_J, c->d_Jc, c->jcopies, (long long)np1, c->stream);
     HFK_HIP(hipEventRecord(c->ev1, c->stream));
     hfk_launch_symmetrize_g(c->n, c->d_J, c->d_K, H_dev, F_dev_flat, nullptr, c->stream);
     HFK_HIP(hipGetLastError());

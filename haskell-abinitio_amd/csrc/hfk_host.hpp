@@ -83,12 +83,15 @@ struct hfk_ctx {
     int *d_pair_A = nullptr, *d_pair_B = nullptr, *d_pair_pp0 = nullptr, *d_pair_npp = nullptr;
     double *d_pp = nullptr, *d_qcomp = nullptr, *d_pair_qhi = nullptr, *d_pair_sch = nullptr, *d_pair_diag = nullptr;
     double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr;
+    double* d_Jc = nullptr;     // private copies of J' for the ket-side sums
+    int jcopies = 1;
     double *d_flatA = nullptr, *d_flatB = nullptr, *d_flatC = nullptr;
     int *d_sh_atom = nullptr, *d_atom_of_bf = nullptr;   // gradient (uploaded on first use)
     hfk_stats stats = {};
     std::vector<PlanEntry> plan;
     std::vector<hipEvent_t> events;
     int timed_op = 0;
+    int kets_per_lane = 1;      // kets per lane in a tiled J/K task
 };
 
 template <typename T>

@@ -36,6 +36,8 @@ struct QuartetArgs {
     int n;                      // number of basis functions
     const double* D;            // [n+1][n+1] density (symmetric), row/column n zero
     double* J;                  // [n+1][n+1] accumulator J' (symmetrised on output)
+    double* Jc;                 // jcopies private copies of J' for the ket-side J'_cd sums
+    int jcopies;                // (workgroup w adds into copy w % jcopies; summed into J)
     double* K;                  // [n+1][n+1] accumulator K'
     double* flat;               // flat ERI output (calcIntegrals order)
     const long long* flat_base; // [n][n] rank of (i,j,i,j)
@@ -108,6 +110,8 @@ void hfk_launch_find_mixed(const QuartetArgs& a, unsigned long long* count, int2
 // J, K padded (leading dimension n+1)
 void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const double* H, double* G_flat,
                              double* F_full, hipStream_t s);
+// J += sum of the `copies` consecutive arrays of Jc (each `count` long)
+void hfk_launch_sum_copies(double* J, const double* Jc, int copies, long long count, hipStream_t s);
 // flat -> padded (n+1)x(n+1) full, row/column n zero
 void hfk_launch_flat_to_padded(int n, const double* flat, double* full, hipStream_t s);
 void hfk_launch_flat_to_full(int n, const double* flat, double* full, hipStream_t s);

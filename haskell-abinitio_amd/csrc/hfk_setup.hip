@@ -424,6 +424,21 @@ void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const doub
     hipLaunchKernelGGL(k_symmetrize_g, dim3(grid_for(m, 256)), dim3(256), 0, s, n, J, K, H, G_flat, F_full);
 }
 
+__global__ void k_sum_copies(double* J, const double* Jc, int copies, long long count)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long long)gridDim.x * blockDim.x) {
+        double v = J[i];
+        for (int c = 0; c < copies; ++c) v += Jc[(size_t)c * count + i];
+        J[i] = v;
+    }
+}
+
+void hfk_launch_sum_copies(double* J, const double* Jc, int copies, long long count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum_copies, dim3(grid_for(count, 256)), dim3(256), 0, s, J, Jc, copies, count);
+}
+
 void hfk_launch_flat_to_full(int n, const double* flat, double* full, hipStream_t s)
 {
     const long long m = (long long)n * (n + 1) / 2;
