@@ -695,6 +695,12 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
         return;
     } else {
         const int4 task = a.tasks[blockIdx.x];
+        // a long task (planned for the per-lane kernel) is cut into slices of
+        // 4 S kets along grid y; every slice digests independently
+        const int kslice = 4 * SH::S;
+        const int kbeg = task.y + (int)blockIdx.y * kslice;
+        if (kbeg >= task.z) return;   // uniform over the workgroup, before any barrier
+        const int kend = min(task.z, kbeg + kslice);
         const int bp = task.x;
         const int b0 = (task.w >> 24) & 0xff, b1 = (task.w >> 16) & 0xff;
         const int k0 = (task.w >> 8) & 0xff, k1 = task.w & 0xff;
@@ -714,12 +720,12 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
         __syncthreads();
         double jab = 0.0;
         double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
-        const int len = task.z - task.y;
+        const int len = kend - kbeg;
         for (int g0 = 0; g0 < len; g0 += SH::S) {
             const int it = g0 + slot;
             int kp = -1;
             if (active && it < len) {
-                kp = a.ket_idx[task.y + it];
+                kp = a.ket_idx[kbeg + it];
                 if (!quartet_selected(a, bp, kp)) kp = -1;
             }
             int C = 0, Dd = 0;
@@ -880,7 +886,9 @@ void launch_coop(const QuartetArgs& a, hipStream_t s)
             const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);
             QuartetArgs b = base;
             b.tasks = a.tasks + done;
-            hipLaunchKernelGGL((k_coop<LA, LB, LC, LD, BOYS, MODE, OP>), dim3((unsigned)cnt), dim3(64), shm, s, b);
+            const int kslice = 4 * SH::S;
+            const unsigned ysl = (unsigned)std::max(1, (a.task_len_max + kslice - 1) / kslice);
+            hipLaunchKernelGGL((k_coop<LA, LB, LC, LD, BOYS, MODE, OP>), dim3((unsigned)cnt, ysl), dim3(64), shm, s, b);
             done += cnt;
         }
     } else {

@@ -260,6 +260,27 @@ int hfk_ctx_init_device(hfk_ctx* c)
     HFK_TRY(dev_upload(c, &c->d_center, c->center));
     HFK_TRY(dev_upload(c, &c->d_lxyz, c->lxyz));
     HFK_TRY(dev_upload(c, &c->d_prim_off, c->prim_off));
+    {
+        // upper-triangle function pairs (flat index) grouped by l_i + l_j
+        std::vector<int> groups[5];
+        const int n = c->n;
+        long long idx = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = i; j < n; ++j, ++idx) {
+                const int ls = c->lxyz[3 * i] + c->lxyz[3 * i + 1] + c->lxyz[3 * i + 2] + c->lxyz[3 * j] +
+                               c->lxyz[3 * j + 1] + c->lxyz[3 * j + 2];
+                if (ls > 4) return hfk_set_error(HFK_EUNSUP, "one-electron integrals: l_i + l_j = %d > 4", ls);
+                groups[ls].push_back((int)idx);
+            }
+        std::vector<int> all;
+        c->vpair_off.assign(6, 0);
+        for (int ls = 0; ls < 5; ++ls) {
+            c->vpair_off[ls] = (int)all.size();
+            all.insert(all.end(), groups[ls].begin(), groups[ls].end());
+        }
+        c->vpair_off[5] = (int)all.size();
+        HFK_TRY(dev_upload(c, &c->d_vpair, all));
+    }
     HFK_TRY(dev_upload(c, &c->d_coef, c->coef));
     HFK_TRY(dev_upload(c, &c->d_expo, c->expo));
     if (!c->Z.empty()) {
@@ -605,6 +626,7 @@ int hfk_ctx_plan(hfk_ctx* c)
                         for (long long j0 = k0; j0 < k1; j0 += chunk)
                             tasks.push_back(make_int4(bl[i], (int)j0, (int)std::min<long long>(k1, j0 + chunk), w));
                         e.bra_stage = std::max(e.bra_stage, b1 - b0);
+                        e.task_len_max = std::max<int>(e.task_len_max, (int)std::min<long long>(chunk, k1 - k0));
                     }
                 }
             }
@@ -633,6 +655,7 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
     a.ket_idx = e.d_kets;
     a.ntask = e.ntask;
     a.bra_stage = e.bra_stage;
+    a.task_len_max = e.task_len_max;
     a.block = (c->n >= 200 && !hfk_class_coop(e.la, e.lb, e.lc, e.ld)) ? 256 : 64;
     {
         const int nr = (e.la + 1) * (e.la + 2) / 2 + (e.lb + 1) * (e.lb + 2) / 2;
@@ -830,6 +853,8 @@ OneEArgs hfk_onee_args(hfk_ctx* c)
     a.R = c->d_R;
     a.bt = c->bt;
     a.boys = c->opt.boys_mode;
+    a.vpair = c->d_vpair;
+    a.vpair_off = c->vpair_off.data();
     return a;
 }
 

@@ -48,6 +48,7 @@ struct PlanEntry {
     int* d_kets;                // explicit ket pair indices
     long long ntask;
     int bra_stage;              // longest bra primitive-pair range of the tasks
+    int task_len_max;           // longest ket run of the tasks
 };
 
 struct hfk_ctx {
@@ -87,6 +88,8 @@ struct hfk_ctx {
     int jcopies = 1;
     double *d_flatA = nullptr, *d_flatB = nullptr, *d_flatC = nullptr;
     int *d_sh_atom = nullptr, *d_atom_of_bf = nullptr;   // gradient (uploaded on first use)
+    int* d_vpair = nullptr;                 // function pairs grouped by l_i + l_j (one-electron V)
+    std::vector<int> vpair_off;             // [6]
     hfk_stats stats = {};
     std::vector<PlanEntry> plan;
     std::vector<hipEvent_t> events;

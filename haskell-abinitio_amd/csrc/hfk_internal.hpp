@@ -49,6 +49,7 @@ struct QuartetArgs {
     int lds_rows;               // K rows staged in LDS (0: global atomics)
     int block;                  // workgroup size of the tiled kernel (64 or 256)
     int bra_stage;              // longest bra primitive-pair range of the tasks (LDS staging)
+    int task_len_max;           // longest ket run of the tasks
     // nuclear gradient
     const int* sh_atom;         // [nshell] atom of each shell
     double* grad;               // [natom][3] accumulator
@@ -130,6 +131,8 @@ struct OneEArgs {
     const double* R;        // [natom][3]
     BoysTables bt;
     int boys;
+    const int* vpair;       // flat pair indices grouped by l_i + l_j = 0..4
+    const int* vpair_off;   // [6] group offsets into vpair (host pointer)
 };
 void hfk_launch_overlap(const OneEArgs& a, double* S_flat, hipStream_t s);
 void hfk_launch_hcore(const OneEArgs& a, double* H_flat, hipStream_t s);
