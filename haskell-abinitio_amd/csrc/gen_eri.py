@@ -126,134 +126,164 @@ def gen_class(la, lb, lc, ld):
         em.raw("    }")
     Ek = [hermite_E(em, lc, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
     out.extend(em.lines)
-    out.append("    for (int ib = 0; ib < nbp; ++ib) {")
-    out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
-    out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
-    out.append("      // reference screen: component (ab|cd) of this primitive quartet is kept iff")
-    out.append("      // !(q_ab*q_cd < tau).  MODE 0: no screen; MODE 1: the screen, branch-free.")
-    out.append("      if (MODE == 1 && bqhi * kqhi < qtau) continue;   // every component screened")
-    out.append("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
-    out.append("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
-    out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
+    unroll = 2 if nbra * nket <= UNROLL2_NV else 1
+
+    def body(em2, sx, ibx):
+        """One bra primitive pair (record pb<sx>, index ibx): returns
+        {V index: (value name, bra component, ket component)}."""
+        em2.raw("      const T bAx%s = T(pb%s[PP_AX]), bAy%s = T(pb%s[PP_AY]), bAz%s = T(pb%s[PP_AZ]);" % ((sx,) * 6))
+        em2.raw("      const T bBx%s = T(pb%s[PP_BX]), bBy%s = T(pb%s[PP_BY]), bBz%s = T(pb%s[PP_BZ]);" % ((sx,) * 6))
+        em2.raw("      const T bK%s = T(pb%s[PP_K]);" % (sx, sx))
+        em2.raw("      const T boo2p%s = T(pb%s[PP_OO2P]);" % (sx, sx))
+        if nket > 1 or nbra > 1:
+            em2.raw("      double bqv%s[%d];" % (sx, nbra))
+            em2.raw("      if constexpr (MODE == 1) {")
+            em2.raw("#pragma unroll")
+            em2.raw("        for (int z = 0; z < %d; ++z) bqv%s[z] = bq[(%s) * %d + z];" % (nbra, sx, ibx, nbra))
+            em2.raw("      }")
+        Eb = [hermite_E(em2, la, lb, "bA" + ax + sx, "bB" + ax + sx, "boo2p" + sx, "b" + ax) for ax in "xyz"]
+        em2.raw("      T F%s[%d];" % (sx, L + 1))
+        if unroll == 2:
+            em2.raw("      boys_select<T, %d, BOYS>(Targ%s, bt, F%s);" % (L, sx, sx))
+        else:
+            em2.raw("      boys_fast<T, %d, BOYS>(Targ%s, bt, F%s);" % (L, sx, sx))
+        em2.raw("      const T pref%s = T(HFK_TWO_PI_2_5) * bK%s * kK * rs%s;" % (sx, sx, sx))
+        # base R^n_000 = pref * (-2 alpha)^n * F_n
+        em2.raw("      const T m2a%s = T(-2) * alpha%s;" % (sx, sx))
+        R = {}
+        pw = "pref" + sx
+        for n in range(0, L + 1):
+            if n == 0:
+                R[(0, 0, 0, 0)] = em2.tmp("pref%s * F%s[0]" % (sx, sx), "r")
+            else:
+                pw = em2.tmp("%s * m2a%s" % (pw, sx), "w")
+                R[(n, 0, 0, 0)] = em2.tmp("%s * F%s[%d]" % (pw, sx, n), "r")
+
+        def getR(n, t, u, v):
+            if t < 0 or u < 0 or v < 0:
+                return None
+            key = (n, t, u, v)
+            if key in R:
+                return R[key]
+            if t >= 1:
+                a_ = getR(n + 1, t - 2, u, v)
+                b_ = getR(n + 1, t - 1, u, v)
+                c_, X = t - 1, "Xpq"
+            elif u >= 1:
+                a_ = getR(n + 1, t, u - 2, v)
+                b_ = getR(n + 1, t, u - 1, v)
+                c_, X = u - 1, "Ypq"
+            else:
+                a_ = getR(n + 1, t, u, v - 2)
+                b_ = getR(n + 1, t, u, v - 1)
+                c_, X = v - 1, "Zpq"
+            terms = []
+            if a_ is not None and c_ != 0:
+                terms.append("T(%d)*%s" % (c_, a_))
+            terms.append("%s%s*%s" % (X, sx, b_))
+            R[key] = em2.tmp(" + ".join(terms), "r")
+            return R[key]
+
+        lab = la + lb
+        btuv = [(t, u, v) for t in range(lab + 1) for u in range(lab + 1 - t) for v in range(lab + 1 - t - u)]
+        vals = {}
+        for ic, c in enumerate(cc):
+            for id_, d in enumerate(cd):
+                ket_idx = ic * nd + id_
+                # W^{cd}_{tuv} for tuv in btuv
+                W = {}
+                for (t, u, v) in btuv:
+                    terms_x = []
+                    for tau in range(c[0] + d[0] + 1):
+                        ex = Ek[0][(c[0], d[0], tau)]
+                        terms_y = []
+                        for nu in range(c[1] + d[1] + 1):
+                            ey = Ek[1][(c[1], d[1], nu)]
+                            terms_z = []
+                            for phi in range(c[2] + d[2] + 1):
+                                ez = Ek[2][(c[2], d[2], phi)]
+                                r = getR(0, t + tau, u + nu, v + phi)
+                                sg = "-" if (phi % 2) else ""
+                                terms_z.append("%s%s" % (sg, r) if ez == "T(1)" else "%s%s*%s" % (sg, ez, r))
+                            sz = " + ".join(terms_z).replace("+ -", "- ")
+                            sgy = (nu % 2) == 1
+                            ty = "(%s)" % sz if ey == "T(1)" else "%s*(%s)" % (ey, sz)
+                            terms_y.append(("-" if sgy else "") + ty)
+                        sy = " + ".join(terms_y).replace("+ -", "- ")
+                        sgx = (tau % 2) == 1
+                        tx = "(%s)" % sy if ex == "T(1)" else "%s*(%s)" % (ex, sy)
+                        terms_x.append(("-" if sgx else "") + tx)
+                    sxp = " + ".join(terms_x).replace("+ -", "- ")
+                    W[(t, u, v)] = em2.tmp(sxp, "W")
+                for ia, a in enumerate(ca):
+                    for ib_, b in enumerate(cb):
+                        bra_idx = ia * nb + ib_
+                        terms_x = []
+                        for t in range(a[0] + b[0] + 1):
+                            ex = Eb[0][(a[0], b[0], t)]
+                            terms_y = []
+                            for u in range(a[1] + b[1] + 1):
+                                ey = Eb[1][(a[1], b[1], u)]
+                                terms_z = []
+                                for v in range(a[2] + b[2] + 1):
+                                    ez = Eb[2][(a[2], b[2], v)]
+                                    w = W[(t, u, v)]
+                                    terms_z.append(w if ez == "T(1)" else "%s*%s" % (ez, w))
+                                sz = " + ".join(terms_z)
+                                terms_y.append("(%s)" % sz if ey == "T(1)" else "%s*(%s)" % (ey, sz))
+                            sy = " + ".join(terms_y)
+                            terms_x.append("(%s)" % sy if ex == "T(1)" else "%s*(%s)" % (ex, sy))
+                        vals[bra_idx * nket + ket_idx] = (em2.tmp(" + ".join(terms_x), "v"), bra_idx, ket_idx)
+        return vals
+
     em2 = Emitter()
     em2.n = em.n
-    em2.raw("      const T bAx = T(pb[PP_AX]), bAy = T(pb[PP_AY]), bAz = T(pb[PP_AZ]);")
-    em2.raw("      const T bBx = T(pb[PP_BX]), bBy = T(pb[PP_BY]), bBz = T(pb[PP_BZ]);")
-    em2.raw("      const T bK = T(pb[PP_K]);")
-    em2.raw("      const T boo2p = T(pb[PP_OO2P]);")
-    if nket > 1 or nbra > 1:
-        em2.raw("      double bqv[%d];" % nbra)
-        em2.raw("      if constexpr (MODE == 1) {")
-        em2.raw("#pragma unroll")
-        em2.raw("        for (int z = 0; z < %d; ++z) bqv[z] = bq[ib * %d + z];" % (nbra, nbra))
-        em2.raw("      }")
-    Eb = [hermite_E(em2, la, lb, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
-    em2.raw("      T F[%d];" % (L + 1))
-    em2.raw("      boys_fast<T, %d, BOYS>(Targ, bt, F);" % L)
-    em2.raw("      const T pref = T(HFK_TWO_PI_2_5) * bK * kK * rs;")
-    # base R^n_000 = pref * (-2 alpha)^n * F_n
-    em2.raw("      const T m2a = T(-2) * alpha;")
-    R = {}
-    pw = "pref"
-    for n in range(0, L + 1):
-        if n == 0:
-            R[(0, 0, 0, 0)] = em2.tmp("pref * F[0]", "r")
-        else:
-            pw = em2.tmp("%s * m2a" % pw, "w")
-            R[(n, 0, 0, 0)] = em2.tmp("%s * F[%d]" % (pw, n), "r")
-
-    def getR(n, t, u, v):
-        if t < 0 or u < 0 or v < 0:
-            return None
-        key = (n, t, u, v)
-        if key in R:
-            return R[key]
-        if t >= 1:
-            a = getR(n + 1, t - 2, u, v)
-            b = getR(n + 1, t - 1, u, v)
-            terms = []
-            if a is not None and t - 1 != 0:
-                terms.append("T(%d)*%s" % (t - 1, a))
-            terms.append("Xpq*%s" % b)
-        elif u >= 1:
-            a = getR(n + 1, t, u - 2, v)
-            b = getR(n + 1, t, u - 1, v)
-            terms = []
-            if a is not None and u - 1 != 0:
-                terms.append("T(%d)*%s" % (u - 1, a))
-            terms.append("Ypq*%s" % b)
-        else:
-            a = getR(n + 1, t, u, v - 2)
-            b = getR(n + 1, t, u, v - 1)
-            terms = []
-            if a is not None and v - 1 != 0:
-                terms.append("T(%d)*%s" % (v - 1, a))
-            terms.append("Zpq*%s" % b)
-        R[key] = em2.tmp(" + ".join(terms), "r")
-        return R[key]
-
-    lab = la + lb
-    # hermite tuv list for the bra side
-    btuv = [(t, u, v) for t in range(lab + 1) for u in range(lab + 1 - t) for v in range(lab + 1 - t - u)]
-    # V accumulation per ket component pair
-    for ic, c in enumerate(cc):
-        for id_, d in enumerate(cd):
-            ket_idx = ic * nd + id_
-            # W^{cd}_{tuv} for tuv in btuv
-            W = {}
-            for (t, u, v) in btuv:
-                terms_x = []
-                for tau in range(c[0] + d[0] + 1):
-                    ex = Ek[0][(c[0], d[0], tau)]
-                    terms_y = []
-                    for nu in range(c[1] + d[1] + 1):
-                        ey = Ek[1][(c[1], d[1], nu)]
-                        terms_z = []
-                        for phi in range(c[2] + d[2] + 1):
-                            ez = Ek[2][(c[2], d[2], phi)]
-                            r = getR(0, t + tau, u + nu, v + phi)
-                            sg = "-" if (phi % 2) else ""
-                            terms_z.append("%s%s" % (sg, r) if ez == "T(1)" else "%s%s*%s" % (sg, ez, r))
-                        sz = " + ".join(terms_z).replace("+ -", "- ")
-                        sgy = (nu % 2) == 1
-                        if ey == "T(1)":
-                            ty = "(%s)" % sz
-                        else:
-                            ty = "%s*(%s)" % (ey, sz)
-                        terms_y.append(("-" if sgy else "") + ty)
-                    sy = " + ".join(terms_y).replace("+ -", "- ")
-                    sgx = (tau % 2) == 1
-                    tx = "(%s)" % sy if ex == "T(1)" else "%s*(%s)" % (ex, sy)
-                    terms_x.append(("-" if sgx else "") + tx)
-                sx = " + ".join(terms_x).replace("+ -", "- ")
-                W[(t, u, v)] = em2.tmp(sx, "W")
-            for ia, a in enumerate(ca):
-                for ib_, b in enumerate(cb):
-                    bra_idx = ia * nb + ib_
-                    terms_x = []
-                    for t in range(a[0] + b[0] + 1):
-                        ex = Eb[0][(a[0], b[0], t)]
-                        terms_y = []
-                        for u in range(a[1] + b[1] + 1):
-                            ey = Eb[1][(a[1], b[1], u)]
-                            terms_z = []
-                            for v in range(a[2] + b[2] + 1):
-                                ez = Eb[2][(a[2], b[2], v)]
-                                w = W[(t, u, v)]
-                                terms_z.append(w if ez == "T(1)" else "%s*%s" % (ez, w))
-                            sz = " + ".join(terms_z)
-                            terms_y.append("(%s)" % sz if ey == "T(1)" else "%s*(%s)" % (ey, sz))
-                        sy = " + ".join(terms_y)
-                        terms_x.append("(%s)" % sy if ex == "T(1)" else "%s*(%s)" % (ex, sy))
-                    val = " + ".join(terms_x)
-                    vi = bra_idx * nket + ket_idx
-                    em2.raw("      { const T v_ = %s;" % val)
-                    if nket > 1 or nbra > 1:
-                        em2.raw("        if constexpr (MODE == 0) V[%d] += v_;" % vi)
-                        em2.raw("        else V[%d] += (allin || !(bqv[%d] * kqv[%d] < qtau)) ? v_ : T(0); }"
-                                % (vi, bra_idx, ket_idx))
-                    else:
-                        em2.raw("        V[%d] += v_; }" % vi)
+    multi = nket > 1 or nbra > 1
+    if unroll == 1:
+        out.append("    for (int ib = 0; ib < nbp; ++ib) {")
+        out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
+        out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
+        out.append("      // reference screen: component (ab|cd) of this primitive quartet is kept iff")
+        out.append("      // !(q_ab*q_cd < tau).  MODE 0: no screen; MODE 1: the screen, branch-free.")
+        out.append("      if (MODE == 1 && bqhi * kqhi < qtau) continue;   // every component screened")
+        out.append("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
+        out.append("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
+        out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
+        vals = body(em2, "", "ib")
+        for vi in sorted(vals):
+            name_, bi, ki = vals[vi]
+            if multi:
+                em2.raw("      if constexpr (MODE == 0) V[%d] += %s;" % (vi, name_))
+                em2.raw("      else V[%d] += (allin || !(bqv[%d] * kqv[%d] < qtau)) ? %s : T(0);" % (vi, bi, ki, name_))
+            else:
+                em2.raw("      V[%d] += %s;" % (vi, name_))
+    else:
+        # two bra primitive pairs per iteration: independent chains the
+        # scheduler interleaves; each contribution carries its own screen
+        out.append("    for (int ib = 0; ib < nbp; ib += 2) {")
+        out.append("      const bool two = ib + 1 < nbp;")
+        out.append("      const double* pb0 = bra + (size_t)ib * PP_STRIDE;")
+        out.append("      const double* pb1 = two ? pb0 + PP_STRIDE : pb0;")
+        out.append("      // reference screen: component (ab|cd) of a primitive quartet is kept iff")
+        out.append("      // !(q_ab*q_cd < tau).  MODE 0: no screen; MODE 1: the screen, branch-free.")
+        out.append("      const bool on0 = !(MODE == 1 && pb0[PP_QHI] * kqhi < qtau);")
+        out.append("      const bool on1 = two && !(MODE == 1 && pb1[PP_QHI] * kqhi < qtau);")
+        out.append("      if (!on0 && !on1) continue;   // every component of both screened")
+        out.append("      const bool allin0 = MODE == 0 || (pb0[PP_QLO] * kqlo >= qtau);")
+        out.append("      const bool allin1 = MODE == 0 || (pb1[PP_QLO] * kqlo >= qtau);")
+        out.append("      T alpha0, Xpq0, Ypq0, Zpq0, rs0, Targ0, alpha1, Xpq1, Ypq1, Zpq1, rs1, Targ1;")
+        out.append("      quartet_geom<T>(pb0, kq, kQx, kQy, kQz, alpha0, Xpq0, Ypq0, Zpq0, rs0, Targ0);")
+        out.append("      quartet_geom<T>(pb1, kq, kQx, kQy, kQz, alpha1, Xpq1, Ypq1, Zpq1, rs1, Targ1);")
+        v0 = body(em2, "0", "ib")
+        v1 = body(em2, "1", "ib + (two ? 1 : 0)")
+        for vi in sorted(v0):
+            n0, bi, ki = v0[vi]
+            n1 = v1[vi][0]
+            if multi:
+                em2.raw("      V[%d] += ((on0 && (allin0 || !(bqv0[%d] * kqv[%d] < qtau))) ? %s : T(0)) + "
+                        "((on1 && (allin1 || !(bqv1[%d] * kqv[%d] < qtau))) ? %s : T(0));" % (vi, bi, ki, n0, bi, ki, n1))
+            else:
+                em2.raw("      V[%d] += (on0 ? %s : T(0)) + (on1 ? %s : T(0));" % (vi, n0, n1))
     out.extend(em2.lines)
     out.append("    }")
     out.append("  }")
@@ -324,13 +354,11 @@ def gen_grad_class(la, lb, lc, ld):
         em2.raw("        for (int z = 0; z < %d; ++z) bqv[z] = qcomp[bqoff + z];" % nbra)
         em2.raw("      }")
     Eb = [hermite_E(em2, la + 1, lb + 1, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
-    em2.raw("      const T pq = bp + kq;")
-    em2.raw("      const T alpha = bp * kq / pq;")
-    em2.raw("      const T Xpq = bPx - kQx, Ypq = bPy - kQy, Zpq = bPz - kQz;")
-    em2.raw("      const T Targ = alpha * ((Xpq * Xpq + Ypq * Ypq) + Zpq * Zpq);")
+    em2.raw("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
+    em2.raw("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
     em2.raw("      T F[%d];" % (L + 2))
-    em2.raw("      boys_eval<T, %d, BOYS>(Targ, bt, F);" % (L + 1))
-    em2.raw("      const T pref = T(HFK_TWO_PI_2_5) * bK * kK / sqrt_t(pq);")
+    em2.raw("      boys_fast<T, %d, BOYS>(Targ, bt, F);" % (L + 1))
+    em2.raw("      const T pref = T(HFK_TWO_PI_2_5) * bK * kK * rs;")
     em2.raw("      const T m2a = T(-2) * alpha;")
     # masked density per component quartet
     Gm = {}
@@ -487,6 +515,9 @@ def gen_grad_class(la, lb, lc, ld):
 # them); the host's hfk_class_coop() uses the same bounds.
 COOP_NV_JK = 54
 COOP_NV_GRAD = 18
+# Classes with at most this many integrals per quartet evaluate two bra
+# primitive pairs per loop iteration (two independent dependency chains).
+UNROLL2_NV = 0
 
 
 def pair_classes(lmax):
@@ -516,7 +547,7 @@ def main():
         for l in q:
             nv *= (l + 1) * (l + 2) // 2
         jk_coop = max(q) >= 2 and nv > COOP_NV_JK   # cooperative wave kernel (hfk_coop_impl.hpp)
-        grad_coop = max(q) >= 2 and nv > COOP_NV_GRAD
+        grad_coop = (max(q) >= 2 and nv > COOP_NV_GRAD) or nv > COOP_NV_JK
         body = ["// Generated by gen_eri.py -- do not edit.", "#include <hip/hip_runtime.h>"]
         if jk_coop:
             body += ['#include "hfk_coop_impl.hpp"',

@@ -275,6 +275,7 @@ __device__ __forceinline__ void boys_fast(T x, const BoysTables& bt, T* F)
     }
 }
 
+
 // Geometry of one primitive quartet from the bra record pb and the ket
 // pair (q, Q): alpha = pq/(p+q), R_PQ, T = alpha |R_PQ|^2, rs = 1/sqrt(p+q).
 template <typename T>

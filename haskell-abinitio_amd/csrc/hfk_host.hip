@@ -502,10 +502,8 @@ void hfk_pair_class_l(int k, int* la, int* lb)
 int hfk_ctx_plan(hfk_ctx* c)
 {
     if (const char* v = getenv("HFK_KETS_PER_LANE")) c->kets_per_lane = std::max(1, atoi(v));
-    for (PlanEntry& e : c->plan) {
+    for (PlanEntry& e : c->plan)
         if (e.d_tasks) dev_free(c, e.d_tasks);
-        if (e.d_kets) dev_free(c, e.d_kets);
-    }
     c->plan.clear();
     const bool ref = c->opt.screen_mode == HFK_SCREEN_REFERENCE && c->opt.screen_tau > 0.0;
     for (int kb = 0; kb < HFK_NPC; ++kb) {
@@ -535,31 +533,6 @@ int hfk_ctx_plan(hfk_ctx* c)
             }
         }
     }
-    // counts (shell, primitive and canonical function quartets) per main launch
-    unsigned long long* d_cnt = nullptr;
-    const size_t np = c->plan.size();
-    HFK_TRY(dev_alloc(&d_cnt, 3 * std::max<size_t>(np, 1)));
-    hfk_launch_fill((double*)d_cnt, 0.0, (long long)(3 * std::max<size_t>(np, 1)), c->stream);
-    QuartetArgs a = hfk_ctx_args(c);
-    a.sch_cut = c->opt.schwarz_cut;
-    for (size_t i = 0; i < np; ++i) {
-        PlanEntry& e = c->plan[i];
-        hfk_plan_args(c, e, a);
-        hfk_launch_count(a, ncart(e.la), ncart(e.lb), ncart(e.lc), ncart(e.ld), d_cnt + 3 * i, c->stream);
-    }
-    HFK_HIP(hipGetLastError());
-    std::vector<unsigned long long> h(3 * std::max<size_t>(np, 1));
-    HFK_HIP(hipMemcpyAsync(h.data(), d_cnt, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
-    HFK_HIP(hipStreamSynchronize(c->stream));
-    dev_free(c, d_cnt);
-    long long fq = 0;
-    for (size_t i = 0; i < np; ++i) {
-        c->plan[i].shell_q = (long long)h[3 * i];
-        c->plan[i].prim_q = (long long)h[3 * i + 1];
-        c->plan[i].func_q = (long long)h[3 * i + 2];
-        fq += c->plan[i].func_q;
-    }
-    c->stats.func_quartets = fq;
     // Task lists of the tiled J/K kernel.  Class lists are ordered by
     // primitive-pair count, then by descending screen value.  For each bra
     // pair, the kets surviving the pair screen form a prefix of each
@@ -582,8 +555,29 @@ int hfk_ctx_plan(hfk_ctx* c)
             segs.push_back({(long long)j, (long long)k});
             j = k;
         }
+        // surviving kets of bra i in segment sg: [sg.first, hi) by binary search
+        auto staircase = [&](size_t i, const std::pair<long long, long long>& sg) -> long long {
+            const double qi = c->h_pair_qhi[bl[i]];
+            // kets need qhi_j >= tau / qhi_i; relaxed so that rounding never drops a survivor
+            const double need = (!ref || std::isinf(qi)) ? -1.0
+                              : (qi > 0.0 ? c->opt.screen_tau / qi * (1.0 - 1e-12) : INFINITY);
+            long long hi_end = sg.second;
+            if (e.kb == e.kk) hi_end = std::min<long long>(hi_end, (long long)i + 1);
+            if (hi_end <= sg.first) return sg.first;
+            long long lo = sg.first, hi = hi_end;
+            while (lo < hi) {
+                const long long mid = (lo + hi) / 2;
+                if (kq[mid] >= need) lo = mid + 1; else hi = mid;
+            }
+            return lo;
+        };
+        // primitive quartets of the surviving shell quartets (before the
+        // primitive-level screen): sizes the primitive-range split below
+        long long est_prim = 0;
+        for (size_t i = 0; i < bl.size(); ++i)
+            for (const auto& sg : segs)
+                est_prim += (long long)c->h_pair_npp[bl[i]] * c->h_pair_npp[kl[sg.first]] * (staircase(i, sg) - sg.first);
         std::vector<int4> tasks;
-        std::vector<int> kets;
         for (size_t i = 0; i < bl.size(); ++i) {
             const double qi = c->h_pair_qhi[bl[i]];
             // kets need qhi_j >= tau / qhi_i; relaxed so that rounding never drops a survivor
@@ -600,9 +594,8 @@ int hfk_ctx_plan(hfk_ctx* c)
                     if (kq[mid] >= need) lo = mid + 1; else hi = mid;
                 }
                 if (lo <= sg.first) continue;
-                const long long k0 = (long long)kets.size();
-                for (long long j = sg.first; j < lo; ++j) kets.push_back(kl[j]);
-                const long long k1 = (long long)kets.size();
+                // the surviving kets are the contiguous run [sg.first, lo) of the class list
+                const long long k0 = sg.first, k1 = lo;
                 // Primitive ranges: a lane's share of a task is about per_lane primitive
                 // quartets, sized from the launch's total so that small launches still
                 // fill the GPU (~2.6e5 lanes in flight) and large ones keep quartets whole
@@ -611,7 +604,7 @@ int hfk_ctx_plan(hfk_ctx* c)
                 const int nkp = c->h_pair_npp[kl[sg.first]];
                 const int L = e.la + e.lb + e.lc + e.ld;
                 const long long floor_l = std::max(4, 32 >> L);   // cheap classes: amortise the digestion
-                const int per_lane = (int)std::max(floor_l, std::min(128LL, e.prim_q / 262144));
+                const int per_lane = (int)std::max(floor_l, std::min(128LL, est_prim / 262144));
                 int bstep = nbp, kstep = nkp;
                 if (!coop && nbp * nkp > per_lane) {
                     bstep = std::max(1, std::min(nbp, per_lane));
@@ -631,15 +624,41 @@ int hfk_ctx_plan(hfk_ctx* c)
                 }
             }
         }
-        if ((long long)kets.size() > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "task list too long");
+        if ((long long)kl.size() > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "pair class too long");
         e.ntask = (long long)tasks.size();
         e.d_tasks = nullptr;
-        e.d_kets = nullptr;
-        if (!tasks.empty()) {
-            HFK_TRY(dev_upload(c, &e.d_tasks, tasks));
-            HFK_TRY(dev_upload(c, &e.d_kets, kets));
-        }
+        e.d_kets = c->d_class[e.kk];   // tasks index the ket class list itself (not owned)
+        if (!tasks.empty()) HFK_TRY(dev_upload(c, &e.d_tasks, tasks));
     }
+    // exact counts (shell, primitive and canonical function quartets) per
+    // launch, over the task lists: only the surviving quartets are visited
+    {
+        unsigned long long* d_cnt = nullptr;
+        const size_t np = c->plan.size();
+        HFK_TRY(dev_alloc(&d_cnt, 3 * std::max<size_t>(np, 1)));
+        hfk_launch_fill((double*)d_cnt, 0.0, (long long)(3 * std::max<size_t>(np, 1)), c->stream);
+        QuartetArgs a = hfk_ctx_args(c);
+        a.sch_cut = c->opt.schwarz_cut;
+        for (size_t i = 0; i < np; ++i) {
+            PlanEntry& e = c->plan[i];
+            hfk_plan_args(c, e, a);
+            hfk_launch_count_tasks(a, ncart(e.la), ncart(e.lb), ncart(e.lc), ncart(e.ld), d_cnt + 3 * i, c->stream);
+        }
+        HFK_HIP(hipGetLastError());
+        std::vector<unsigned long long> h(3 * std::max<size_t>(np, 1));
+        HFK_HIP(hipMemcpyAsync(h.data(), d_cnt, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
+        HFK_HIP(hipStreamSynchronize(c->stream));
+        dev_free(c, d_cnt);
+        long long fq = 0;
+        for (size_t i = 0; i < np; ++i) {
+            c->plan[i].shell_q = (long long)h[3 * i];
+            c->plan[i].prim_q = (long long)h[3 * i + 1];
+            c->plan[i].func_q = (long long)h[3 * i + 2];
+            fq += c->plan[i].func_q;
+        }
+        c->stats.func_quartets = fq;
+    }
+    const size_t np = c->plan.size();
     // event pool for per-launch timing
     while (c->events.size() < 2 * np + 2) {
         hipEvent_t ev;

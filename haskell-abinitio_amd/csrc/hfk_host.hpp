@@ -45,7 +45,7 @@ struct PlanEntry {
     double ms;                  // last timed duration
     int mode;                   // 0 main pass, 1 screen-correction pass (shares the main tasks)
     int4* d_tasks;              // tiled J/K tasks {bra pair, begin, end, 0} into d_kets
-    int* d_kets;                // explicit ket pair indices
+    int* d_kets;                // ket pair indices the tasks refer to (the ket class list)
     long long ntask;
     int bra_stage;              // longest bra primitive-pair range of the tasks
     int task_len_max;           // longest ket run of the tasks

@@ -105,6 +105,10 @@ void hfk_launch_boys_tables(const double* glx, const double* glw, double* ref64,
                             double* acc64, float* acc32, hipStream_t s);
 void hfk_launch_fill(double* p, double v, long long n, hipStream_t s);
 void hfk_launch_count(const QuartetArgs& a, int na, int nb, int nc, int nd, unsigned long long* out, hipStream_t s);
+// The same counts over a launch's task list (surviving quartets only): shell and
+// function quartets once per quartet, primitive quartets per primitive range.
+void hfk_launch_count_tasks(const QuartetArgs& a, int na, int nb, int nc, int nd, unsigned long long* out,
+                            hipStream_t s);
 // Quartets of the task list that hold a partially screened primitive quartet:
 // out==nullptr counts them into *count; otherwise appends {bra, ket} pairs.
 void hfk_launch_find_mixed(const QuartetArgs& a, unsigned long long* count, int2* out, hipStream_t s);

@@ -4,6 +4,7 @@
 // eigen-solve is rocSOLVER dsyevd; the dense products are the MFMA GEMM.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdio.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -190,6 +191,8 @@ extern "C" void hfk_default_scf_options(hfk_scf_options* o)
     o->max_steps = 30;      // HartreeFock.hs:87
     o->diis_size = 5;       // HartreeFock.hs:78
     o->diis_start = 0.1;    // DIIS.hs:58
+    o->verbose = 0;
+    o->pad = 0;
     o->converge = 1.0e-7;   // HartreeFock.hs:113
     o->charge = 0.0;
 }
@@ -400,6 +403,10 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
             res->max_error = maxe;
             res->steps = step + 1;
             res->occupied = occ;
+            if (opt.verbose) {
+                fprintf(stderr, "hfk_scf: step %d E %.10f max|e| %.3e%s\n", step + 1, etot, maxe, diis_on ? " diis" : "");
+                fflush(stderr);
+            }
             if (step != 0 && maxe < opt.converge) {
                 if (F_out) HFK_HIP(hipMemcpyAsync(F_out, Fuse, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
                 if (D_out) HFK_HIP(hipMemcpyAsync(D_out, Dflat, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));

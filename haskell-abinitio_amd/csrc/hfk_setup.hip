@@ -6,6 +6,7 @@
 #include <algorithm>
 
 #include "hfk_internal.hpp"
+#include "hfk_quartet_common.hpp"
 
 namespace {
 
@@ -338,6 +339,42 @@ __global__ void k_count(QuartetArgs a, int na, int nb, int nc, int nd, unsigned 
     atomicAdd(out + 2, cf);
 }
 
+// Counts over the task lists: one 64-lane workgroup per task (grid-stride).
+__global__ void __launch_bounds__(64) k_count_tasks(QuartetArgs a, int na, int nb, int nc, int nd,
+                                                    unsigned long long* out)
+{
+    unsigned long long cs = 0, cp = 0, cf = 0;
+    for (long long t = blockIdx.x; t < a.ntask; t += gridDim.x) {
+        const int4 task = a.tasks[t];
+        const int bp = task.x;
+        const int b0 = (task.w >> 24) & 0xff, b1 = (task.w >> 16) & 0xff;
+        const int k0 = (task.w >> 8) & 0xff, k1 = task.w & 0xff;
+        const bool first = (b0 == 0 && k0 == 0);   // each quartet counted once
+        const double* bra = a.pp + (size_t)a.pair_pp0[bp] * PP_STRIDE;
+        for (int j = task.y + (int)threadIdx.x; j < task.z; j += blockDim.x) {
+            const int kp = a.ket_idx[j];
+            if (!quartet_selected(a, bp, kp)) continue;
+            const double* ket = a.pp + (size_t)a.pair_pp0[kp] * PP_STRIDE;
+            for (int x = b0; x < b1; ++x) {
+                const double bq = bra[(size_t)x * PP_STRIDE + PP_QHI];
+                for (int y = k0; y < k1; ++y)
+                    if (!(a.qtau > 0.0 && bq * ket[(size_t)y * PP_STRIDE + PP_QHI] < a.qtau)) cp += 1;
+            }
+            if (first) {
+                cs += 1;
+                const unsigned long long nbra = (a.pair_A[bp] == a.pair_B[bp]) ? (unsigned long long)na * (na + 1) / 2
+                                                                                  : (unsigned long long)na * nb;
+                const unsigned long long nket = (a.pair_A[kp] == a.pair_B[kp]) ? (unsigned long long)nc * (nc + 1) / 2
+                                                                                  : (unsigned long long)nc * nd;
+                cf += (bp == kp) ? nbra * (nbra + 1) / 2 : nbra * nket;
+            }
+        }
+    }
+    if (cs) atomicAdd(out + 0, cs);
+    if (cp) atomicAdd(out + 1, cp);
+    if (cf) atomicAdd(out + 2, cf);
+}
+
 // One 64-lane workgroup per task (same layout as the tiled J/K kernel).
 __global__ void __launch_bounds__(64) k_find_mixed(QuartetArgs a, unsigned long long* count, int2* out)
 {
@@ -397,6 +434,14 @@ void hfk_launch_count(const QuartetArgs& a, int na, int nb, int nc, int nd, unsi
     long long g = (a.nquart + 255) / 256;
     if (g > 4096) g = 4096;
     hipLaunchKernelGGL(k_count, dim3((unsigned)g), dim3(256), 0, s, a, na, nb, nc, nd, out);
+}
+
+void hfk_launch_count_tasks(const QuartetArgs& a, int na, int nb, int nc, int nd, unsigned long long* out,
+                            hipStream_t s)
+{
+    if (a.ntask <= 0) return;
+    const long long g = std::min<long long>(a.ntask, 8192);
+    hipLaunchKernelGGL(k_count_tasks, dim3((unsigned)g), dim3(64), 0, s, a, na, nb, nc, nd, out);
 }
 
 void hfk_launch_find_mixed(const QuartetArgs& a, unsigned long long* count, int2* out, hipStream_t s)

@@ -166,7 +166,8 @@ class Context:
         check(_lib.lib().hfk_hcore(self._h, _ptr(H)))
         return H
 
-    def scfHF(self, charge=None, guess=HFK_GUESS_CORE, max_steps=30, diis_size=5, converge=1e-7, diis_start=0.1):
+    def scfHF(self, charge=None, guess=HFK_GUESS_CORE, max_steps=30, diis_size=5, converge=1e-7, diis_start=0.1,
+              verbose=False):
         """scfHF.  Defaults are the reference's (30 steps, 5 DIIS vectors,
         DIIS switched on once max|e| < 0.1, converged at max|e| < 1e-7);
         diis_start=float('inf') switches DIIS on from the second step."""
@@ -177,6 +178,7 @@ class Context:
         o.diis_size = diis_size
         o.converge = converge
         o.diis_start = diis_start
+        o.verbose = 1 if verbose else 0
         o.charge = self.mol.charge if charge is None else float(charge)
         r = _lib.HfkScfResult()
         n = self.n

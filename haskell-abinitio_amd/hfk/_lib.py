@@ -43,6 +43,7 @@ class HfkScfOptions(C.Structure):
     _fields_ = [
         ("guess", C.c_int32), ("max_steps", C.c_int32), ("diis_size", C.c_int32),
         ("diis_start", C.c_double), ("converge", C.c_double), ("charge", C.c_double),
+        ("verbose", C.c_int32), ("pad", C.c_int32),
     ]
 
 

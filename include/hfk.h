@@ -136,6 +136,8 @@ typedef struct hfk_scf_options {
     double diis_start;         /* reference: 0.1 (DIIS.hs:58)                          */
     double converge;           /* reference: max|X^T(FDS-SDF)X| < 1e-7 (HartreeFock.hs:113) */
     double charge;
+    int32_t verbose;           /* new: 1 = one progress line per step on stderr        */
+    int32_t pad;
 } hfk_scf_options;
 
 typedef struct hfk_scf_result {
