@@ -193,6 +193,7 @@ extern "C" void hfk_default_scf_options(hfk_scf_options* o)
     o->diis_start = 0.1;    // DIIS.hs:58
     o->verbose = 0;
     o->pad = 0;
+    o->guess_density = nullptr;
     o->converge = 1.0e-7;   // HartreeFock.hs:113
     o->charge = 0.0;
 }
@@ -271,6 +272,10 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
     for (double z : c->Z) zsum += z;
     const int occ = (int)floor((zsum - opt.charge) / 2.0);   // HartreeFock.hs:75-76
     if (occ <= 0 || occ > n) return hfk_set_error(HFK_EINVAL, "hfk_scf: occupied=%d for n=%d", occ, n);
+    if (opt.guess != HFK_GUESS_CORE && opt.guess != HFK_GUESS_INTENDED && opt.guess != HFK_GUESS_DENSITY)
+        return hfk_set_error(HFK_EINVAL, "hfk_scf: unknown guess %d", opt.guess);
+    if (opt.guess == HFK_GUESS_DENSITY && !opt.guess_density)
+        return hfk_set_error(HFK_EINVAL, "hfk_scf: HFK_GUESS_DENSITY needs guess_density");
     // nuclearRep (HartreeFock.hs:300-305)
     double vnn = 0.0;
     for (int a = 0; a < c->natom; ++a)
@@ -309,6 +314,8 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
         // initial density
         if (opt.guess == HFK_GUESS_CORE) {
             HFK_TRY(diagonal_hf(W, W.H, occ, Dflat));
+        } else if (opt.guess == HFK_GUESS_DENSITY) {
+            HFK_HIP(hipMemcpyAsync(Dflat, opt.guess_density, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
         } else {
             hfk_launch_guess_density(n, occ, Dflat, c->stream);
             auto f0 = std::chrono::steady_clock::now();

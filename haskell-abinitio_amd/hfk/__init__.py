@@ -27,14 +27,15 @@ from collections import namedtuple
 import numpy as np
 
 from . import _lib
-from ._lib import (HFK_BOYS_ACCURATE, HFK_BOYS_REFERENCE, HFK_GUESS_CORE, HFK_GUESS_INTENDED,
+from ._lib import (HFK_BOYS_ACCURATE, HFK_BOYS_REFERENCE, HFK_GUESS_CORE, HFK_GUESS_DENSITY, HFK_GUESS_INTENDED,
                    HFK_SCREEN_NONE, HFK_SCREEN_REFERENCE, HfkError, check)
 from .basis import Molecule, build_molecule
+from . import guess as _guess
 
 __all__ = [
     "Context", "HFData", "HfkError", "Molecule", "build_molecule", "options",
     "HFK_BOYS_ACCURATE", "HFK_BOYS_REFERENCE", "HFK_SCREEN_NONE", "HFK_SCREEN_REFERENCE",
-    "HFK_GUESS_CORE", "HFK_GUESS_INTENDED", "flat_len", "eri_flat_len",
+    "HFK_GUESS_CORE", "HFK_GUESS_DENSITY", "HFK_GUESS_INTENDED", "flat_len", "eri_flat_len",
 ]
 
 # HFData (GlobalTypes.hs:122-128) plus run statistics
@@ -167,13 +168,20 @@ class Context:
         return H
 
     def scfHF(self, charge=None, guess=HFK_GUESS_CORE, max_steps=30, diis_size=5, converge=1e-7, diis_start=0.1,
-              verbose=False):
+              verbose=False, guess_density=None):
         """scfHF.  Defaults are the reference's (30 steps, 5 DIIS vectors,
         DIIS switched on once max|e| < 0.1, converged at max|e| < 1e-7);
-        diis_start=float('inf') switches DIIS on from the second step."""
+        diis_start=float('inf') switches DIIS on from the second step.
+        guess_density: flat starting density (overrides guess)."""
         o = _lib.HfkScfOptions()
         _lib.lib().hfk_default_scf_options(C.byref(o))
         o.guess = guess
+        if guess_density is not None:
+            Dg = np.ascontiguousarray(guess_density, dtype=np.float64)
+            if Dg.shape != (flat_len(self.n),):
+                raise ValueError("guess_density has shape %s, expected (%d,)" % (Dg.shape, flat_len(self.n)))
+            o.guess = HFK_GUESS_DENSITY
+            o.guess_density = _ptr(Dg)
         o.max_steps = max_steps
         o.diis_size = diis_size
         o.converge = converge
@@ -188,6 +196,12 @@ class Context:
         F = np.zeros(flat_len(n))
         check(_lib.lib().hfk_scf(self._h, C.byref(o), C.byref(r), _ptr(Cm), _ptr(eps), _ptr(D), _ptr(F)))
         return HFData(F, Cm, D, eps, r.energy, r.steps, r.max_error, r.seconds, r.fock_seconds)
+
+    def fragment_guess(self, **scf_kw):
+        """Block-diagonal starting density from SCF runs on each bonded
+        fragment (same options as this context), or None for a single
+        fragment; pass it to scfHF(guess_density=...)."""
+        return _guess.fragment_guess(self.mol, lambda sub: Context(sub, self._opts), **scf_kw)
 
     def gradient(self, D_flat, W_flat=None):
         """Nuclear gradient dE/dR (natom x 3, Eh/bohr) at density D_flat.

@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(HERE, "libhfk.so")
 
 HFK_SCREEN_REFERENCE, HFK_SCREEN_NONE = 0, 1
 HFK_BOYS_REFERENCE, HFK_BOYS_ACCURATE = 0, 1
-HFK_GUESS_CORE, HFK_GUESS_INTENDED = 0, 1
+HFK_GUESS_CORE, HFK_GUESS_INTENDED, HFK_GUESS_DENSITY = 0, 1, 2
 
 # every symbol include/hfk.h declares
 EXPORTS = (
@@ -43,7 +43,7 @@ class HfkScfOptions(C.Structure):
     _fields_ = [
         ("guess", C.c_int32), ("max_steps", C.c_int32), ("diis_size", C.c_int32),
         ("diis_start", C.c_double), ("converge", C.c_double), ("charge", C.c_double),
-        ("verbose", C.c_int32), ("pad", C.c_int32),
+        ("verbose", C.c_int32), ("pad", C.c_int32), ("guess_density", C.POINTER(C.c_double)),
     ]
 
 

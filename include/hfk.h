@@ -126,6 +126,7 @@ int hfk_overlap(hfk_ctx *ctx, double *S_flat);
 int hfk_hcore(hfk_ctx *ctx, double *H_flat);
 
 #define HFK_GUESS_CORE 0       /* D from diagonalising H                               */
+#define HFK_GUESS_DENSITY 2    /* new: start from hfk_scf_options.guess_density        */
 #define HFK_GUESS_INTENDED 1   /* Harris step with D = 2 on the first occ diagonal AOs,
                                   correctly indexed (HartreeFock.hs:128-149 intent)     */
 
@@ -138,6 +139,7 @@ typedef struct hfk_scf_options {
     double charge;
     int32_t verbose;           /* new: 1 = one progress line per step on stderr        */
     int32_t pad;
+    const double *guess_density; /* new: flat starting density for HFK_GUESS_DENSITY     */
 } hfk_scf_options;
 
 typedef struct hfk_scf_result {
