@@ -20,6 +20,7 @@ index resolved at generation time, so the compiler keeps it in registers.
 
 Usage: python gen_eri.py LMAX OUTDIR
 """
+import os
 import sys
 
 
@@ -126,7 +127,7 @@ def gen_class(la, lb, lc, ld):
         em.raw("    }")
     Ek = [hermite_E(em, lc, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
     out.extend(em.lines)
-    unroll = 2 if nbra * nket <= UNROLL2_NV else 1
+    pipe = nbra * nket <= PIPE_NV
 
     def body(em2, sx, ibx):
         """One bra primitive pair (record pb<sx>, index ibx): returns
@@ -143,8 +144,8 @@ def gen_class(la, lb, lc, ld):
             em2.raw("      }")
         Eb = [hermite_E(em2, la, lb, "bA" + ax + sx, "bB" + ax + sx, "boo2p" + sx, "b" + ax) for ax in "xyz"]
         em2.raw("      T F%s[%d];" % (sx, L + 1))
-        if unroll == 2:
-            em2.raw("      boys_select<T, %d, BOYS>(Targ%s, bt, F%s);" % (L, sx, sx))
+        if pipe:
+            em2.raw("      boys_finish<T, %d, BOYS>(bf%s, bt, F%s);" % (L, sx, sx))
         else:
             em2.raw("      boys_fast<T, %d, BOYS>(Targ%s, bt, F%s);" % (L, sx, sx))
         em2.raw("      const T pref%s = T(HFK_TWO_PI_2_5) * bK%s * kK * rs%s;" % (sx, sx, sx))
@@ -239,7 +240,7 @@ def gen_class(la, lb, lc, ld):
     em2 = Emitter()
     em2.n = em.n
     multi = nket > 1 or nbra > 1
-    if unroll == 1:
+    if not pipe:
         out.append("    for (int ib = 0; ib < nbp; ++ib) {")
         out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
         out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
@@ -249,41 +250,45 @@ def gen_class(la, lb, lc, ld):
         out.append("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
         out.append("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
         out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
-        vals = body(em2, "", "ib")
-        for vi in sorted(vals):
-            name_, bi, ki = vals[vi]
-            if multi:
-                em2.raw("      if constexpr (MODE == 0) V[%d] += %s;" % (vi, name_))
-                em2.raw("      else V[%d] += (allin || !(bqv[%d] * kqv[%d] < qtau)) ? %s : T(0);" % (vi, bi, ki, name_))
-            else:
-                em2.raw("      V[%d] += %s;" % (vi, name_))
     else:
-        # two bra primitive pairs per iteration: independent chains the
-        # scheduler interleaves; each contribution carries its own screen
-        out.append("    for (int ib = 0; ib < nbp; ib += 2) {")
-        out.append("      const bool two = ib + 1 < nbp;")
-        out.append("      const double* pb0 = bra + (size_t)ib * PP_STRIDE;")
-        out.append("      const double* pb1 = two ? pb0 + PP_STRIDE : pb0;")
-        out.append("      // reference screen: component (ab|cd) of a primitive quartet is kept iff")
-        out.append("      // !(q_ab*q_cd < tau).  MODE 0: no screen; MODE 1: the screen, branch-free.")
-        out.append("      const bool on0 = !(MODE == 1 && pb0[PP_QHI] * kqhi < qtau);")
-        out.append("      const bool on1 = two && !(MODE == 1 && pb1[PP_QHI] * kqhi < qtau);")
-        out.append("      if (!on0 && !on1) continue;   // every component of both screened")
-        out.append("      const bool allin0 = MODE == 0 || (pb0[PP_QLO] * kqlo >= qtau);")
-        out.append("      const bool allin1 = MODE == 0 || (pb1[PP_QLO] * kqlo >= qtau);")
-        out.append("      T alpha0, Xpq0, Ypq0, Zpq0, rs0, Targ0, alpha1, Xpq1, Ypq1, Zpq1, rs1, Targ1;")
-        out.append("      quartet_geom<T>(pb0, kq, kQx, kQy, kQz, alpha0, Xpq0, Ypq0, Zpq0, rs0, Targ0);")
-        out.append("      quartet_geom<T>(pb1, kq, kQx, kQy, kQz, alpha1, Xpq1, Ypq1, Zpq1, rs1, Targ1);")
-        v0 = body(em2, "0", "ib")
-        v1 = body(em2, "1", "ib + (two ? 1 : 0)")
-        for vi in sorted(v0):
-            n0, bi, ki = v0[vi]
-            n1 = v1[vi][0]
-            if multi:
-                em2.raw("      V[%d] += ((on0 && (allin0 || !(bqv0[%d] * kqv[%d] < qtau))) ? %s : T(0)) + "
-                        "((on1 && (allin1 || !(bqv1[%d] * kqv[%d] < qtau))) ? %s : T(0));" % (vi, bi, ki, n0, bi, ki, n1))
-            else:
-                em2.raw("      V[%d] += (on0 ? %s : T(0)) + (on1 ? %s : T(0));" % (vi, n0, n1))
+        # software pipeline: the geometry and Boys table row of bra primitive
+        # pair ib+1 are fetched before ib is evaluated, so that the table
+        # loads are in flight during the arithmetic of ib
+        out.append("    T alphaN = T(0), XpqN = T(0), YpqN = T(0), ZpqN = T(0), rsN = T(0), TargN = T(0);")
+        out.append("    BoysFetch<T, %d, BOYS> bfN;" % L)
+        out.append("    bool onN = false;")
+        out.append("    if (nbp > 0) {")
+        out.append("      onN = !(MODE == 1 && bra[PP_QHI] * kqhi < qtau);")
+        out.append("      if (onN) {")
+        out.append("        quartet_geom<T>(bra, kq, kQx, kQy, kQz, alphaN, XpqN, YpqN, ZpqN, rsN, TargN);")
+        out.append("        boys_fetch<T, %d, BOYS>(TargN, bt, bfN);" % L)
+        out.append("      }")
+        out.append("    }")
+        out.append("    for (int ib = 0; ib < nbp; ++ib) {")
+        out.append("      const bool on = onN;")
+        out.append("      const T alpha = alphaN, Xpq = XpqN, Ypq = YpqN, Zpq = ZpqN, rs = rsN;")
+        out.append("      const BoysFetch<T, %d, BOYS> bf = bfN;" % L)
+        out.append("      if (ib + 1 < nbp) {")
+        out.append("        const double* pn = bra + (size_t)(ib + 1) * PP_STRIDE;")
+        out.append("        // reference screen: component (ab|cd) of a primitive quartet is kept iff")
+        out.append("        // !(q_ab*q_cd < tau); a pair whose largest value fails is skipped")
+        out.append("        onN = !(MODE == 1 && pn[PP_QHI] * kqhi < qtau);")
+        out.append("        if (onN) {")
+        out.append("          quartet_geom<T>(pn, kq, kQx, kQy, kQz, alphaN, XpqN, YpqN, ZpqN, rsN, TargN);")
+        out.append("          boys_fetch<T, %d, BOYS>(TargN, bt, bfN);" % L)
+        out.append("        }")
+        out.append("      }")
+        out.append("      if (!on) continue;")
+        out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
+        out.append("      const bool allin = MODE == 0 || (pb[PP_QLO] * kqlo >= qtau);")
+    vals = body(em2, "", "ib")
+    for vi in sorted(vals):
+        name_, bi, ki = vals[vi]
+        if multi:
+            em2.raw("      if constexpr (MODE == 0) V[%d] += %s;" % (vi, name_))
+            em2.raw("      else V[%d] += (allin || !(bqv[%d] * kqv[%d] < qtau)) ? %s : T(0);" % (vi, bi, ki, name_))
+        else:
+            em2.raw("      V[%d] += %s;" % (vi, name_))
     out.extend(em2.lines)
     out.append("    }")
     out.append("  }")
@@ -515,9 +520,10 @@ def gen_grad_class(la, lb, lc, ld):
 # them); the host's hfk_class_coop() uses the same bounds.
 COOP_NV_JK = 54
 COOP_NV_GRAD = 18
-# Classes with at most this many integrals per quartet evaluate two bra
-# primitive pairs per loop iteration (two independent dependency chains).
-UNROLL2_NV = 0
+# Classes with at most this many integrals per quartet software-pipeline the
+# bra primitive loop (geometry and Boys table row of the next primitive pair
+# fetched before the current one is evaluated).
+PIPE_NV = int(os.environ.get("HFK_PIPE_NV", "0"))
 
 
 def pair_classes(lmax):

@@ -795,8 +795,7 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
                     for (int z = li; z < NKET; z += NBRA) {
                         const double v = sm.jcd[slot][z];
                         if (v != 0.0)
-                            atomic_add_f64(&a.Jc[(size_t)(blockIdx.x % (unsigned)a.jcopies) * ld * ld +
-                                                 (size_t)ic[z / ND] * ld + id[z % ND]], v);
+                            atomic_add_f64(&a.Jk[((size_t)(blockIdx.x % (unsigned)a.jk_copies) * a.nket + kbeg + it) * NKET + z], v);
                         sm.jcd[slot][z] = 0.0;
                     }
                 }
@@ -831,12 +830,13 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
             if (active && jab != 0.0) atomic_add_f64(&a.J[(size_t)ia * ld + ib], jab);
             if (lds) {
                 __syncthreads();
-                for (int t = lane; t < (NA + NB) * ld; t += 64) {
-                    const double v = dyn[t];
-                    if (v != 0.0) {
-                        const int r = t / ld, col = t - r * ld;
-                        const int row = (r < NA) ? a.sh_bfp[A * HFK_MAXCOMP + r] : a.sh_bfp[B * HFK_MAXCOMP + r - NA];
-                        atomic_add_f64(&a.K[(size_t)row * ld + col], v);
+                for (int r = 0; r < NA + NB; ++r) {
+                    const int row = (r < NA) ? a.sh_bfp[A * HFK_MAXCOMP + r] : a.sh_bfp[B * HFK_MAXCOMP + r - NA];
+                    double* __restrict__ Kr = a.K + (size_t)row * ld;
+                    const double* sr = dyn + r * ld;
+                    for (int col = lane; col < ld; col += 64) {
+                        const double v = sr[col];
+                        if (v != 0.0) atomic_add_f64(&Kr[col], v);
                     }
                 }
             }

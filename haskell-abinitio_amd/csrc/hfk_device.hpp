@@ -116,13 +116,6 @@ __device__ __forceinline__ void boys_eval(T x, const BoysTables& bt, T* F)
 // Atomic add of a double into global memory with the hardware FP64 atomic.
 __device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
 
-// ---------------------------------------------------------------------------
-// Boys function in two steps, so that a loop can fetch the table row of the
-// next primitive quartet before it evaluates the current one:
-//   boys_row_load(x)  -> the node values F_m(x_k) the Taylor step needs (loads)
-//   boys_from_row     -> F_0..F_L (arithmetic only)
-// Same mathematics as boys_eval.
-
 // floor(x / 0.1) of the reference (TableBoys.hs:47-48) without a division:
 // x*10 and x/0.1 differ by less than 2^-51 relative, so their floors can
 // differ only when x*10 lies within that distance of an integer; those rare
@@ -156,83 +149,6 @@ __device__ __forceinline__ float rsqrt_full(float x)
     y = y * (1.5f - (0.5f * x) * (y * y));
     return y;
 }
-
-template <typename T, int L, int BOYS>
-struct BoysRow {
-    static constexpr int NF = (BOYS == 0) ? L + 6 : L + 8;
-    T f[NF];   // node values F_m(x_k), m = 0..NF-1 (unused when asymptotic)
-    T dx;      // x - x_k
-    T xk;      // the node
-};
-
-template <typename T, int L, int BOYS>
-__device__ __forceinline__ void boys_row_load(T x, const BoysTables& bt, BoysRow<T, L, BOYS>& r)
-{
-    if constexpr (BOYS == 0) {
-        const T fl = boys_ref_cell(x);
-        r.xk = fl * T(0.1);
-        r.dx = x - r.xk;
-        int idx = (int)fl;
-        idx = idx < 0 ? 0 : (idx > BOYS_NX - 1 ? BOYS_NX - 1 : idx);   // also x_k > 100 and NaN
-        const T* tab = boys_ref_tab<T>(bt) + (size_t)idx * BOYS_REF_STRIDE;
-#pragma unroll
-        for (int m = 0; m < r.NF; ++m) r.f[m] = tab[m];
-    } else {
-        const T fl = rint(x * T(10));
-        r.xk = x;
-        r.dx = x - fl * T(0.1);
-        int idx = (int)fl;
-        idx = idx < 0 ? 0 : (idx > BOYS_ACC_NX - 1 ? BOYS_ACC_NX - 1 : idx);
-        const T* tab = boys_acc_tab<T>(bt) + (size_t)idx * BOYS_ACC_STRIDE;
-#pragma unroll
-        for (int m = 0; m < r.NF; ++m) r.f[m] = tab[m];
-    }
-}
-
-template <typename T, int L, int BOYS>
-__device__ __forceinline__ void boys_from_row(T x, const BoysRow<T, L, BOYS>& r, const BoysTables& bt, T* F)
-{
-    if constexpr (BOYS == 0) {
-        T f[L + 6];
-#pragma unroll
-        for (int m = 0; m < L + 6; ++m) f[m] = r.f[m];
-        if (r.xk > T(100)) {
-            // asymptBoysF at the node: a_m sqrt(pi / x_k^(2m+1))
-            const T rs = rsqrt_full(r.xk * T(1.0 / HFK_PI));   // sqrt(pi / x_k)
-            const T inv = (rs * rs) * T(1.0 / HFK_PI);         // 1 / x_k
-            T s = rs;
-#pragma unroll
-            for (int m = 0; m < L + 6; ++m) {
-                f[m] = T(bt.asym[m]) * s;
-                s *= inv;
-            }
-        }
-        const T dx = r.dx;
-        const T d2 = dx * dx, d3 = d2 * dx, d4 = d2 * d2, d5 = d4 * dx;
-#pragma unroll
-        for (int m = 0; m <= L; ++m)
-            F[m] = f[m] - f[m + 1] * dx + (T(0.5) * f[m + 2]) * d2 - (T(1.0 / 6.0) * f[m + 3]) * d3
-                   + (T(1.0 / 24.0) * f[m + 4]) * d4 - (T(1.0 / 120.0) * f[m + 5]) * d5;
-    } else {
-        if (x < T(BOYS_ACC_XMAX)) {
-            const T mdx = -r.dx;
-#pragma unroll
-            for (int m = 0; m <= L; ++m) {
-                T acc = r.f[m + 7];
-#pragma unroll
-                for (int k = 7; k >= 1; --k) acc = r.f[m + k - 1] + (mdx / T(k)) * acc;
-                F[m] = acc;
-            }
-        } else {
-            const T inv = T(1) / x;
-            const T e = (T)exp((double)-x);
-            F[0] = T(0.5) * sqrt_t(T(HFK_PI) * inv);
-#pragma unroll
-            for (int m = 0; m < L; ++m) F[m + 1] = (T(2 * m + 1) * F[m] - e) * (T(0.5) * inv);
-        }
-    }
-}
-
 
 // F_0..F_L for the per-lane kernels, with as few instructions as possible.
 //  BOYS == 0: x_k <= 100: the reference's 6-term Taylor step from the node
@@ -272,6 +188,59 @@ __device__ __forceinline__ void boys_fast(T x, const BoysTables& bt, T* F)
         }
     } else {
         boys_eval<T, L, BOYS>(x, bt, F);
+    }
+}
+
+
+// boys_fast in two steps, so that a loop can fetch the table row of the next
+// primitive quartet before it evaluates the current one:
+//   boys_fetch(x)  -> the node values F_m(x_k) (loads only; none for x_k > 100)
+//   boys_finish    -> F_0..F_L, the same arithmetic as boys_fast.
+template <typename T, int L, int BOYS>
+struct BoysFetch {
+    T f[BOYS == 0 ? L + 6 : 1];
+    T x, xk;
+};
+
+template <typename T, int L, int BOYS>
+__device__ __forceinline__ void boys_fetch(T x, const BoysTables& bt, BoysFetch<T, L, BOYS>& r)
+{
+    r.x = x;
+    if constexpr (BOYS == 0) {
+        const T fl = boys_ref_cell(x);
+        r.xk = fl * T(0.1);
+        if (r.xk <= T(100)) {
+            int idx = (int)fl;
+            idx = idx < 0 ? 0 : idx;   // NaN guard
+            const T* tab = boys_ref_tab<T>(bt) + (size_t)idx * BOYS_REF_STRIDE;
+#pragma unroll
+            for (int m = 0; m < L + 6; ++m) r.f[m] = tab[m];
+        }
+    }
+}
+
+template <typename T, int L, int BOYS>
+__device__ __forceinline__ void boys_finish(const BoysFetch<T, L, BOYS>& r, const BoysTables& bt, T* F)
+{
+    if constexpr (BOYS == 0) {
+        if (r.xk <= T(100)) {
+            const T d = r.xk - r.x;   // -dx
+            const T w2 = (T(0.5) * d) * d, w3 = (T(1.0 / 3.0) * d) * w2, w4 = (T(0.25) * d) * w3, w5 = (T(0.2) * d) * w4;
+#pragma unroll
+            for (int m = 0; m <= L; ++m)
+                F[m] = r.f[m] + d * r.f[m + 1] + w2 * r.f[m + 2] + w3 * r.f[m + 3] + w4 * r.f[m + 4] + w5 * r.f[m + 5];
+        } else {
+            const T rs = rsqrt_full(r.x * T(1.0 / HFK_PI));   // sqrt(pi / x)
+            const T inv = (rs * rs) * T(1.0 / HFK_PI);        // 1 / x
+            T s = rs;
+#pragma unroll
+            for (int m = 0; m <= L; ++m) {
+                F[m] = T(bt.asym[m]) * s;
+                s *= inv;
+            }
+        }
+    } else {
+        boys_eval<T, L, BOYS>(r.x, bt, F);
     }
 }
 

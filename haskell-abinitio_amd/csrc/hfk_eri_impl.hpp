@@ -129,7 +129,8 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 //   J'_ab += 2f sum_cd V D_cd      lane registers, one wave reduction per task
 //   K'_a. , K'_b.                  rows of the two bra shells staged in LDS
 //                                  (ds_add_f64), flushed once per task
-//   J'_cd += 2f sum_ab V D_ab      one global FP64 atomic per ket function pair
+//   J'_cd += 2f sum_ab V D_ab      one global FP64 atomic per ket function pair,
+//                                  by ket position (coalesced), scattered into J' later
 // ---------------------------------------------------------------------
 // Workgroups are 64 lanes, or 256 lanes (4 waves sharing one set of staged K
 // rows) when the rows are long enough to limit occupancy through LDS.
@@ -181,9 +182,10 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
 #pragma unroll
     for (int k = 0; k < NBRA; ++k) jab[k] = 0.0;
 
-    // ket-side J'_cd: every task of the launch adds into the same few ket
-    // pairs, so workgroups spread those atomics over private copies of J'
-    double* __restrict__ Jc = a.Jc + (size_t)(blockIdx.x % (unsigned)a.jcopies) * ld * ld;
+    // ket-side J'_cd: summed by position in the ket class list, so the
+    // lanes' atomics fall on consecutive addresses; tasks resident at the same
+    // time share ket runs, so workgroups spread them over a few copies
+    double* __restrict__ Jk = a.Jk + (size_t)(blockIdx.x % (unsigned)a.jk_copies) * a.nket * NKET;
     const int len = task.z - task.y;
     const int rot = (int)((unsigned)task.x * 37u % (unsigned)(len > 0 ? len : 1));
     for (int it = tid; it < len; it += nthr) {
@@ -234,7 +236,7 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
             double s = 0.0;
 #pragma unroll
             for (int b = 0; b < NBRA; ++b) s += V[b * NKET + k] * Dab[b];
-            atomic_add_f64(&Jc[(size_t)ic[k / ND] * ld + id[k % ND]], 2.0 * s);
+            atomic_add_f64(&Jk[(size_t)(task.y + jj) * NKET + k], 2.0 * s);
         }
         // density blocks D_bd, D_bc, D_ad, D_ac
         double Dbd[NB * ND], Dbc[NB * NC], Dad[NA * ND], Dac[NA * NC];
@@ -308,15 +310,16 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
         for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
         if (lane == 0 && v != 0.0) atomic_add_f64(&a.J[(size_t)ia[b / NB] * ld + ib[b % NB]], 2.0 * v);
     }
-    // flush the staged K rows
+    // flush the staged K rows (row by row: no index division)
     if (lds) {
         __syncthreads();
-        for (int t = tid; t < (NA + NB) * ld; t += nthr) {
-            const double v = krow[t];
-            if (v != 0.0) {
-                const int r = t / ld, col = t - r * ld;
-                const int row = (r < NA) ? ia[r] : ib[r - NA];
-                atomic_add_f64(&a.K[(size_t)row * ld + col], v);
+#pragma unroll
+        for (int r = 0; r < NA + NB; ++r) {
+            double* __restrict__ Kr = a.K + (size_t)(r < NA ? ia[r] : ib[r - NA]) * ld;
+            const double* sr = krow + r * ld;
+            for (int col = tid; col < ld; col += nthr) {
+                const double v = sr[col];
+                if (v != 0.0) atomic_add_f64(&Kr[col], v);
             }
         }
     }

@@ -320,9 +320,6 @@ int hfk_ctx_init_device(hfk_ctx* c)
     HFK_TRY(dev_alloc(&c->d_D, np1));
     HFK_TRY(dev_alloc(&c->d_J, np1));
     HFK_TRY(dev_alloc(&c->d_K, np1));
-    // copies of J' for the ket-side atomics: up to 64, about 32 MB in all
-    c->jcopies = (int)std::max<size_t>(1, std::min<size_t>(64, ((size_t)32 << 20) / (np1 * sizeof(double))));
-    HFK_TRY(dev_alloc(&c->d_Jc, np1 * (size_t)c->jcopies));
     HFK_TRY(dev_alloc(&c->d_flatA, nf));
     HFK_TRY(dev_alloc(&c->d_flatB, nf));
     HFK_TRY(dev_alloc(&c->d_flatC, nf));
@@ -543,7 +540,11 @@ int hfk_ctx_plan(hfk_ctx* c)
         const bool coop = hfk_class_coop(e.la, e.lb, e.lc, e.ld);
         // per-lane kernel: one ket per lane of the workgroup; cooperative kernel:
         // 64 / (na nb) kets at a time, four rounds per task
+#ifdef HFK_EXP_B64
+        const int chunk = coop ? 4 * (64 / (ncart(e.la) * ncart(e.lb))) : 64 * c->kets_per_lane;
+#else
         const int chunk = coop ? 4 * (64 / (ncart(e.la) * ncart(e.lb))) : (c->n >= 200 ? 256 : 64) * c->kets_per_lane;
+#endif
         const std::vector<int>& bl = c->h_class[e.kb];
         const std::vector<int>& kl = c->h_class[e.kk];
         std::vector<double> kq(kl.size());
@@ -658,6 +659,21 @@ int hfk_ctx_plan(hfk_ctx* c)
         }
         c->stats.func_quartets = fq;
     }
+    // ket-side J'_cd sums: one region per launch, [copies][nket][NKET]
+    {
+        size_t per = 0;
+        for (const PlanEntry& e : c->plan) per += (size_t)c->class_count[e.kk] * ncart(e.lc) * ncart(e.ld);
+        c->jk_copies = (int)std::max<size_t>(1, std::min<size_t>(16, per ? ((size_t)64 << 20) / (per * sizeof(double)) : 1));
+        size_t off = 0;
+        for (PlanEntry& e : c->plan) {
+            e.jk_off = off;
+            off += (size_t)c->jk_copies * c->class_count[e.kk] * ncart(e.lc) * ncart(e.ld);
+        }
+        if (c->d_Jk) dev_free(c, c->d_Jk);
+        c->d_Jk = nullptr;
+        c->jk_len = off;
+        HFK_TRY(dev_alloc(&c->d_Jk, std::max<size_t>(off, 1)));
+    }
     const size_t np = c->plan.size();
     // event pool for per-launch timing
     while (c->events.size() < 2 * np + 2) {
@@ -675,12 +691,20 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
     a.ntask = e.ntask;
     a.bra_stage = e.bra_stage;
     a.task_len_max = e.task_len_max;
+#ifdef HFK_EXP_B64
+    a.block = 64;
+#else
     a.block = (c->n >= 200 && !hfk_class_coop(e.la, e.lb, e.lc, e.ld)) ? 256 : 64;
+#endif
     {
         const int nr = (e.la + 1) * (e.la + 2) / 2 + (e.lb + 1) * (e.lb + 2) / 2;
         // the cooperative kernel also holds ~12 KB of static LDS
+#ifdef HFK_EXP_KCAP
+        const size_t cap = hfk_class_coop(e.la, e.lb, e.lc, e.ld) ? 48 * 1024 : HFK_EXP_KCAP;
+#else
         const size_t cap = hfk_class_coop(e.la, e.lb, e.lc, e.ld) ? 48 * 1024
                                                                   : 64 * 1024 - hfk_bra_stage_bytes(e.la, e.lb, e.bra_stage);
+#endif
         a.lds_rows = ((size_t)nr * (c->n + 1) * sizeof(double) <= cap) ? 1 : 0;
     }
     a.bra_list = c->d_class[e.kb];
@@ -696,7 +720,7 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
 // Run the plan for one operation (J/K or flat export); per-launch HIP events.
 int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
 {
-    if (op == HFK_OP_JK && (!a.Jc || a.jcopies < 1 || !a.J || !a.K || !a.D))
+    if (op == HFK_OP_JK && (!c->d_Jk || !a.J || !a.K || !a.D))
         return hfk_set_error(HFK_EINVAL, "J/K launch without its accumulators");
     long long total = 0;
     const size_t np = c->plan.size();
@@ -705,6 +729,10 @@ int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
         if (op != HFK_OP_JK && e.prec == 1) continue;   // exports are FP64
         hfk_plan_args(c, e, a);
         if (op != HFK_OP_JK) a.prec_sel = 0;
+        if (op == HFK_OP_JK) {
+            a.Jk = c->d_Jk + e.jk_off;
+            a.jk_copies = c->jk_copies;
+        }
         total += e.nquart;
         HFK_HIP(hipEventRecord(c->events[2 * i], c->stream));
         int rc = hfk_launch_quartets(e.la, e.lb, e.lc, e.ld, op, e.prec, c->opt.boys_mode, op == HFK_OP_JK ? e.mode : 0, a, c->stream);
@@ -748,6 +776,7 @@ extern "C" int hfk_get_class_stats(hfk_ctx* c, hfk_class_stat* out, int32_t cap,
         s.prim_quartets = e.prim_q;
         s.func_quartets = e.func_q;
         s.ms = e.ms;
+        s.tasks = e.ntask;
     }
     return HFK_OK;
 }
@@ -759,18 +788,18 @@ int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_fla
     hfk_launch_flat_to_padded(c->n, D_dev_flat, c->d_D, c->stream);
     hfk_launch_fill(c->d_J, 0.0, (long long)np1, c->stream);
     hfk_launch_fill(c->d_K, 0.0, (long long)np1, c->stream);
-    // timed: the J' copies (cleared, then summed) belong to the digestion
+    // timed: the ket-side sums (cleared, then scattered) belong to the digestion
     HFK_HIP(hipEventRecord(c->ev0, c->stream));
-    hfk_launch_fill(c->d_Jc, 0.0, (long long)(np1 * (size_t)c->jcopies), c->stream);
+    if (c->jk_len) hfk_launch_fill(c->d_Jk, 0.0, (long long)c->jk_len, c->stream);
     QuartetArgs a = hfk_ctx_args(c);
     a.sch_cut = c->opt.schwarz_cut;
     a.D = c->d_D;
     a.J = c->d_J;
     a.K = c->d_K;
-    a.Jc = c->d_Jc;
-    a.jcopies = c->jcopies;
     HFK_TRY(hfk_ctx_run_quartets(c, a, HFK_OP_JK));
-    hfk_launch_sum_copies(c->d_J, c->d_Jc, c->jcopies, (long long)np1, c->stream);
+    for (const PlanEntry& e : c->plan)
+        hfk_launch_scatter_jk(c->d_J, c->d_Jk + e.jk_off, c->jk_copies, c->class_count[e.kk], ncart(e.lc), ncart(e.ld),
+                              c->d_class[e.kk], c->d_pair_A, c->d_pair_B, c->d_sh_bfp, c->n + 1, c->stream);
     HFK_HIP(hipEventRecord(c->ev1, c->stream));
     hfk_launch_symmetrize_g(c->n, c->d_J, c->d_K, H_dev, F_dev_flat, nullptr, c->stream);
     HFK_HIP(hipGetLastError());

@@ -49,6 +49,7 @@ struct PlanEntry {
     long long ntask;
     int bra_stage;              // longest bra primitive-pair range of the tasks
     int task_len_max;           // longest ket run of the tasks
+    size_t jk_off;              // its ket-side J'_cd sums in hfk_ctx::d_Jk
 };
 
 struct hfk_ctx {
@@ -84,8 +85,9 @@ struct hfk_ctx {
     int *d_pair_A = nullptr, *d_pair_B = nullptr, *d_pair_pp0 = nullptr, *d_pair_npp = nullptr;
     double *d_pp = nullptr, *d_qcomp = nullptr, *d_pair_qhi = nullptr, *d_pair_sch = nullptr, *d_pair_diag = nullptr;
     double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr;
-    double* d_Jc = nullptr;     // private copies of J' for the ket-side sums
-    int jcopies = 1;
+    double* d_Jk = nullptr;     // ket-side J'_cd sums of all J/K launches, by ket position
+    size_t jk_len = 0;          // doubles in d_Jk
+    int jk_copies = 1;
     double *d_flatA = nullptr, *d_flatB = nullptr, *d_flatC = nullptr;
     int *d_sh_atom = nullptr, *d_atom_of_bf = nullptr;   // gradient (uploaded on first use)
     int* d_vpair = nullptr;                 // function pairs grouped by l_i + l_j (one-electron V)

@@ -36,8 +36,8 @@ struct QuartetArgs {
     int n;                      // number of basis functions
     const double* D;            // [n+1][n+1] density (symmetric), row/column n zero
     double* J;                  // [n+1][n+1] accumulator J' (symmetrised on output)
-    double* Jc;                 // jcopies private copies of J' for the ket-side J'_cd sums
-    int jcopies;                // (workgroup w adds into copy w % jcopies; summed into J)
+    double* Jk;                 // ket-side sums J'_cd by ket position: [jk_copies][nket][NKET]
+    int jk_copies;              // (workgroup w adds into copy w % jk_copies; scattered into J)
     double* K;                  // [n+1][n+1] accumulator K'
     double* flat;               // flat ERI output (calcIntegrals order)
     const long long* flat_base; // [n][n] rank of (i,j,i,j)
@@ -115,8 +115,10 @@ void hfk_launch_find_mixed(const QuartetArgs& a, unsigned long long* count, int2
 // J, K padded (leading dimension n+1)
 void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const double* H, double* G_flat,
                              double* F_full, hipStream_t s);
-// J += sum of the `copies` consecutive arrays of Jc (each `count` long)
-void hfk_launch_sum_copies(double* J, const double* Jc, int copies, long long count, hipStream_t s);
+// J'[c][d] += sum over copies of Jk[copy][pos][z] for the ket pairs ket_list[pos]
+// (components z = (c, d) of an nc x nd class); ld = n + 1
+void hfk_launch_scatter_jk(double* J, const double* Jk, int copies, long long nket, int nc, int nd, const int* ket_list,
+                           const int* pair_A, const int* pair_B, const int* sh_bfp, int ld, hipStream_t s);
 // flat -> padded (n+1)x(n+1) full, row/column n zero
 void hfk_launch_flat_to_padded(int n, const double* flat, double* full, hipStream_t s);
 void hfk_launch_flat_to_full(int n, const double* flat, double* full, hipStream_t s);

@@ -469,19 +469,34 @@ void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const doub
     hipLaunchKernelGGL(k_symmetrize_g, dim3(grid_for(m, 256)), dim3(256), 0, s, n, J, K, H, G_flat, F_full);
 }
 
-__global__ void k_sum_copies(double* J, const double* Jc, int copies, long long count)
+// Ket-side J'_cd sums, kept by position in the ket class list during the
+// J/K launch, added into J' (atomic: absent components of different pairs
+// share the zero row/column n).
+__global__ void k_scatter_jk(double* J, const double* Jk, int copies, long long nket, int nc, int nd,
+                             const int* ket_list, const int* pair_A, const int* pair_B, const int* sh_bfp, int ld)
 {
+    const int nz = nc * nd;
+    const long long count = nket * nz;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
          i += (long long)gridDim.x * blockDim.x) {
-        double v = J[i];
-        for (int c = 0; c < copies; ++c) v += Jc[(size_t)c * count + i];
-        J[i] = v;
+        double v = 0.0;
+        for (int c = 0; c < copies; ++c) v += Jk[(size_t)c * count + i];
+        if (v == 0.0) continue;
+        const long long pos = i / nz;
+        const int z = (int)(i - pos * nz);
+        const int kp = ket_list[pos];
+        const int ic = sh_bfp[pair_A[kp] * HFK_MAXCOMP + z / nd], id = sh_bfp[pair_B[kp] * HFK_MAXCOMP + z % nd];
+        atomic_add_f64(&J[(size_t)ic * ld + id], v);
     }
 }
 
-void hfk_launch_sum_copies(double* J, const double* Jc, int copies, long long count, hipStream_t s)
+void hfk_launch_scatter_jk(double* J, const double* Jk, int copies, long long nket, int nc, int nd, const int* ket_list,
+                           const int* pair_A, const int* pair_B, const int* sh_bfp, int ld, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sum_copies, dim3(grid_for(count, 256)), dim3(256), 0, s, J, Jc, copies, count);
+    const long long count = nket * nc * nd;
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_scatter_jk, dim3(grid_for(count, 256)), dim3(256), 0, s, J, Jk, copies, nket, nc, nd,
+                       ket_list, pair_A, pair_B, sh_bfp, ld);
 }
 
 void hfk_launch_flat_to_full(int n, const double* flat, double* full, hipStream_t s)

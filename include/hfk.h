@@ -187,6 +187,7 @@ typedef struct hfk_class_stat {
     int64_t prim_quartets;        /* primitive quartets not screened as a whole         */
     int64_t func_quartets;        /* canonical function quartets they hold              */
     double ms;                    /* kernel time of the last Fock build / export        */
+    int64_t tasks;                /* workgroups of its J/K launch (bra pair x ket run)   */
 } hfk_class_stat;
 
 int hfk_get_class_stats(hfk_ctx *ctx, hfk_class_stat *out, int32_t cap, int32_t *n);
