@@ -631,34 +631,7 @@ int hfk_ctx_plan(hfk_ctx* c)
         e.d_kets = c->d_class[e.kk];   // tasks index the ket class list itself (not owned)
         if (!tasks.empty()) HFK_TRY(dev_upload(c, &e.d_tasks, tasks));
     }
-    // exact counts (shell, primitive and canonical function quartets) per
-    // launch, over the task lists: only the surviving quartets are visited
-    {
-        unsigned long long* d_cnt = nullptr;
-        const size_t np = c->plan.size();
-        HFK_TRY(dev_alloc(&d_cnt, 3 * std::max<size_t>(np, 1)));
-        hfk_launch_fill((double*)d_cnt, 0.0, (long long)(3 * std::max<size_t>(np, 1)), c->stream);
-        QuartetArgs a = hfk_ctx_args(c);
-        a.sch_cut = c->opt.schwarz_cut;
-        for (size_t i = 0; i < np; ++i) {
-            PlanEntry& e = c->plan[i];
-            hfk_plan_args(c, e, a);
-            hfk_launch_count_tasks(a, ncart(e.la), ncart(e.lb), ncart(e.lc), ncart(e.ld), d_cnt + 3 * i, c->stream);
-        }
-        HFK_HIP(hipGetLastError());
-        std::vector<unsigned long long> h(3 * std::max<size_t>(np, 1));
-        HFK_HIP(hipMemcpyAsync(h.data(), d_cnt, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
-        HFK_HIP(hipStreamSynchronize(c->stream));
-        dev_free(c, d_cnt);
-        long long fq = 0;
-        for (size_t i = 0; i < np; ++i) {
-            c->plan[i].shell_q = (long long)h[3 * i];
-            c->plan[i].prim_q = (long long)h[3 * i + 1];
-            c->plan[i].func_q = (long long)h[3 * i + 2];
-            fq += c->plan[i].func_q;
-        }
-        c->stats.func_quartets = fq;
-    }
+    c->counted = false;   // exact counts on first request (hfk_ctx_count)
     // ket-side J'_cd sums: one region per launch, [copies][nket][NKET]
     {
         size_t per = 0;
@@ -673,6 +646,23 @@ int hfk_ctx_plan(hfk_ctx* c)
         c->d_Jk = nullptr;
         c->jk_len = off;
         HFK_TRY(dev_alloc(&c->d_Jk, std::max<size_t>(off, 1)));
+        std::vector<JkScatter> ent;
+        c->jk_max_count = 0;
+        for (const PlanEntry& e : c->plan) {
+            if (c->class_count[e.kk] == 0) continue;
+            JkScatter j;
+            j.ket_list = c->d_class[e.kk];
+            j.nket = c->class_count[e.kk];
+            j.off = (long long)e.jk_off;
+            j.nc = ncart(e.lc);
+            j.nd = ncart(e.ld);
+            ent.push_back(j);
+            c->jk_max_count = std::max(c->jk_max_count, j.nket * j.nc * j.nd);
+        }
+        if (c->d_jk_ent) dev_free(c, c->d_jk_ent);
+        c->d_jk_ent = nullptr;
+        c->jk_nent = (int)ent.size();
+        if (!ent.empty()) HFK_TRY(dev_upload(c, &c->d_jk_ent, ent));
     }
     const size_t np = c->plan.size();
     // event pool for per-launch timing
@@ -758,10 +748,45 @@ int hfk_ctx_collect_times(hfk_ctx* c)
     return HFK_OK;
 }
 
+// Exact counts (shell, primitive and canonical function quartets) per
+// launch, over the task lists: only the surviving quartets are visited.  For
+// statistics only, so computed on the first request rather than at planning.
+static int hfk_ctx_count(hfk_ctx* c)
+{
+    if (c->counted) return HFK_OK;
+    unsigned long long* d_cnt = nullptr;
+    const size_t np = c->plan.size();
+    HFK_TRY(dev_alloc(&d_cnt, 3 * std::max<size_t>(np, 1)));
+    hfk_launch_fill((double*)d_cnt, 0.0, (long long)(3 * std::max<size_t>(np, 1)), c->stream);
+    QuartetArgs a = hfk_ctx_args(c);
+    a.sch_cut = c->opt.schwarz_cut;
+    for (size_t i = 0; i < np; ++i) {
+        PlanEntry& e = c->plan[i];
+        hfk_plan_args(c, e, a);
+        hfk_launch_count_tasks(a, ncart(e.la), ncart(e.lb), ncart(e.lc), ncart(e.ld), d_cnt + 3 * i, c->stream);
+    }
+    HFK_HIP(hipGetLastError());
+    std::vector<unsigned long long> h(3 * std::max<size_t>(np, 1));
+    HFK_HIP(hipMemcpyAsync(h.data(), d_cnt, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HFK_HIP(hipStreamSynchronize(c->stream));
+    dev_free(c, d_cnt);
+    long long fq = 0;
+    for (size_t i = 0; i < np; ++i) {
+        c->plan[i].shell_q = (long long)h[3 * i];
+        c->plan[i].prim_q = (long long)h[3 * i + 1];
+        c->plan[i].func_q = (long long)h[3 * i + 2];
+        fq += c->plan[i].func_q;
+    }
+    c->stats.func_quartets = fq;
+    c->counted = true;
+    return HFK_OK;
+}
+
 extern "C" int hfk_get_class_stats(hfk_ctx* c, hfk_class_stat* out, int32_t cap, int32_t* n)
 {
     if (!c || !n) return hfk_set_error(HFK_EINVAL, "hfk_get_class_stats: null argument");
     std::lock_guard<std::mutex> g(c->mu);
+    HFK_TRY(hfk_ctx_count(c));
     *n = (int32_t)c->plan.size();
     for (size_t i = 0; i < c->plan.size() && (int32_t)i < cap && out; ++i) {
         const PlanEntry& e = c->plan[i];
@@ -797,9 +822,8 @@ int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_fla
     a.J = c->d_J;
     a.K = c->d_K;
     HFK_TRY(hfk_ctx_run_quartets(c, a, HFK_OP_JK));
-    for (const PlanEntry& e : c->plan)
-        hfk_launch_scatter_jk(c->d_J, c->d_Jk + e.jk_off, c->jk_copies, c->class_count[e.kk], ncart(e.lc), ncart(e.ld),
-                              c->d_class[e.kk], c->d_pair_A, c->d_pair_B, c->d_sh_bfp, c->n + 1, c->stream);
+    hfk_launch_scatter_jk(c->d_J, c->d_Jk, c->jk_copies, c->d_jk_ent, c->jk_nent, c->jk_max_count, c->d_pair_A,
+                          c->d_pair_B, c->d_sh_bfp, c->n + 1, c->stream);
     HFK_HIP(hipEventRecord(c->ev1, c->stream));
     hfk_launch_symmetrize_g(c->n, c->d_J, c->d_K, H_dev, F_dev_flat, nullptr, c->stream);
     HFK_HIP(hipGetLastError());
@@ -883,6 +907,7 @@ extern "C" int hfk_get_stats(hfk_ctx* c, hfk_stats* st)
 {
     if (!c || !st) return hfk_set_error(HFK_EINVAL, "hfk_get_stats: null argument");
     std::lock_guard<std::mutex> g(c->mu);
+    HFK_TRY(hfk_ctx_count(c));
     *st = c->stats;
     return HFK_OK;
 }

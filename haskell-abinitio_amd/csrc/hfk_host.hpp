@@ -88,6 +88,9 @@ struct hfk_ctx {
     double* d_Jk = nullptr;     // ket-side J'_cd sums of all J/K launches, by ket position
     size_t jk_len = 0;          // doubles in d_Jk
     int jk_copies = 1;
+    JkScatter* d_jk_ent = nullptr;   // per J/K launch: where its sums sit in d_Jk
+    int jk_nent = 0;
+    long long jk_max_count = 0;
     double *d_flatA = nullptr, *d_flatB = nullptr, *d_flatC = nullptr;
     int *d_sh_atom = nullptr, *d_atom_of_bf = nullptr;   // gradient (uploaded on first use)
     int* d_vpair = nullptr;                 // function pairs grouped by l_i + l_j (one-electron V)
@@ -96,6 +99,7 @@ struct hfk_ctx {
     std::vector<PlanEntry> plan;
     std::vector<hipEvent_t> events;
     int timed_op = 0;
+    bool counted = false;       // plan[].shell_q/prim_q/func_q and stats.func_quartets are current
     int kets_per_lane = 1;      // kets per lane in a tiled J/K task
 };
 

@@ -115,9 +115,17 @@ void hfk_launch_find_mixed(const QuartetArgs& a, unsigned long long* count, int2
 // J, K padded (leading dimension n+1)
 void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const double* H, double* G_flat,
                              double* F_full, hipStream_t s);
-// J'[c][d] += sum over copies of Jk[copy][pos][z] for the ket pairs ket_list[pos]
-// (components z = (c, d) of an nc x nd class); ld = n + 1
-void hfk_launch_scatter_jk(double* J, const double* Jk, int copies, long long nket, int nc, int nd, const int* ket_list,
+// One J/K launch's ket-side sums in Jk: [copies][nket][nc * nd] from offset off,
+// for the ket pairs ket_list[0..nket)
+struct JkScatter {
+    const int* ket_list;
+    long long nket;
+    long long off;
+    int nc, nd;
+};
+// J'[c][d] += sum over copies of Jk[off + (copy * nket + pos) * nc * nd + z] for
+// every launch e of ent[0..nent) (one kernel; ld = n + 1)
+void hfk_launch_scatter_jk(double* J, const double* Jk, int copies, const JkScatter* ent, int nent, long long max_count,
                            const int* pair_A, const int* pair_B, const int* sh_bfp, int ld, hipStream_t s);
 // flat -> padded (n+1)x(n+1) full, row/column n zero
 void hfk_launch_flat_to_padded(int n, const double* flat, double* full, hipStream_t s);

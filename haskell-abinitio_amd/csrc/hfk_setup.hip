@@ -472,31 +472,33 @@ void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const doub
 // Ket-side J'_cd sums, kept by position in the ket class list during the
 // J/K launch, added into J' (atomic: absent components of different pairs
 // share the zero row/column n).
-__global__ void k_scatter_jk(double* J, const double* Jk, int copies, long long nket, int nc, int nd,
-                             const int* ket_list, const int* pair_A, const int* pair_B, const int* sh_bfp, int ld)
+// grid.y runs over the launches
+__global__ void k_scatter_jk(double* J, const double* Jk, int copies, const JkScatter* ent, const int* pair_A,
+                             const int* pair_B, const int* sh_bfp, int ld)
 {
-    const int nz = nc * nd;
-    const long long count = nket * nz;
+    const JkScatter e = ent[blockIdx.y];
+    const int nz = e.nc * e.nd;
+    const long long count = e.nket * nz;
+    const double* src = Jk + e.off;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
          i += (long long)gridDim.x * blockDim.x) {
         double v = 0.0;
-        for (int c = 0; c < copies; ++c) v += Jk[(size_t)c * count + i];
+        for (int c = 0; c < copies; ++c) v += src[(size_t)c * count + i];
         if (v == 0.0) continue;
         const long long pos = i / nz;
         const int z = (int)(i - pos * nz);
-        const int kp = ket_list[pos];
-        const int ic = sh_bfp[pair_A[kp] * HFK_MAXCOMP + z / nd], id = sh_bfp[pair_B[kp] * HFK_MAXCOMP + z % nd];
+        const int kp = e.ket_list[pos];
+        const int ic = sh_bfp[pair_A[kp] * HFK_MAXCOMP + z / e.nd], id = sh_bfp[pair_B[kp] * HFK_MAXCOMP + z % e.nd];
         atomic_add_f64(&J[(size_t)ic * ld + id], v);
     }
 }
 
-void hfk_launch_scatter_jk(double* J, const double* Jk, int copies, long long nket, int nc, int nd, const int* ket_list,
+void hfk_launch_scatter_jk(double* J, const double* Jk, int copies, const JkScatter* ent, int nent, long long max_count,
                            const int* pair_A, const int* pair_B, const int* sh_bfp, int ld, hipStream_t s)
 {
-    const long long count = nket * nc * nd;
-    if (count <= 0) return;
-    hipLaunchKernelGGL(k_scatter_jk, dim3(grid_for(count, 256)), dim3(256), 0, s, J, Jk, copies, nket, nc, nd,
-                       ket_list, pair_A, pair_B, sh_bfp, ld);
+    if (nent <= 0 || max_count <= 0) return;
+    hipLaunchKernelGGL(k_scatter_jk, dim3(grid_for(max_count, 256), (unsigned)nent), dim3(256), 0, s, J, Jk, copies, ent,
+                       pair_A, pair_B, sh_bfp, ld);
 }
 
 void hfk_launch_flat_to_full(int n, const double* flat, double* full, hipStream_t s)
