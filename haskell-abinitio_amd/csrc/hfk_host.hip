@@ -234,6 +234,12 @@ extern "C" int hfk_set_options(hfk_ctx* c, const hfk_options* opt)
 int hfk_ctx_init_device(hfk_ctx* c)
 {
     HFK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int k = 0; k < hfk_ctx::NAUX; ++k) {
+        HFK_HIP(hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking));
+        HFK_HIP(hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming));
+    }
+    HFK_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    if (const char* v = getenv("HFK_CONCURRENT")) c->concurrent = atoi(v) != 0;
     // Boys tables
     std::vector<double> glx(128), glw(128);
     gl256(glx.data(), glw.data());
@@ -338,6 +344,14 @@ void hfk_ctx_free_device(hfk_ctx* c)
     c->events.clear();
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    for (int k = 0; k < hfk_ctx::NAUX; ++k) {
+        if (c->aux[k]) hipStreamDestroy(c->aux[k]);
+        if (c->ev_join[k]) hipEventDestroy(c->ev_join[k]);
+        c->aux[k] = nullptr;
+        c->ev_join[k] = nullptr;
+    }
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    c->ev_fork = nullptr;
     if (c->blas) hfk_linalg_destroy(c);
     if (c->stream) hipStreamDestroy(c->stream);
     c->stream = nullptr;
@@ -632,6 +646,8 @@ int hfk_ctx_plan(hfk_ctx* c)
         if (!tasks.empty()) HFK_TRY(dev_upload(c, &e.d_tasks, tasks));
     }
     c->counted = false;   // exact counts on first request (hfk_ctx_count)
+    c->first_launch = -1;
+    c->serial_runs = 0;
     // ket-side J'_cd sums: one region per launch, [copies][nket][NKET]
     {
         size_t per = 0;
@@ -714,8 +730,28 @@ int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
         return hfk_set_error(HFK_EINVAL, "J/K launch without its accumulators");
     long long total = 0;
     const size_t np = c->plan.size();
-    for (size_t i = 0; i < np; ++i) {
+    // J/K launches only accumulate (atomically) into J', K' and their own
+    // ket-side sums, so they may run concurrently.  The longest launch (as
+    // measured in the context's second J/K run; the first two are serial and
+    // the first pays one-time costs) runs first,
+    // alone on the context stream: it fills the GPU by itself and keeps a
+    // clean time.  The others then run round-robin on the auxiliary streams,
+    // forked from and joined back into the context stream, and fill each
+    // other's tails.
+    const bool conc = op == HFK_OP_JK && c->concurrent && np > 2 && c->first_launch >= 0;
+    const size_t first = conc ? (size_t)c->first_launch : 0;
+    std::vector<size_t> order;
+    if (conc) order.push_back(first);
+    for (size_t i = 0; i < np; ++i)
+        if (!conc || i != first) order.push_back(i);
+    int slot = 0, missing = -1;
+    for (size_t oi = 0; oi < order.size() && missing < 0; ++oi) {
+        const size_t i = order[oi];
         const PlanEntry& e = c->plan[i];
+        if (conc && oi == 1) {
+            HFK_HIP(hipEventRecord(c->ev_fork, c->stream));
+            for (int k = 0; k < hfk_ctx::NAUX; ++k) HFK_HIP(hipStreamWaitEvent(c->aux[k], c->ev_fork, 0));
+        }
         if (op != HFK_OP_JK && e.prec == 1) continue;   // exports are FP64
         hfk_plan_args(c, e, a);
         if (op != HFK_OP_JK) a.prec_sel = 0;
@@ -724,14 +760,26 @@ int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
             a.jk_copies = c->jk_copies;
         }
         total += e.nquart;
-        HFK_HIP(hipEventRecord(c->events[2 * i], c->stream));
-        int rc = hfk_launch_quartets(e.la, e.lb, e.lc, e.ld, op, e.prec, c->opt.boys_mode, op == HFK_OP_JK ? e.mode : 0, a, c->stream);
-        if (rc) return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", e.la, e.lb, e.lc, e.ld);
-        HFK_HIP(hipEventRecord(c->events[2 * i + 1], c->stream));
+        hipStream_t s = (conc && oi > 0) ? c->aux[slot++ % hfk_ctx::NAUX] : c->stream;
+        HFK_HIP(hipEventRecord(c->events[2 * i], s));
+        if (hfk_launch_quartets(e.la, e.lb, e.lc, e.ld, op, e.prec, c->opt.boys_mode, op == HFK_OP_JK ? e.mode : 0, a, s))
+            missing = (int)i;
+        c->plan[i].solo = (!conc || oi == 0) ? 1 : 0;
+        HFK_HIP(hipEventRecord(c->events[2 * i + 1], s));
+    }
+    if (conc)
+        for (int k = 0; k < hfk_ctx::NAUX; ++k) {
+            HFK_HIP(hipEventRecord(c->ev_join[k], c->aux[k]));
+            HFK_HIP(hipStreamWaitEvent(c->stream, c->ev_join[k], 0));
+        }
+    if (missing >= 0) {
+        const PlanEntry& e = c->plan[missing];
+        return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", e.la, e.lb, e.lc, e.ld);
     }
     HFK_HIP(hipGetLastError());
     c->stats.quartets_launched = total;
     c->timed_op = op;
+    c->timed_serial = !conc;
     return HFK_OK;
 }
 
@@ -744,6 +792,14 @@ int hfk_ctx_collect_times(hfk_ctx* c)
         float ms = 0.f;
         HFK_HIP(hipEventElapsedTime(&ms, c->events[2 * i], c->events[2 * i + 1]));
         e.ms = ms;
+    }
+    // the second serial J/K run ranks the launches for the concurrent ones that follow
+    if (c->timed_op == HFK_OP_JK && c->timed_serial && c->first_launch < 0 && !c->plan.empty() &&
+        ++c->serial_runs >= 2) {
+        size_t first = 0;
+        for (size_t i = 1; i < c->plan.size(); ++i)
+            if (c->plan[i].ms > c->plan[first].ms) first = i;
+        c->first_launch = (int)first;
     }
     return HFK_OK;
 }
@@ -796,7 +852,7 @@ extern "C" int hfk_get_class_stats(hfk_ctx* c, hfk_class_stat* out, int32_t cap,
         s.lc = e.lc;
         s.ld = e.ld;
         s.prec = e.prec;
-        s.pad = 0;
+        s.solo = e.solo;
         s.shell_quartets = e.shell_q;
         s.prim_quartets = e.prim_q;
         s.func_quartets = e.func_q;

@@ -50,6 +50,7 @@ struct PlanEntry {
     int bra_stage;              // longest bra primitive-pair range of the tasks
     int task_len_max;           // longest ket run of the tasks
     size_t jk_off;              // its ket-side J'_cd sums in hfk_ctx::d_Jk
+    int solo;                   // its last timed run had the GPU to itself
 };
 
 struct hfk_ctx {
@@ -71,6 +72,15 @@ struct hfk_ctx {
     int* d_class[HFK_NPC] = {nullptr};
     // device data
     hipStream_t stream = nullptr;
+    // J/K launches of one Fock build run concurrently on these (forked from
+    // and joined into `stream`); default priority
+    static constexpr int NAUX = 3;
+    hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[NAUX] = {nullptr, nullptr, nullptr};
+    int concurrent = 1;
+    int first_launch = -1;      // plan entry that runs alone first (-1: not yet measured)
+    int serial_runs = 0;        // serial J/K runs since planning (the first is cold)
+    bool timed_serial = false;  // the last timed run launched serially
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void* blas = nullptr;                 // rocblas_handle
     std::vector<void*> allocs;

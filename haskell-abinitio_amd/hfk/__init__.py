@@ -223,7 +223,7 @@ class Context:
         _lib.lib().hfk_get_class_stats(self._h, None, 0, C.byref(n))
         arr = (_lib.HfkClassStat * max(1, n.value))()
         check(_lib.lib().hfk_get_class_stats(self._h, arr, n.value, C.byref(n)))
-        return [{k: getattr(arr[i], k) for k, _ in arr[i]._fields_ if k != "pad"} for i in range(n.value)]
+        return [{k: getattr(arr[i], k) for k, _ in arr[i]._fields_} for i in range(n.value)]
 
     def stats(self):
         s = _lib.HfkStats()

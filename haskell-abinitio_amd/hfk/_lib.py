@@ -66,7 +66,7 @@ class HfkStats(C.Structure):
 class HfkClassStat(C.Structure):
     _fields_ = [
         ("la", C.c_int32), ("lb", C.c_int32), ("lc", C.c_int32), ("ld", C.c_int32),
-        ("prec", C.c_int32), ("pad", C.c_int32), ("shell_quartets", C.c_int64),
+        ("prec", C.c_int32), ("solo", C.c_int32), ("shell_quartets", C.c_int64),
         ("prim_quartets", C.c_int64), ("func_quartets", C.c_int64), ("ms", C.c_double),
         ("tasks", C.c_int64),
     ]

@@ -182,7 +182,7 @@ int hfk_get_stats(hfk_ctx *ctx, hfk_stats *st);
 typedef struct hfk_class_stat {
     int32_t la, lb, lc, ld;
     int32_t prec;                 /* 0 FP64, 1 FP32                                     */
-    int32_t pad;
+    int32_t solo;                 /* 1: its last run had the GPU to itself (ms is its own) */
     int64_t shell_quartets;       /* shell quartets passing the pair screens            */
     int64_t prim_quartets;        /* primitive quartets not screened as a whole         */
     int64_t func_quartets;        /* canonical function quartets they hold              */
