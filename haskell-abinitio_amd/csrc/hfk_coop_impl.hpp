@@ -698,12 +698,15 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
         // a long task (planned for the per-lane kernel) is cut into slices of
         // 4 S kets along grid y; every slice digests independently
         const int kslice = 4 * SH::S;
-        const int kbeg = task.y + (int)blockIdx.y * kslice;
-        if (kbeg >= task.z) return;   // uniform over the workgroup, before any barrier
-        const int kend = min(task.z, kbeg + kslice);
+        const int kbeg = (int)blockIdx.y * kslice;   // kets of the task, in run order
+        const int ktot = task_ket_count(a, task);
+        const int k0 = (task.w >> 8) & 0xff, k1 = task.w & 0xff;
+        // uniform over the workgroup, before any barrier; this kernel needs an
+        // explicit ket primitive range (the planner never gives it k1 = all)
+        if (kbeg >= ktot || k1 == 0xff) return;
+        const int kend = min(ktot, kbeg + kslice);
         const int bp = task.x;
         const int b0 = (task.w >> 24) & 0xff, b1 = (task.w >> 16) & 0xff;
-        const int k0 = (task.w >> 8) & 0xff, k1 = task.w & 0xff;
         const int A = a.pair_A[bp], B = a.pair_B[bp];
         const int ia = a.sh_bfp[A * HFK_MAXCOMP + ca], ib = a.sh_bfp[B * HFK_MAXCOMP + cb];
         const double sab = active ? a.sh_scale[A * HFK_MAXCOMP + ca] * a.sh_scale[B * HFK_MAXCOMP + cb] : 0.0;
@@ -723,9 +726,10 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
         const int len = kend - kbeg;
         for (int g0 = 0; g0 < len; g0 += SH::S) {
             const int it = g0 + slot;
-            int kp = -1;
+            int kp = -1, kpos = -1;
             if (active && it < len) {
-                kp = a.ket_idx[kbeg + it];
+                kpos = task_ket_pos(a, task, kbeg + it);
+                kp = a.ket_idx[kpos];
                 if (!quartet_selected(a, bp, kp)) kp = -1;
             }
             int C = 0, Dd = 0;
@@ -795,7 +799,7 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
                     for (int z = li; z < NKET; z += NBRA) {
                         const double v = sm.jcd[slot][z];
                         if (v != 0.0)
-                            atomic_add_f64(&a.Jk[((size_t)(blockIdx.x % (unsigned)a.jk_copies) * a.nket + kbeg + it) * NKET + z], v);
+                            atomic_add_f64(&a.Jk[((size_t)(blockIdx.x % (unsigned)a.jk_copies) * a.nket + kpos) * NKET + z], v);
                         sm.jcd[slot][z] = 0.0;
                     }
                 }

@@ -115,9 +115,9 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------
-// Tiled J/K kernel: one workgroup per task = (bra shell pair, run of an
-// explicit ket pair array with equal primitive-pair counts, primitive-pair
-// ranges of bra and ket).  Each lane loops over its ket primitive pairs
+// Tiled J/K kernel: one workgroup per task = (bra shell pair, runs of the ket
+// class list (each of one primitive-pair count), primitive-pair ranges of
+// bra and ket).  Each lane loops over its ket primitive pairs
 // (vector loads, once each) and, inside, over the bra primitive pairs
 // (wave-uniform, scalar loads).  Splitting heavy quartets into primitive
 // ranges evens out the work per lane; the digestion is linear in the
@@ -186,16 +186,17 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
     // lanes' atomics fall on consecutive addresses; tasks resident at the same
     // time share ket runs, so workgroups spread them over a few copies
     double* __restrict__ Jk = a.Jk + (size_t)(blockIdx.x % (unsigned)a.jk_copies) * a.nket * NKET;
-    const int len = task.z - task.y;
+    const int len = task_ket_count(a, task);
     const int rot = (int)((unsigned)task.x * 37u % (unsigned)(len > 0 ? len : 1));
     for (int it = tid; it < len; it += nthr) {
         int jj = it + rot;
         if (jj >= len) jj -= len;
-        const int kp = a.ket_idx[task.y + jj];
+        const int pos = task_ket_pos(a, task, jj);
+        const int kp = a.ket_idx[pos];
         if (!quartet_selected(a, bp, kp)) continue;
         T Vt[NBRA * NKET];
         eri_class<LA, LB, LC, LD, T, BOYS, MODE>(sbra, sbq, nbs, a.pp + (size_t)(a.pair_pp0[kp] + kprim0) * PP_STRIDE,
-                                                 kprim1 - kprim0, a.qcomp, a.bt, Vt, a.qtau);
+                                                 task_kprim_end(a, kprim1, kp) - kprim0, a.qcomp, a.bt, Vt, a.qtau);
         const int C = a.pair_A[kp], Dd = a.pair_B[kp];
         int ic[NC], id[ND];
         double sc[NC], sd[ND];
@@ -236,7 +237,7 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
             double s = 0.0;
 #pragma unroll
             for (int b = 0; b < NBRA; ++b) s += V[b * NKET + k] * Dab[b];
-            atomic_add_f64(&Jk[(size_t)(task.y + jj) * NKET + k], 2.0 * s);
+            atomic_add_f64(&Jk[(size_t)pos * NKET + k], 2.0 * s);
         }
         // density blocks D_bd, D_bc, D_ad, D_ac
         double Dbd[NB * ND], Dbc[NB * NC], Dad[NA * ND], Dac[NA * NC];

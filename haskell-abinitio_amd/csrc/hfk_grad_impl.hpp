@@ -50,9 +50,9 @@ __global__ void __launch_bounds__(256) k_grad_tiles(QuartetArgs a)
     const int nbp = bprim1 - bprim0;   // ranges are explicit: 0 <= b0 < b1 <= npp
     double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
 
-    const int len = task.z - task.y;
+    const int len = task_ket_count(a, task);
     for (int it = tid; it < len; it += nthr) {
-        const int kp = a.ket_idx[task.y + it];
+        const int kp = a.ket_idx[task_ket_pos(a, task, it)];
         if (!quartet_selected(a, bp, kp)) continue;
         const int C = a.pair_A[kp], Dd = a.pair_B[kp];
         int ic[NC], id[ND];
@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_grad_tiles(QuartetArgs a)
                         G[((x * NB + y) * NC + z) * ND + w] = (f * sa[x] * sb[y] * sc[z] * sd[w]) * gam;
                     }
         const double* ket = a.pp + (size_t)(a.pair_pp0[kp] + kprim0) * PP_STRIDE;
-        const int nkp = kprim1 - kprim0;
+        const int nkp = task_kprim_end(a, kprim1, kp) - kprim0;
         double g[9];
         grad_class<LA, LB, LC, LD, double, BOYS, MODE>(bra, nbp, ket, nkp, a.qcomp, a.bt, G, a.qtau, g);
 #pragma unroll

@@ -513,8 +513,10 @@ void hfk_pair_class_l(int k, int* la, int* lb)
 int hfk_ctx_plan(hfk_ctx* c)
 {
     if (const char* v = getenv("HFK_KETS_PER_LANE")) c->kets_per_lane = std::max(1, atoi(v));
-    for (PlanEntry& e : c->plan)
+    for (PlanEntry& e : c->plan) {
         if (e.d_tasks) dev_free(c, e.d_tasks);
+        if (e.d_runs) dev_free(c, e.d_runs);
+    }
     c->plan.clear();
     const bool ref = c->opt.screen_mode == HFK_SCREEN_REFERENCE && c->opt.screen_tau > 0.0;
     for (int kb = 0; kb < HFK_NPC; ++kb) {
@@ -544,26 +546,25 @@ int hfk_ctx_plan(hfk_ctx* c)
             }
         }
     }
-    // Task lists of the tiled J/K kernel.  Class lists are ordered by
-    // primitive-pair count, then by descending screen value.  For each bra
-    // pair, the kets surviving the pair screen form a prefix of each
-    // primitive-pair-count run; those prefixes are concatenated into an
-    // explicit ket array and cut into tasks of `chunk` kets, so the lanes of
-    // a workgroup are busy and mostly do equal work.
+    // Task lists of the J/K (and gradient) kernels.  Class lists are ordered
+    // by primitive-pair count, then by descending screen value, so for each
+    // bra pair the kets surviving the pair screen are a prefix of each
+    // segment of equal primitive-pair count.  A task is a bra pair, a list of
+    // runs (such prefixes, as position ranges in the ket class list) and
+    // primitive ranges of bra and ket.  Lanes walk the runs in order, so a
+    // wave mostly meets one primitive-pair count.  Every task flushes the K
+    // rows it staged once, so when the launch has enough bra pairs to fill
+    // the GPU a bra pair keeps up to 1024 kets in one task.
     for (PlanEntry& e : c->plan) {
         const bool coop = hfk_class_coop(e.la, e.lb, e.lc, e.ld);
-        // per-lane kernel: one ket per lane of the workgroup; cooperative kernel:
-        // 64 / (na nb) kets at a time, four rounds per task
-#ifdef HFK_EXP_B64
-        const int chunk = coop ? 4 * (64 / (ncart(e.la) * ncart(e.lb))) : 64 * c->kets_per_lane;
-#else
-        const int chunk = coop ? 4 * (64 / (ncart(e.la) * ncart(e.lb))) : (c->n >= 200 ? 256 : 64) * c->kets_per_lane;
-#endif
+        const int block = hfk_plan_block(c, e);
+        // cooperative kernel: 64 / (na nb) kets at a time, four rounds per task
+        const int chunk = coop ? 4 * (64 / (ncart(e.la) * ncart(e.lb))) : block * c->kets_per_lane;
         const std::vector<int>& bl = c->h_class[e.kb];
         const std::vector<int>& kl = c->h_class[e.kk];
         std::vector<double> kq(kl.size());
         for (size_t j = 0; j < kl.size(); ++j) kq[j] = c->h_pair_qhi[kl[j]];
-        std::vector<std::pair<long long, long long>> segs;   // runs of equal primitive-pair count
+        std::vector<std::pair<long long, long long>> segs;   // segments of equal primitive-pair count
         for (size_t j = 0; j < kl.size();) {
             size_t k = j + 1;
             while (k < kl.size() && c->h_pair_npp[kl[k]] == c->h_pair_npp[kl[j]]) ++k;
@@ -592,34 +593,64 @@ int hfk_ctx_plan(hfk_ctx* c)
         for (size_t i = 0; i < bl.size(); ++i)
             for (const auto& sg : segs)
                 est_prim += (long long)c->h_pair_npp[bl[i]] * c->h_pair_npp[kl[sg.first]] * (staircase(i, sg) - sg.first);
+        // Primitive ranges: a lane's share of a task is about per_lane primitive
+        // quartets, sized from the launch's total so that small launches still
+        // fill the GPU (~2.6e5 lanes in flight) and large ones keep quartets whole
+        // (fewer partial digestions).  Bra primitives are wave-uniform (LDS
+        // broadcast), so a lane keeps as many of them as fit and the ket range is
+        // split.  Segments that need no split join the long multi-run tasks.
+        const int L = e.la + e.lb + e.lc + e.ld;
+        const long long floor_l = std::max(4, 32 >> L);   // cheap classes: amortise the digestion
+        const int per_lane = (int)std::max(floor_l, std::min(128LL, est_prim / 262144));
+        const bool single = coop || hfk_class_grad_coop(e.la, e.lb, e.lc, e.ld);   // cooperative kernels read these
+        auto whole = [&](int nbp, int nkp) { return !single && nbp <= HFK_BRA_STAGE && nbp * nkp <= per_lane; };
+        long long merged_kets = 0;
+        for (size_t i = 0; i < bl.size(); ++i)
+            for (const auto& sg : segs)
+                if (whole(c->h_pair_npp[bl[i]], c->h_pair_npp[kl[sg.first]])) merged_kets += staircase(i, sg) - sg.first;
+        // long tasks: enough of them for ~40960 waves in flight, and at most
+        // 1024 kets (longer ones leave a few workgroups running alone at the
+        // end of the launch: measured slower from 4096 kets up)
+        const long long target = std::max(1, 40960 * 64 / block);
+        const long long cmerge = std::max<long long>(chunk, std::min<long long>(1024, (merged_kets + target - 1) / target));
         std::vector<int4> tasks;
+        std::vector<int2> runs;
+        auto push = [&](int bp, long long r0, int w, int len) {
+            if (r0 > INT32_MAX || (long long)runs.size() > INT32_MAX) return false;
+            tasks.push_back(make_int4(bp, (int)r0, (int)runs.size(), w));
+            e.task_len_max = std::max(e.task_len_max, len);
+            return true;
+        };
         for (size_t i = 0; i < bl.size(); ++i) {
-            const double qi = c->h_pair_qhi[bl[i]];
-            // kets need qhi_j >= tau / qhi_i; relaxed so that rounding never drops a survivor
-            const double need = (!ref || std::isinf(qi)) ? -1.0
-                              : (qi > 0.0 ? c->opt.screen_tau / qi * (1.0 - 1e-12) : INFINITY);
             const int nbp = c->h_pair_npp[bl[i]];
+            // kets of the unsplit segments, as one list of runs cut every cmerge kets
+            long long r0 = (long long)runs.size();
+            int len = 0;
             for (const auto& sg : segs) {
-                long long hi_end = sg.second;
-                if (e.kb == e.kk) hi_end = std::min<long long>(hi_end, (long long)i + 1);
-                if (hi_end <= sg.first) continue;
-                long long lo = sg.first, hi = hi_end;
-                while (lo < hi) {
-                    const long long mid = (lo + hi) / 2;
-                    if (kq[mid] >= need) lo = mid + 1; else hi = mid;
-                }
-                if (lo <= sg.first) continue;
-                // the surviving kets are the contiguous run [sg.first, lo) of the class list
-                const long long k0 = sg.first, k1 = lo;
-                // Primitive ranges: a lane's share of a task is about per_lane primitive
-                // quartets, sized from the launch's total so that small launches still
-                // fill the GPU (~2.6e5 lanes in flight) and large ones keep quartets whole
-                // (fewer partial digestions).  Bra primitives are wave-uniform (scalar
-                // loads), so a lane keeps as many of them as fit and the ket range is split.
                 const int nkp = c->h_pair_npp[kl[sg.first]];
-                const int L = e.la + e.lb + e.lc + e.ld;
-                const long long floor_l = std::max(4, 32 >> L);   // cheap classes: amortise the digestion
-                const int per_lane = (int)std::max(floor_l, std::min(128LL, est_prim / 262144));
+                if (!whole(nbp, nkp)) continue;
+                long long x = sg.first;
+                const long long x1 = staircase(i, sg);
+                while (x < x1) {
+                    const long long take = std::min<long long>(x1 - x, cmerge - len);
+                    runs.push_back(make_int2((int)x, (int)(x + take)));
+                    x += take;
+                    len += (int)take;
+                    if (len == cmerge) {
+                        if (!push(bl[i], r0, (nbp << 16) | 0xff, len)) return hfk_set_error(HFK_EUNSUP, "task list too long");
+                        r0 = (long long)runs.size();
+                        len = 0;
+                    }
+                }
+            }
+            if (len > 0 && !push(bl[i], r0, (nbp << 16) | 0xff, len)) return hfk_set_error(HFK_EUNSUP, "task list too long");
+            if (nbp > 0) e.bra_stage = std::max(e.bra_stage, coop ? nbp : std::min(nbp, HFK_BRA_STAGE));
+            // segments split into primitive ranges: one run per task
+            for (const auto& sg : segs) {
+                const int nkp = c->h_pair_npp[kl[sg.first]];
+                if (whole(nbp, nkp)) continue;
+                const long long k0 = sg.first, k1 = staircase(i, sg);
+                if (k1 <= k0) continue;
                 int bstep = nbp, kstep = nkp;
                 if (!coop && nbp * nkp > per_lane) {
                     bstep = std::max(1, std::min(nbp, per_lane));
@@ -631,10 +662,13 @@ int hfk_ctx_plan(hfk_ctx* c)
                     for (int q0 = 0; q0 < nkp; q0 += kstep) {
                         const int q1 = std::min(nkp, q0 + kstep);
                         const int w = (b0 << 24) | (b1 << 16) | (q0 << 8) | q1;
-                        for (long long j0 = k0; j0 < k1; j0 += chunk)
-                            tasks.push_back(make_int4(bl[i], (int)j0, (int)std::min<long long>(k1, j0 + chunk), w));
+                        for (long long j0 = k0; j0 < k1; j0 += chunk) {
+                            const long long j1 = std::min<long long>(k1, j0 + chunk);
+                            const long long rr = (long long)runs.size();
+                            runs.push_back(make_int2((int)j0, (int)j1));
+                            if (!push(bl[i], rr, w, (int)(j1 - j0))) return hfk_set_error(HFK_EUNSUP, "task list too long");
+                        }
                         e.bra_stage = std::max(e.bra_stage, b1 - b0);
-                        e.task_len_max = std::max<int>(e.task_len_max, (int)std::min<long long>(chunk, k1 - k0));
                     }
                 }
             }
@@ -642,8 +676,12 @@ int hfk_ctx_plan(hfk_ctx* c)
         if ((long long)kl.size() > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "pair class too long");
         e.ntask = (long long)tasks.size();
         e.d_tasks = nullptr;
-        e.d_kets = c->d_class[e.kk];   // tasks index the ket class list itself (not owned)
-        if (!tasks.empty()) HFK_TRY(dev_upload(c, &e.d_tasks, tasks));
+        e.d_runs = nullptr;
+        e.d_kets = c->d_class[e.kk];   // runs index the ket class list itself (not owned)
+        if (!tasks.empty()) {
+            HFK_TRY(dev_upload(c, &e.d_tasks, tasks));
+            HFK_TRY(dev_upload(c, &e.d_runs, runs));
+        }
     }
     c->counted = false;   // exact counts on first request (hfk_ctx_count)
     c->first_launch = -1;
@@ -693,24 +731,17 @@ int hfk_ctx_plan(hfk_ctx* c)
 void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
 {
     a.tasks = e.d_tasks;
+    a.runs = e.d_runs;
     a.ket_idx = e.d_kets;
     a.ntask = e.ntask;
     a.bra_stage = e.bra_stage;
     a.task_len_max = e.task_len_max;
-#ifdef HFK_EXP_B64
-    a.block = 64;
-#else
-    a.block = (c->n >= 200 && !hfk_class_coop(e.la, e.lb, e.lc, e.ld)) ? 256 : 64;
-#endif
+    a.block = hfk_plan_block(c, e);
     {
         const int nr = (e.la + 1) * (e.la + 2) / 2 + (e.lb + 1) * (e.lb + 2) / 2;
         // the cooperative kernel also holds ~12 KB of static LDS
-#ifdef HFK_EXP_KCAP
-        const size_t cap = hfk_class_coop(e.la, e.lb, e.lc, e.ld) ? 48 * 1024 : HFK_EXP_KCAP;
-#else
         const size_t cap = hfk_class_coop(e.la, e.lb, e.lc, e.ld) ? 48 * 1024
                                                                   : 64 * 1024 - hfk_bra_stage_bytes(e.la, e.lb, e.bra_stage);
-#endif
         a.lds_rows = ((size_t)nr * (c->n + 1) * sizeof(double) <= cap) ? 1 : 0;
     }
     a.bra_list = c->d_class[e.kb];

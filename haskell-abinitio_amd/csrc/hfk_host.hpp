@@ -44,7 +44,8 @@ struct PlanEntry {
     long long shell_q, prim_q, func_q;   // counted survivors
     double ms;                  // last timed duration
     int mode;                   // 0 main pass, 1 screen-correction pass (shares the main tasks)
-    int4* d_tasks;              // tiled J/K tasks {bra pair, begin, end, 0} into d_kets
+    int4* d_tasks;              // tasks {bra pair, first run, end run, primitive ranges}
+    int2* d_runs;               // runs [begin, end) of positions in d_kets
     int* d_kets;                // ket pair indices the tasks refer to (the ket class list)
     long long ntask;
     int bra_stage;              // longest bra primitive-pair range of the tasks
@@ -112,6 +113,14 @@ struct hfk_ctx {
     bool counted = false;       // plan[].shell_q/prim_q/func_q and stats.func_quartets are current
     int kets_per_lane = 1;      // kets per lane in a tiled J/K task
 };
+
+// Workgroup size of a J/K launch: 256 lanes (four waves sharing one set of
+// staged K rows) when the rows are long enough to limit occupancy through LDS
+// and the class runs per lane; otherwise 64.
+inline int hfk_plan_block(const hfk_ctx* c, const PlanEntry& e)
+{
+    return (c->n >= 200 && !hfk_class_coop(e.la, e.lb, e.lc, e.ld)) ? 256 : 64;
+}
 
 template <typename T>
 int dev_alloc(T** p, size_t n)

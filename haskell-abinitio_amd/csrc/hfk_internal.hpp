@@ -42,9 +42,11 @@ struct QuartetArgs {
     double* flat;               // flat ERI output (calcIntegrals order)
     const long long* flat_base; // [n][n] rank of (i,j,i,j)
     double* diag;               // per-pair max |(ab|ab)| output (Schwarz setup)
-    // task list of the tiled J/K kernel: {bra pair, begin, end, 0} into ket_idx
+    // task list of the tiled kernels: {bra pair, first run, end run, primitive
+    // ranges (b0 << 24 | b1 << 16 | k0 << 8 | k1; k1 = 255: all of the ket's)}
     const int4* tasks;
-    const int* ket_idx;         // explicit ket pair indices of the tasks
+    const int2* runs;           // [begin, end) positions in ket_idx, walked in order
+    const int* ket_idx;         // ket pair indices (the ket class list)
     long long ntask;
     int lds_rows;               // K rows staged in LDS (0: global atomics)
     int block;                  // workgroup size of the tiled kernel (64 or 256)
@@ -66,6 +68,16 @@ inline bool hfk_class_coop(int la, int lb, int lc, int ld)
 {
     const int nv = (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2) * ((lc + 1) * (lc + 2) / 2) * ((ld + 1) * (ld + 2) / 2);
     return (la >= 2 || lb >= 2 || lc >= 2 || ld >= 2) && nv > 54;
+}
+
+// Classes whose gradient runs on the cooperative kernel (d classes with more
+// than 18 integrals per quartet, and any class with more than 54); gen_eri.py
+// uses the same bounds.  That kernel needs tasks of one run with an explicit
+// ket primitive range (uniform over the workgroup).
+inline bool hfk_class_grad_coop(int la, int lb, int lc, int ld)
+{
+    const int nv = (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2) * ((lc + 1) * (lc + 2) / 2) * ((ld + 1) * (ld + 2) / 2);
+    return ((la >= 2 || lb >= 2 || lc >= 2 || ld >= 2) && nv > 18) || nv > 54;
 }
 
 // The per-lane J/K kernel stages the task's bra primitive-pair records and

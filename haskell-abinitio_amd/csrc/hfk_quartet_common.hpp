@@ -31,6 +31,32 @@ __device__ __forceinline__ void decode_quartet(const QuartetArgs& a, long long q
     }
 }
 
+// Kets of a task: its runs [x, y) of positions in the ket class list.
+__device__ __forceinline__ int task_ket_count(const QuartetArgs& a, const int4& task)
+{
+    int n = 0;
+    for (int r = task.y; r < task.z; ++r) n += a.runs[r].y - a.runs[r].x;
+    return n;
+}
+
+// Position of the task's v-th ket (0 <= v < task_ket_count).
+__device__ __forceinline__ int task_ket_pos(const QuartetArgs& a, const int4& task, int v)
+{
+    for (int r = task.y; r < task.z; ++r) {
+        const int2 q = a.runs[r];
+        const int len = q.y - q.x;
+        if (v < len) return q.x + v;
+        v -= len;
+    }
+    return -1;
+}
+
+// End of the task's ket primitive range for ket pair kp (k1 = 255: all).
+__device__ __forceinline__ int task_kprim_end(const QuartetArgs& a, int k1, int kp)
+{
+    return min(k1, a.pair_npp[kp]);
+}
+
 __device__ __forceinline__ bool quartet_selected(const QuartetArgs& a, int bp, int kp)
 {
     if (a.qtau > 0.0 && a.pair_qhi[bp] * a.pair_qhi[kp] < a.qtau) return false;

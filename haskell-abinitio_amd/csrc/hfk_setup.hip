@@ -351,13 +351,15 @@ __global__ void __launch_bounds__(64) k_count_tasks(QuartetArgs a, int na, int n
         const int k0 = (task.w >> 8) & 0xff, k1 = task.w & 0xff;
         const bool first = (b0 == 0 && k0 == 0);   // each quartet counted once
         const double* bra = a.pp + (size_t)a.pair_pp0[bp] * PP_STRIDE;
-        for (int j = task.y + (int)threadIdx.x; j < task.z; j += blockDim.x) {
-            const int kp = a.ket_idx[j];
+        const int len = task_ket_count(a, task);
+        for (int j = (int)threadIdx.x; j < len; j += blockDim.x) {
+            const int kp = a.ket_idx[task_ket_pos(a, task, j)];
             if (!quartet_selected(a, bp, kp)) continue;
             const double* ket = a.pp + (size_t)a.pair_pp0[kp] * PP_STRIDE;
+            const int ke = task_kprim_end(a, k1, kp);
             for (int x = b0; x < b1; ++x) {
                 const double bq = bra[(size_t)x * PP_STRIDE + PP_QHI];
-                for (int y = k0; y < k1; ++y)
+                for (int y = k0; y < ke; ++y)
                     if (!(a.qtau > 0.0 && bq * ket[(size_t)y * PP_STRIDE + PP_QHI] < a.qtau)) cp += 1;
             }
             if (first) {
@@ -382,8 +384,9 @@ __global__ void __launch_bounds__(64) k_find_mixed(QuartetArgs a, unsigned long 
     const int bp = task.x;
     const double* bra = a.pp + (size_t)a.pair_pp0[bp] * PP_STRIDE;
     const int nbp = a.pair_npp[bp];
-    for (int j = task.y + (int)threadIdx.x; j < task.z; j += 64) {
-        const int kp = a.ket_idx[j];
+    const int len = task_ket_count(a, task);
+    for (int j = (int)threadIdx.x; j < len; j += 64) {
+        const int kp = a.ket_idx[task_ket_pos(a, task, j)];
         if (a.qtau > 0.0 && a.pair_qhi[bp] * a.pair_qhi[kp] < a.qtau) continue;
         const double* ket = a.pp + (size_t)a.pair_pp0[kp] * PP_STRIDE;
         bool mixed = false;
