@@ -894,7 +894,7 @@ extern "C" int hfk_get_class_stats(hfk_ctx* c, hfk_class_stat* out, int32_t cap,
 }
 
 // Device Fock build: G (or H+G) flat from a flat density, all on device.
-int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat)
+int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat, double extra_cut)
 {
     const size_t np1 = (size_t)(c->n + 1) * (c->n + 1);
     hfk_launch_flat_to_padded(c->n, D_dev_flat, c->d_D, c->stream);
@@ -904,7 +904,7 @@ int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_fla
     HFK_HIP(hipEventRecord(c->ev0, c->stream));
     if (c->jk_len) hfk_launch_fill(c->d_Jk, 0.0, (long long)c->jk_len, c->stream);
     QuartetArgs a = hfk_ctx_args(c);
-    a.sch_cut = c->opt.schwarz_cut;
+    a.sch_cut = std::max(c->opt.schwarz_cut, extra_cut);
     a.D = c->d_D;
     a.J = c->d_J;
     a.K = c->d_K;

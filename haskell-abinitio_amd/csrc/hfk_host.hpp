@@ -178,7 +178,9 @@ int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op);
 int hfk_ctx_plan(hfk_ctx* c);
 void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a);
 int hfk_ctx_collect_times(hfk_ctx* c);
-int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat);
+// extra_cut > 0 also skips shell quartets with Schwarz product below it
+int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat,
+                        double extra_cut = 0.0);
 int hfk_ctx_overlap_dev(hfk_ctx* c, double* S_dev);
 int hfk_ctx_hcore_dev(hfk_ctx* c, double* H_dev);
 OneEArgs hfk_onee_args(hfk_ctx* c);

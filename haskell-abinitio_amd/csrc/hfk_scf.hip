@@ -194,6 +194,9 @@ extern "C" void hfk_default_scf_options(hfk_scf_options* o)
     o->verbose = 0;
     o->pad = 0;
     o->guess_density = nullptr;
+    o->incremental_tau = 0.0;   // the reference rebuilds F from D every step
+    o->rebuild_every = 8;
+    o->pad2 = 0;
     o->converge = 1.0e-7;   // HartreeFock.hs:113
     o->charge = 0.0;
 }
@@ -276,6 +279,7 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
         return hfk_set_error(HFK_EINVAL, "hfk_scf: unknown guess %d", opt.guess);
     if (opt.guess == HFK_GUESS_DENSITY && !opt.guess_density)
         return hfk_set_error(HFK_EINVAL, "hfk_scf: HFK_GUESS_DENSITY needs guess_density");
+    if (!(opt.incremental_tau >= 0.0)) return hfk_set_error(HFK_EINVAL, "hfk_scf: incremental_tau must be >= 0");
     // nuclearRep (HartreeFock.hs:300-305)
     double vnn = 0.0;
     for (int a = 0; a < c->natom; ++a)
@@ -291,12 +295,16 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
     Work W;
     int rc = work_alloc(c, W);
     double *Dflat = nullptr, *Fflat = nullptr, *Fd = nullptr, *Dfull = nullptr, *Sfull = nullptr;
+    double *Dprev = nullptr, *dD = nullptr;   // incremental builds: last D built from, D - Dprev
     std::vector<double*> errs, focks;   // DIIS history, newest first
     if (rc == HFK_OK) rc = dev_alloc(&Dflat, nf);
     if (rc == HFK_OK) rc = dev_alloc(&Fflat, nf);
     if (rc == HFK_OK) rc = dev_alloc(&Fd, nf);
     if (rc == HFK_OK) rc = dev_alloc(&Dfull, nn);
     if (rc == HFK_OK) rc = dev_alloc(&Sfull, nn);
+    const bool incremental = opt.incremental_tau > 0.0;
+    if (rc == HFK_OK && incremental) rc = dev_alloc(&Dprev, nf);
+    if (rc == HFK_OK && incremental) rc = dev_alloc(&dD, nf);
     for (int k = 0; k < opt.diis_size && rc == HFK_OK; ++k) {
         double *e = nullptr, *f = nullptr;
         rc = dev_alloc(&e, nn);
@@ -324,9 +332,27 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
             HFK_TRY(diagonal_hf(W, Fflat, occ, Dflat));
         }
         bool diis_on = false;
+        int since_full = 0;
         for (int step = 0; step < opt.max_steps; ++step) {
             auto f0 = std::chrono::steady_clock::now();
-            HFK_TRY(hfk_ctx_fock_device(c, W.H, Dflat, Fflat));   // fock (HartreeFock.hs:213-222)
+            // fock (HartreeFock.hs:213-222).  Incremental builds (not in the
+            // reference): G is linear in D, so F(D_k) = F(D_k-1) + G(D_k - D_k-1);
+            // a quartet contributes at most 4 |(ab|cd)| max|dD| <= 4 Q_ab Q_cd max|dD|
+            // to any element, so those below tau / max|dD| are skipped.  Fflat
+            // still holds F(D_k-1) (DIIS works on copies).
+            if (incremental && step > 0 && since_full + 1 < std::max(1, opt.rebuild_every)) {
+                HFK_HIP(hipMemcpyAsync(dD, Dflat, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream));
+                hfk_launch_axpby((long long)nf, -1.0, Dprev, 1.0, dD, c->stream);
+                hfk_launch_reduce((long long)nf, dD, nullptr, 1, W.part, W.scal + 3, c->stream);
+                const double dmax = read_scalar(c, W.scal + 3);
+                const double cut = dmax > 0.0 ? opt.incremental_tau / dmax : INFINITY;
+                HFK_TRY(hfk_ctx_fock_device(c, Fflat, dD, Fflat, cut));
+                ++since_full;
+            } else {
+                HFK_TRY(hfk_ctx_fock_device(c, W.H, Dflat, Fflat));
+                since_full = 0;
+            }
+            if (incremental) HFK_HIP(hipMemcpyAsync(Dprev, Dflat, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream));
             fock_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - f0).count();
             // calcErrorMtx (DIIS.hs:119-132): X^T (F D S - S D F) X
             hfk_launch_flat_to_full(n, Fflat, W.Ffull, c->stream);
@@ -435,6 +461,8 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
     dev_free(c, Fd);
     dev_free(c, Dfull);
     dev_free(c, Sfull);
+    dev_free(c, Dprev);
+    dev_free(c, dD);
     work_free(W);
     return rc;
 }

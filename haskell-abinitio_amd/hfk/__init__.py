@@ -168,11 +168,14 @@ class Context:
         return H
 
     def scfHF(self, charge=None, guess=HFK_GUESS_CORE, max_steps=30, diis_size=5, converge=1e-7, diis_start=0.1,
-              verbose=False, guess_density=None):
+              verbose=False, guess_density=None, incremental_tau=0.0, rebuild_every=8):
         """scfHF.  Defaults are the reference's (30 steps, 5 DIIS vectors,
         DIIS switched on once max|e| < 0.1, converged at max|e| < 1e-7);
         diis_start=float('inf') switches DIIS on from the second step.
-        guess_density: flat starting density (overrides guess)."""
+        guess_density: flat starting density (overrides guess).
+        incremental_tau > 0: Fock builds from the density change, skipping
+        shell quartets with Q_ab Q_cd max|dD| < tau, with a full build every
+        rebuild_every steps (the reference rebuilds every step)."""
         o = _lib.HfkScfOptions()
         _lib.lib().hfk_default_scf_options(C.byref(o))
         o.guess = guess
@@ -187,6 +190,8 @@ class Context:
         o.converge = converge
         o.diis_start = diis_start
         o.verbose = 1 if verbose else 0
+        o.incremental_tau = float(incremental_tau)
+        o.rebuild_every = int(rebuild_every)
         o.charge = self.mol.charge if charge is None else float(charge)
         r = _lib.HfkScfResult()
         n = self.n

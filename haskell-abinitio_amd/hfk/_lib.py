@@ -44,6 +44,7 @@ class HfkScfOptions(C.Structure):
         ("guess", C.c_int32), ("max_steps", C.c_int32), ("diis_size", C.c_int32),
         ("diis_start", C.c_double), ("converge", C.c_double), ("charge", C.c_double),
         ("verbose", C.c_int32), ("pad", C.c_int32), ("guess_density", C.POINTER(C.c_double)),
+        ("incremental_tau", C.c_double), ("rebuild_every", C.c_int32), ("pad2", C.c_int32),
     ]
 
 

@@ -140,6 +140,12 @@ typedef struct hfk_scf_options {
     int32_t verbose;           /* new: 1 = one progress line per step on stderr        */
     int32_t pad;
     const double *guess_density; /* new: flat starting density for HFK_GUESS_DENSITY     */
+    double incremental_tau;    /* new: 0 = full Fock build every step (the reference);
+                                  > 0: F_k = F(D_k-1) + G(D_k - D_k-1), skipping shell
+                                  quartets with Q_ab Q_cd max|dD| < tau (Q: Schwarz)   */
+    int32_t rebuild_every;     /* new: with incremental_tau, a full build every this
+                                  many steps (<= 1: every step)                        */
+    int32_t pad2;
 } hfk_scf_options;
 
 typedef struct hfk_scf_result {
