@@ -137,6 +137,50 @@ def gradient_bench(workload, reps=3):
     return res
 
 
+def target_bench(nwat=64, thr=1e-4, tau=1e-12):
+    """(H2O)_64/6-31G: energy and gradient in FP64 and in mixed precision
+    (FP32 below Schwarz thr), both from the fragment guess, with incremental
+    Fock builds (incremental_tau=tau); plus one FP64 SCF with a full Fock build
+    every step (the reference's scheme) for its wall time and energy."""
+    import numpy as np
+    import hfk
+    from hfk import systems
+    mol = systems.water_cluster(nwat, "6-31g")
+    res = {"workload": "(H2O)_%d RHF/6-31G, %d basis functions, %d atoms" % (nwat, mol.nbf, mol.natom),
+           "fp32_threshold": thr, "incremental_tau": tau}
+    scf = dict(diis_start=float("inf"), diis_size=8, max_steps=100)
+    t0 = time.perf_counter()
+    with hfk.Context(mol) as ctx:
+        Dg = ctx.fragment_guess()
+    res["fragment_guess_s"] = time.perf_counter() - t0
+    log("bench: target %s fragment guess %.2fs" % (res["workload"], res["fragment_guess_s"]))
+    out = {}
+    for name, opts, inc in (("fp64_full_builds", hfk.options(), 0.0), ("fp64", hfk.options(), tau),
+                            ("mixed", hfk.options(fp32_threshold=thr), tau)):
+        t0 = time.perf_counter()
+        with hfk.Context(mol, opts) as ctx:
+            t_ctx = time.perf_counter() - t0
+            r = ctx.scfHF(guess_density=Dg, incremental_tau=inc, **scf)
+            rec = {"energy": r.getEnergy, "steps": r.steps, "max_error": r.max_error, "ctx_s": t_ctx,
+                   "scf_s": r.seconds, "fock_s": r.fock_seconds,
+                   "wall_s": t_ctx + res["fragment_guess_s"] + r.seconds}
+            if name != "fp64_full_builds":
+                t = time.perf_counter()
+                g = ctx.gradient(r.getDensity)
+                rec["gradient_s"] = time.perf_counter() - t
+                rec["translation_residual"] = float(np.max(np.abs(g.sum(axis=0))))
+                out[name] = (r, g)
+        res[name] = rec
+        log("bench: target %s E=%.10f %d steps scf %.2fs%s" % (name, r.getEnergy, r.steps, r.seconds,
+                                                                 " grad %.2fs" % rec["gradient_s"] if "gradient_s" in rec else ""))
+    (r64, g64), (r32, g32) = out["fp64"], out["mixed"]
+    res["dE"] = r32.getEnergy - r64.getEnergy
+    res["max_dgrad"] = float(np.max(np.abs(g32 - g64)))
+    res["max_deps"] = float(np.max(np.abs(np.sort(r32.getOrbE) - np.sort(r64.getOrbE))))
+    res["dE_incremental_vs_full"] = r64.getEnergy - res["fp64_full_builds"]["energy"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +193,7 @@ def main():
     ap.add_argument("--mixed-workload", default="h2o32", choices=sorted(WORKLOADS))
     ap.add_argument("--mixed-thr", type=float, default=1e-4)
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-target", action="store_true", help="skip the (H2O)_64 FP64-vs-mixed energy+gradient record")
     ap.add_argument("--grad-workloads", default="h2o16,c12h26",
                     help="comma list of gradient/SCF workloads (h2oN or c12h26)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -279,6 +324,8 @@ def main():
         out["mixed_vs_fp64"] = mixed_vs_fp64(args.mixed_workload, args.mixed_thr)
     if not args.no_grad:
         out["gradient"] = [gradient_bench(w) for w in args.grad_workloads.split(",") if w]
+    if not args.no_target:
+        out["target_h2o64"] = target_bench()
     if not args.no_cpu:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(mol)
