@@ -45,12 +45,14 @@ def fragments(mol, scale=1.3):
     return [groups[r] for r in sorted(groups)]
 
 
-def fragment_guess(mol, make_context, converge=1e-8, max_steps=60, diis_size=6):
+def fragment_guess(mol, make_context, converge=1e-8, max_steps=60, diis_size=6, workers=4):
     """Block-diagonal flat density from per-fragment SCF runs, or None when
     the molecule is one fragment, charged, or has an open-shell fragment
     (the caller then keeps its own guess).
 
-    make_context(sub_molecule) returns an open Context for a fragment."""
+    make_context(sub_molecule) returns an open Context for a fragment.  The
+    fragments run on `workers` threads, each with its own context (library
+    calls release the GIL; contexts share no state)."""
     if mol.charge != 0.0:
         return None
     frags = fragments(mol)
@@ -60,17 +62,26 @@ def fragment_guess(mol, make_context, converge=1e-8, max_steps=60, diis_size=6):
         if int(round(mol.Z[atoms].sum())) % 2:
             return None
     n = mol.nbf
-    D = np.zeros((n, n))
-    for atoms in frags:
+
+    def solve(atoms):
         sub = Molecule([mol.atoms[a] for a in atoms])
-        idx = np.nonzero(np.isin(mol.atom_of_bf, atoms))[0]
-        assert len(idx) == sub.nbf
         with make_context(sub) as ctx:
             r = ctx.scfHF(converge=converge, max_steps=max_steps, diis_start=float("inf"), diis_size=diis_size)
-        m = sub.nbf
+        return sub.nbf, r.getDensity
+
+    if workers > 1 and len(frags) > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(max_workers=workers) as pool:
+            dens = list(pool.map(solve, frags))
+    else:
+        dens = [solve(atoms) for atoms in frags]
+    D = np.zeros((n, n))
+    for atoms, (m, flat) in zip(frags, dens):
+        idx = np.nonzero(np.isin(mol.atom_of_bf, atoms))[0]
+        assert len(idx) == m
         Df = np.zeros((m, m))
         iu = np.triu_indices(m)
-        Df[iu] = r.getDensity
-        Df[(iu[1], iu[0])] = r.getDensity
+        Df[iu] = flat
+        Df[(iu[1], iu[0])] = flat
         D[np.ix_(idx, idx)] = Df
     return D[np.triu_indices(n)].copy()
