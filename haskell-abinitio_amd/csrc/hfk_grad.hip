@@ -87,13 +87,28 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
     a.natom = natom;
     a.grad_lds = 1;
     int rc = HFK_OK;
+    // the class launches only add (atomically) into the per-atom sums, so
+    // they run concurrently, round-robin over the auxiliary streams (forked
+    // from and joined back into the context stream)
+    const bool conc = c->concurrent && c->plan.size() > 1;
+    if (conc) {
+        HFK_HIP(hipEventRecord(c->ev_fork, c->stream));
+        for (int k = 0; k < hfk_ctx::NAUX; ++k) HFK_HIP(hipStreamWaitEvent(c->aux[k], c->ev_fork, 0));
+    }
+    int slot = 0;
     for (const PlanEntry& e : c->plan) {
         hfk_plan_args(c, e, a);   // FP32 entries select their quartets; all run in FP64 here
-        if (hfk_launch_grad(e.la, e.lb, e.lc, e.ld, c->opt.boys_mode, e.mode, a, c->stream)) {
+        hipStream_t s = conc ? c->aux[slot++ % hfk_ctx::NAUX] : c->stream;
+        if (hfk_launch_grad(e.la, e.lb, e.lc, e.ld, c->opt.boys_mode, e.mode, a, s)) {
             rc = hfk_set_error(HFK_EUNSUP, "no gradient kernel for class (%d%d|%d%d)", e.la, e.lb, e.lc, e.ld);
             break;
         }
     }
+    if (conc)
+        for (int k = 0; k < hfk_ctx::NAUX; ++k) {
+            HFK_HIP(hipEventRecord(c->ev_join[k], c->aux[k]));
+            HFK_HIP(hipStreamWaitEvent(c->stream, c->ev_join[k], 0));
+        }
     if (rc == HFK_OK) {
         hfk_launch_grad_onee(hfk_onee_args(c), c->d_atom_of_bf, c->d_flatA, c->d_flatB, d_grad, c->stream);
         if (hipGetLastError() != hipSuccess ||
