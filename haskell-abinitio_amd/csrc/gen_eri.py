@@ -127,8 +127,6 @@ def gen_class(la, lb, lc, ld):
         em.raw("    }")
     Ek = [hermite_E(em, lc, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
     out.extend(em.lines)
-    pipe = nbra * nket <= PIPE_NV
-
     def body(em2, sx, ibx):
         """One bra primitive pair (record pb<sx>, index ibx): returns
         {V index: (value name, bra component, ket component)}."""
@@ -144,10 +142,7 @@ def gen_class(la, lb, lc, ld):
             em2.raw("      }")
         Eb = [hermite_E(em2, la, lb, "bA" + ax + sx, "bB" + ax + sx, "boo2p" + sx, "b" + ax) for ax in "xyz"]
         em2.raw("      T F%s[%d];" % (sx, L + 1))
-        if pipe:
-            em2.raw("      boys_finish<T, %d, BOYS>(bf%s, bt, F%s);" % (L, sx, sx))
-        else:
-            em2.raw("      boys_fast<T, %d, BOYS>(Targ%s, bt, F%s);" % (L, sx, sx))
+        em2.raw("      boys_fast<T, %d, BOYS>(Targ%s, bt, F%s);" % (L, sx, sx))
         em2.raw("      const T pref%s = T(HFK_TWO_PI_2_5) * bK%s * kK * rs%s;" % (sx, sx, sx))
         # base R^n_000 = pref * (-2 alpha)^n * F_n
         em2.raw("      const T m2a%s = T(-2) * alpha%s;" % (sx, sx))
@@ -240,47 +235,15 @@ def gen_class(la, lb, lc, ld):
     em2 = Emitter()
     em2.n = em.n
     multi = nket > 1 or nbra > 1
-    if not pipe:
-        out.append("    for (int ib = 0; ib < nbp; ++ib) {")
-        out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
-        out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
-        out.append("      // reference screen: component (ab|cd) of this primitive quartet is kept iff")
-        out.append("      // !(q_ab*q_cd < tau).  MODE 0: no screen; MODE 1: the screen, branch-free.")
-        out.append("      if (MODE == 1 && bqhi * kqhi < qtau) continue;   // every component screened")
-        out.append("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
-        out.append("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
-        out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
-    else:
-        # software pipeline: the geometry and Boys table row of bra primitive
-        # pair ib+1 are fetched before ib is evaluated, so that the table
-        # loads are in flight during the arithmetic of ib
-        out.append("    T alphaN = T(0), XpqN = T(0), YpqN = T(0), ZpqN = T(0), rsN = T(0), TargN = T(0);")
-        out.append("    BoysFetch<T, %d, BOYS> bfN;" % L)
-        out.append("    bool onN = false;")
-        out.append("    if (nbp > 0) {")
-        out.append("      onN = !(MODE == 1 && bra[PP_QHI] * kqhi < qtau);")
-        out.append("      if (onN) {")
-        out.append("        quartet_geom<T>(bra, kq, kQx, kQy, kQz, alphaN, XpqN, YpqN, ZpqN, rsN, TargN);")
-        out.append("        boys_fetch<T, %d, BOYS>(TargN, bt, bfN);" % L)
-        out.append("      }")
-        out.append("    }")
-        out.append("    for (int ib = 0; ib < nbp; ++ib) {")
-        out.append("      const bool on = onN;")
-        out.append("      const T alpha = alphaN, Xpq = XpqN, Ypq = YpqN, Zpq = ZpqN, rs = rsN;")
-        out.append("      const BoysFetch<T, %d, BOYS> bf = bfN;" % L)
-        out.append("      if (ib + 1 < nbp) {")
-        out.append("        const double* pn = bra + (size_t)(ib + 1) * PP_STRIDE;")
-        out.append("        // reference screen: component (ab|cd) of a primitive quartet is kept iff")
-        out.append("        // !(q_ab*q_cd < tau); a pair whose largest value fails is skipped")
-        out.append("        onN = !(MODE == 1 && pn[PP_QHI] * kqhi < qtau);")
-        out.append("        if (onN) {")
-        out.append("          quartet_geom<T>(pn, kq, kQx, kQy, kQz, alphaN, XpqN, YpqN, ZpqN, rsN, TargN);")
-        out.append("          boys_fetch<T, %d, BOYS>(TargN, bt, bfN);" % L)
-        out.append("        }")
-        out.append("      }")
-        out.append("      if (!on) continue;")
-        out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
-        out.append("      const bool allin = MODE == 0 || (pb[PP_QLO] * kqlo >= qtau);")
+    out.append("    for (int ib = 0; ib < nbp; ++ib) {")
+    out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
+    out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
+    out.append("      // reference screen: component (ab|cd) of this primitive quartet is kept iff")
+    out.append("      // !(q_ab*q_cd < tau).  MODE 0: no screen; MODE 1: the screen, branch-free.")
+    out.append("      if (MODE == 1 && bqhi * kqhi < qtau) continue;   // every component screened")
+    out.append("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
+    out.append("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
+    out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
     vals = body(em2, "", "ib")
     for vi in sorted(vals):
         name_, bi, ki = vals[vi]
@@ -520,10 +483,6 @@ def gen_grad_class(la, lb, lc, ld):
 # them); the host's hfk_class_coop() uses the same bounds.
 COOP_NV_JK = 54
 COOP_NV_GRAD = 18
-# Classes with at most this many integrals per quartet software-pipeline the
-# bra primitive loop (geometry and Boys table row of the next primitive pair
-# fetched before the current one is evaluated).
-PIPE_NV = int(os.environ.get("HFK_PIPE_NV", "0"))
 
 
 def pair_classes(lmax):

@@ -192,59 +192,6 @@ __device__ __forceinline__ void boys_fast(T x, const BoysTables& bt, T* F)
 }
 
 
-// boys_fast in two steps, so that a loop can fetch the table row of the next
-// primitive quartet before it evaluates the current one:
-//   boys_fetch(x)  -> the node values F_m(x_k) (loads only; none for x_k > 100)
-//   boys_finish    -> F_0..F_L, the same arithmetic as boys_fast.
-template <typename T, int L, int BOYS>
-struct BoysFetch {
-    T f[BOYS == 0 ? L + 6 : 1];
-    T x, xk;
-};
-
-template <typename T, int L, int BOYS>
-__device__ __forceinline__ void boys_fetch(T x, const BoysTables& bt, BoysFetch<T, L, BOYS>& r)
-{
-    r.x = x;
-    if constexpr (BOYS == 0) {
-        const T fl = boys_ref_cell(x);
-        r.xk = fl * T(0.1);
-        if (r.xk <= T(100)) {
-            int idx = (int)fl;
-            idx = idx < 0 ? 0 : idx;   // NaN guard
-            const T* tab = boys_ref_tab<T>(bt) + (size_t)idx * BOYS_REF_STRIDE;
-#pragma unroll
-            for (int m = 0; m < L + 6; ++m) r.f[m] = tab[m];
-        }
-    }
-}
-
-template <typename T, int L, int BOYS>
-__device__ __forceinline__ void boys_finish(const BoysFetch<T, L, BOYS>& r, const BoysTables& bt, T* F)
-{
-    if constexpr (BOYS == 0) {
-        if (r.xk <= T(100)) {
-            const T d = r.xk - r.x;   // -dx
-            const T w2 = (T(0.5) * d) * d, w3 = (T(1.0 / 3.0) * d) * w2, w4 = (T(0.25) * d) * w3, w5 = (T(0.2) * d) * w4;
-#pragma unroll
-            for (int m = 0; m <= L; ++m)
-                F[m] = r.f[m] + d * r.f[m + 1] + w2 * r.f[m + 2] + w3 * r.f[m + 3] + w4 * r.f[m + 4] + w5 * r.f[m + 5];
-        } else {
-            const T rs = rsqrt_full(r.x * T(1.0 / HFK_PI));   // sqrt(pi / x)
-            const T inv = (rs * rs) * T(1.0 / HFK_PI);        // 1 / x
-            T s = rs;
-#pragma unroll
-            for (int m = 0; m <= L; ++m) {
-                F[m] = T(bt.asym[m]) * s;
-                s *= inv;
-            }
-        }
-    } else {
-        boys_eval<T, L, BOYS>(r.x, bt, F);
-    }
-}
-
-
 // Geometry of one primitive quartet from the bra record pb and the ket
 // pair (q, Q): alpha = pq/(p+q), R_PQ, T = alpha |R_PQ|^2, rs = 1/sqrt(p+q).
 template <typename T>

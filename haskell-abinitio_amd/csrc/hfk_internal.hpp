@@ -116,14 +116,12 @@ void hfk_launch_pair_setup(const ShellDev& sh, int npair, const int* pair_A, con
 void hfk_launch_boys_tables(const double* glx, const double* glw, double* ref64, float* ref32,
                             double* acc64, float* acc32, hipStream_t s);
 void hfk_launch_fill(double* p, double v, long long n, hipStream_t s);
-void hfk_launch_count(const QuartetArgs& a, int na, int nb, int nc, int nd, unsigned long long* out, hipStream_t s);
-// The same counts over a launch's task list (surviving quartets only): shell and
-// function quartets once per quartet, primitive quartets per primitive range.
+// Exact counts over a launch's task list (surviving quartets only): shell,
+// primitive and canonical function quartets, out[0..2] (atomic adds).  Shell and
+// function quartets are counted once per quartet, primitive quartets per
+// primitive range.
 void hfk_launch_count_tasks(const QuartetArgs& a, int na, int nb, int nc, int nd, unsigned long long* out,
                             hipStream_t s);
-// Quartets of the task list that hold a partially screened primitive quartet:
-// out==nullptr counts them into *count; otherwise appends {bra, ket} pairs.
-void hfk_launch_find_mixed(const QuartetArgs& a, unsigned long long* count, int2* out, hipStream_t s);
 // J, K padded (leading dimension n+1)
 void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const double* H, double* G_flat,
                              double* F_full, hipStream_t s);

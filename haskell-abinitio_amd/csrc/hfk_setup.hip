@@ -292,53 +292,6 @@ __global__ void k_full_to_flat(int n, const double* full, double* flat)
     }
 }
 
-// Counts, for one class launch, the shell quartets passing the pair screens,
-// the primitive quartets not screened as a whole, and the canonical
-// function quartets (i<=j, k<=l, (ij)<=(kl)) the shell quartets hold.
-__global__ void k_count(QuartetArgs a, int na, int nb, int nc, int nd, unsigned long long* out)
-{
-    unsigned long long cs = 0, cp = 0, cf = 0;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < a.nquart; q += stride) {
-        long long i, j;
-        if (a.same) {
-            double x = sqrt(8.0 * (double)q + 1.0);
-            long long ii = (long long)((x - 1.0) * 0.5);
-            while (ii * (ii + 1) / 2 > q) --ii;
-            while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
-            i = ii;
-            j = q - ii * (ii + 1) / 2;
-        } else {
-            i = q / a.nket;
-            j = q - i * a.nket;
-        }
-        const int bp = a.bra_list[i], kp = a.ket_list[j];
-        if (a.qtau > 0.0 && a.pair_qhi[bp] * a.pair_qhi[kp] < a.qtau) continue;
-        if (a.sch_cut > 0.0 || a.prec_sel != 0) {
-            const double bound = a.pair_sch[bp] * a.pair_sch[kp];
-            if (bound < a.sch_cut) continue;
-            if (a.prec_sel == 1 && !(bound >= a.fp32_thr)) continue;
-            if (a.prec_sel == 2 && !(bound < a.fp32_thr)) continue;
-        }
-        cs += 1;
-        const double* bra = a.pp + (size_t)a.pair_pp0[bp] * PP_STRIDE;
-        const double* ket = a.pp + (size_t)a.pair_pp0[kp] * PP_STRIDE;
-        for (int x = 0; x < a.pair_npp[bp]; ++x) {
-            const double bq = bra[(size_t)x * PP_STRIDE + PP_QHI];
-            for (int y = 0; y < a.pair_npp[kp]; ++y)
-                if (!(a.qtau > 0.0 && bq * ket[(size_t)y * PP_STRIDE + PP_QHI] < a.qtau)) cp += 1;
-        }
-        const unsigned long long nbra =
-            (a.pair_A[bp] == a.pair_B[bp]) ? (unsigned long long)na * (na + 1) / 2 : (unsigned long long)na * nb;
-        const unsigned long long nket =
-            (a.pair_A[kp] == a.pair_B[kp]) ? (unsigned long long)nc * (nc + 1) / 2 : (unsigned long long)nc * nd;
-        cf += (bp == kp) ? nbra * (nbra + 1) / 2 : nbra * nket;
-    }
-    atomicAdd(out + 0, cs);
-    atomicAdd(out + 1, cp);
-    atomicAdd(out + 2, cf);
-}
-
 // Counts over the task lists: one 64-lane workgroup per task (grid-stride).
 __global__ void __launch_bounds__(64) k_count_tasks(QuartetArgs a, int na, int nb, int nc, int nd,
                                                     unsigned long long* out)
@@ -377,33 +330,6 @@ __global__ void __launch_bounds__(64) k_count_tasks(QuartetArgs a, int na, int n
     if (cf) atomicAdd(out + 2, cf);
 }
 
-// One 64-lane workgroup per task (same layout as the tiled J/K kernel).
-__global__ void __launch_bounds__(64) k_find_mixed(QuartetArgs a, unsigned long long* count, int2* out)
-{
-    const int4 task = a.tasks[blockIdx.x];
-    const int bp = task.x;
-    const double* bra = a.pp + (size_t)a.pair_pp0[bp] * PP_STRIDE;
-    const int nbp = a.pair_npp[bp];
-    const int len = task_ket_count(a, task);
-    for (int j = (int)threadIdx.x; j < len; j += 64) {
-        const int kp = a.ket_idx[task_ket_pos(a, task, j)];
-        if (a.qtau > 0.0 && a.pair_qhi[bp] * a.pair_qhi[kp] < a.qtau) continue;
-        const double* ket = a.pp + (size_t)a.pair_pp0[kp] * PP_STRIDE;
-        bool mixed = false;
-        for (int x = 0; x < nbp && !mixed; ++x) {
-            const double blo = bra[(size_t)x * PP_STRIDE + PP_QLO], bhi = bra[(size_t)x * PP_STRIDE + PP_QHI];
-            for (int y = 0; y < a.pair_npp[kp]; ++y) {
-                const double klo = ket[(size_t)y * PP_STRIDE + PP_QLO], khi = ket[(size_t)y * PP_STRIDE + PP_QHI];
-                if (!(bhi * khi < a.qtau) && !(blo * klo >= a.qtau)) { mixed = true; break; }
-            }
-        }
-        if (mixed) {
-            const unsigned long long k = atomicAdd(count, 1ULL);
-            if (out) out[k] = make_int2(bp, kp);
-        }
-    }
-}
-
 unsigned grid_for(long long n, int block)
 {
     long long g = (n + block - 1) / block;
@@ -431,32 +357,12 @@ void hfk_launch_boys_tables(const double* glx, const double* glw, double* ref64,
                        acc32);
 }
 
-void hfk_launch_count(const QuartetArgs& a, int na, int nb, int nc, int nd, unsigned long long* out, hipStream_t s)
-{
-    if (a.nquart <= 0) return;
-    long long g = (a.nquart + 255) / 256;
-    if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(k_count, dim3((unsigned)g), dim3(256), 0, s, a, na, nb, nc, nd, out);
-}
-
 void hfk_launch_count_tasks(const QuartetArgs& a, int na, int nb, int nc, int nd, unsigned long long* out,
                             hipStream_t s)
 {
     if (a.ntask <= 0) return;
     const long long g = std::min<long long>(a.ntask, 8192);
     hipLaunchKernelGGL(k_count_tasks, dim3((unsigned)g), dim3(64), 0, s, a, na, nb, nc, nd, out);
-}
-
-void hfk_launch_find_mixed(const QuartetArgs& a, unsigned long long* count, int2* out, hipStream_t s)
-{
-    long long done = 0;
-    while (done < a.ntask) {
-        const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);
-        QuartetArgs b = a;
-        b.tasks = a.tasks + done;
-        hipLaunchKernelGGL(k_find_mixed, dim3((unsigned)cnt), dim3(64), 0, s, b, count, out);
-        done += cnt;
-    }
 }
 
 void hfk_launch_fill(double* p, double v, long long n, hipStream_t s)
