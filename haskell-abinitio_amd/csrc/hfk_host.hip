@@ -896,13 +896,10 @@ extern "C" int hfk_get_class_stats(hfk_ctx* c, hfk_class_stat* out, int32_t cap,
 // Device Fock build: G (or H+G) flat from a flat density, all on device.
 int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat, double extra_cut)
 {
-    const size_t np1 = (size_t)(c->n + 1) * (c->n + 1);
-    hfk_launch_flat_to_padded(c->n, D_dev_flat, c->d_D, c->stream);
-    hfk_launch_fill(c->d_J, 0.0, (long long)np1, c->stream);
-    hfk_launch_fill(c->d_K, 0.0, (long long)np1, c->stream);
-    // timed: the ket-side sums (cleared, then scattered) belong to the digestion
+    // timed from here: the set-up launch (D padded, J', K' and the ket-side
+    // sums cleared), the class launches and the scatter of the ket-side sums
     HFK_HIP(hipEventRecord(c->ev0, c->stream));
-    if (c->jk_len) hfk_launch_fill(c->d_Jk, 0.0, (long long)c->jk_len, c->stream);
+    hfk_launch_fock_prepare(c->n, D_dev_flat, c->d_D, c->d_J, c->d_K, c->d_Jk, (long long)c->jk_len, c->stream);
     QuartetArgs a = hfk_ctx_args(c);
     a.sch_cut = std::max(c->opt.schwarz_cut, extra_cut);
     a.D = c->d_D;

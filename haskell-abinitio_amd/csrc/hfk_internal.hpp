@@ -139,6 +139,9 @@ void hfk_launch_scatter_jk(double* J, const double* Jk, int copies, const JkScat
                            const int* pair_A, const int* pair_B, const int* sh_bfp, int ld, hipStream_t s);
 // flat -> padded (n+1)x(n+1) full, row/column n zero
 void hfk_launch_flat_to_padded(int n, const double* flat, double* full, hipStream_t s);
+// padded D from the flat density; J', K' ((n+1)^2) and Jk (jk_len) zeroed; one launch
+void hfk_launch_fock_prepare(int n, const double* D_flat, double* D, double* J, double* K, double* Jk, long long jk_len,
+                             hipStream_t s);
 void hfk_launch_flat_to_full(int n, const double* flat, double* full, hipStream_t s);
 void hfk_launch_full_to_flat(int n, const double* full, double* flat, hipStream_t s);
 

@@ -254,26 +254,49 @@ __global__ void k_flat_to_full(int n, const double* flat, double* full)
     }
 }
 
-__global__ void k_flat_to_padded(int n, const double* flat, double* full)
+// One element idx < n(n+1)/2 + n + 1 of the flat -> padded (n+1)^2 conversion.
+__device__ __forceinline__ void flat_to_padded_elem(int n, const double* flat, double* full, long long idx)
 {
     const long long m = (long long)n * (n + 1) / 2;
     const long long ld = n + 1;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < m + ld;
+    if (idx >= m) {   // zero row/column n
+        const long long t = idx - m;
+        full[n * ld + t] = 0.0;
+        full[t * ld + n] = 0.0;
+        return;
+    }
+    const double nn = (double)n + 0.5;
+    long long i = (long long)(nn - sqrt(nn * nn - 2.0 * (double)idx));
+    if (i < 0) i = 0;
+    while (i > 0 && i * n - i * (i - 1) / 2 > idx) --i;
+    while ((i + 1) * n - (i + 1) * i / 2 <= idx) ++i;
+    const long long j = i + (idx - (i * n - i * (i - 1) / 2));
+    full[i * ld + j] = flat[idx];
+    full[j * ld + i] = flat[idx];
+}
+
+__global__ void k_flat_to_padded(int n, const double* flat, double* full)
+{
+    const long long m = (long long)n * (n + 1) / 2;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < m + n + 1;
+         idx += (long long)gridDim.x * blockDim.x)
+        flat_to_padded_elem(n, flat, full, idx);
+}
+
+// Start of a Fock build in one launch: padded D from the flat density, and
+// J', K' ((n+1)^2 each) and the ket-side sums (jk_len) cleared.
+__global__ void k_fock_prepare(int n, const double* D_flat, double* D, double* J, double* K, double* Jk, long long jk_len)
+{
+    const long long m = (long long)n * (n + 1) / 2, np1 = (long long)(n + 1) * (n + 1);
+    const long long total = max(max(m + n + 1, np1), jk_len);
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (long long)gridDim.x * blockDim.x) {
-        if (idx >= m) {   // zero row/column n
-            const long long t = idx - m;
-            full[n * ld + t] = 0.0;
-            full[t * ld + n] = 0.0;
-            continue;
+        if (idx < m + n + 1) flat_to_padded_elem(n, D_flat, D, idx);
+        if (idx < np1) {
+            J[idx] = 0.0;
+            K[idx] = 0.0;
         }
-        const double nn = (double)n + 0.5;
-        long long i = (long long)(nn - sqrt(nn * nn - 2.0 * (double)idx));
-        if (i < 0) i = 0;
-        while (i > 0 && i * n - i * (i - 1) / 2 > idx) --i;
-        while ((i + 1) * n - (i + 1) * i / 2 <= idx) ++i;
-        const long long j = i + (idx - (i * n - i * (i - 1) / 2));
-        full[i * ld + j] = flat[idx];
-        full[j * ld + i] = flat[idx];
+        if (idx < jk_len) Jk[idx] = 0.0;
     }
 }
 
@@ -420,6 +443,14 @@ void hfk_launch_flat_to_padded(int n, const double* flat, double* full, hipStrea
 {
     const long long m = (long long)n * (n + 1) / 2 + n + 1;
     hipLaunchKernelGGL(k_flat_to_padded, dim3(grid_for(m, 256)), dim3(256), 0, s, n, flat, full);
+}
+
+void hfk_launch_fock_prepare(int n, const double* D_flat, double* D, double* J, double* K, double* Jk, long long jk_len,
+                             hipStream_t s)
+{
+    const long long m = (long long)n * (n + 1) / 2, np1 = (long long)(n + 1) * (n + 1);
+    const long long total = std::max(std::max(m + n + 1, np1), jk_len);
+    hipLaunchKernelGGL(k_fock_prepare, dim3(grid_for(total, 256)), dim3(256), 0, s, n, D_flat, D, J, K, Jk, jk_len);
 }
 
 void hfk_launch_full_to_flat(int n, const double* full, double* flat, hipStream_t s)
