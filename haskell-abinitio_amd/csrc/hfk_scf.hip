@@ -195,7 +195,7 @@ extern "C" void hfk_default_scf_options(hfk_scf_options* o)
     o->pad = 0;
     o->guess_density = nullptr;
     o->incremental_tau = 0.0;   // the reference rebuilds F from D every step
-    o->rebuild_every = 8;
+    o->rebuild_every = 16;
     o->pad2 = 0;
     o->converge = 1.0e-7;   // HartreeFock.hs:113
     o->charge = 0.0;

@@ -168,7 +168,7 @@ class Context:
         return H
 
     def scfHF(self, charge=None, guess=HFK_GUESS_CORE, max_steps=30, diis_size=5, converge=1e-7, diis_start=0.1,
-              verbose=False, guess_density=None, incremental_tau=0.0, rebuild_every=8):
+              verbose=False, guess_density=None, incremental_tau=0.0, rebuild_every=16):
         """scfHF.  Defaults are the reference's (30 steps, 5 DIIS vectors,
         DIIS switched on once max|e| < 0.1, converged at max|e| < 1e-7);
         diis_start=float('inf') switches DIIS on from the second step.
