@@ -7,6 +7,7 @@
 // with W = 1/2 D F D (= 2 sum_occ eps_i C_i C_i^T at convergence).
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "hfk_host.hpp"
@@ -28,7 +29,11 @@ static int grad_upload_maps(hfk_ctx* c)
     return HFK_OK;
 }
 
-// W = 1/2 D F D with F = H + G(D); D, W flat on device.
+// W = 1/2 D F D with F = H + G(D); D, W flat on device.  After a converged
+// hfk_scf the context holds a density D0 and its Fock matrix F(D0); since G is
+// linear, F(D) = F(D0) + G(D - D0), built with the Schwarz x max|D - D0| screen
+// at tau = 1e-13 (each skipped shell quartet changes an element of F by less
+// than 4e-13).  Near convergence that costs a small part of a full build.
 static int grad_energy_weighted(hfk_ctx* c, const double* Dflat, double* Wflat)
 {
     const int n = c->n;
@@ -38,9 +43,25 @@ static int grad_energy_weighted(hfk_ctx* c, const double* Dflat, double* Wflat)
     if (rc == HFK_OK) rc = dev_alloc(&F, nf);
     if (rc == HFK_OK) rc = dev_alloc(&Df, nn);
     if (rc == HFK_OK) rc = dev_alloc(&Ff, nn);
-    if (rc == HFK_OK) rc = dev_alloc(&T, nn);
-    if (rc == HFK_OK) rc = hfk_ctx_hcore_dev(c, H);
-    if (rc == HFK_OK) rc = hfk_ctx_fock_device(c, H, Dflat, F);
+    if (rc == HFK_OK) rc = dev_alloc(&T, std::max<size_t>(nn, 1024));   // also the reduction partials
+    if (rc == HFK_OK && c->scf_cached) {
+        // Df (n*n >= nf) holds D - D0, T the reduction partials, Ff[0] max|D - D0|
+        hipMemcpyAsync(Df, Dflat, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream);
+        hfk_launch_axpby((long long)nf, -1.0, c->d_scf_D, 1.0, Df, c->stream);
+        hfk_launch_reduce((long long)nf, Df, nullptr, 1, T, Ff, c->stream);
+        double dmax = 0.0;
+        if (hipMemcpyAsync(&dmax, Ff, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = hfk_set_error(HFK_EHIP, "hfk_gradient: density difference failed");
+        if (rc == HFK_OK) {
+            if (dmax > 0.0) rc = hfk_ctx_fock_device(c, c->d_scf_F, Df, F, 1e-13 / dmax);
+            else if (hipMemcpyAsync(F, c->d_scf_F, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                rc = hfk_set_error(HFK_EHIP, "hfk_gradient: copy failed");
+        }
+    } else {
+        if (rc == HFK_OK) rc = hfk_ctx_hcore_dev(c, H);
+        if (rc == HFK_OK) rc = hfk_ctx_fock_device(c, H, Dflat, F);
+    }
     if (rc == HFK_OK) {
         hfk_launch_flat_to_full(n, Dflat, Df, c->stream);
         hfk_launch_flat_to_full(n, F, Ff, c->stream);

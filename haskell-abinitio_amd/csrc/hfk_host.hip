@@ -222,6 +222,7 @@ extern "C" int hfk_set_options(hfk_ctx* c, const hfk_options* opt)
     const bool rebuild = opt->boys_mode != c->opt.boys_mode || opt->screen_mode != c->opt.screen_mode ||
                          opt->screen_tau != c->opt.screen_tau;
     c->opt = *opt;
+    c->scf_cached = false;   // its Fock matrix was built under the old options
     int rc = HFK_OK;
     if (rebuild) {
         rc = hfk_ctx_classify_pairs(c);

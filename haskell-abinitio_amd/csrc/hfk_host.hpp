@@ -79,6 +79,11 @@ struct hfk_ctx {
     hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[NAUX] = {nullptr, nullptr, nullptr};
     int concurrent = 1;
+    // the last converged hfk_scf: the density its last Fock build used and that
+    // (raw, not extrapolated) Fock matrix, flat; hfk_gradient builds F(D) from them
+    double* d_scf_D = nullptr;
+    double* d_scf_F = nullptr;
+    bool scf_cached = false;
     int first_launch = -1;      // plan entry that runs alone first (-1: not yet measured)
     int serial_runs = 0;        // serial J/K runs since planning (the first is cold)
     bool timed_serial = false;  // the last timed run launched serially

@@ -295,7 +295,7 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
     Work W;
     int rc = work_alloc(c, W);
     double *Dflat = nullptr, *Fflat = nullptr, *Fd = nullptr, *Dfull = nullptr, *Sfull = nullptr;
-    double *Dprev = nullptr, *dD = nullptr;   // incremental builds: last D built from, D - Dprev
+    double *Dprev = nullptr, *dD = nullptr;   // last D a Fock matrix was built from; D - Dprev
     std::vector<double*> errs, focks;   // DIIS history, newest first
     if (rc == HFK_OK) rc = dev_alloc(&Dflat, nf);
     if (rc == HFK_OK) rc = dev_alloc(&Fflat, nf);
@@ -303,8 +303,9 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
     if (rc == HFK_OK) rc = dev_alloc(&Dfull, nn);
     if (rc == HFK_OK) rc = dev_alloc(&Sfull, nn);
     const bool incremental = opt.incremental_tau > 0.0;
-    if (rc == HFK_OK && incremental) rc = dev_alloc(&Dprev, nf);
+    if (rc == HFK_OK) rc = dev_alloc(&Dprev, nf);
     if (rc == HFK_OK && incremental) rc = dev_alloc(&dD, nf);
+    c->scf_cached = false;
     for (int k = 0; k < opt.diis_size && rc == HFK_OK; ++k) {
         double *e = nullptr, *f = nullptr;
         rc = dev_alloc(&e, nn);
@@ -352,7 +353,7 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
                 HFK_TRY(hfk_ctx_fock_device(c, W.H, Dflat, Fflat));
                 since_full = 0;
             }
-            if (incremental) HFK_HIP(hipMemcpyAsync(Dprev, Dflat, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream));
+            HFK_HIP(hipMemcpyAsync(Dprev, Dflat, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream));
             fock_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - f0).count();
             // calcErrorMtx (DIIS.hs:119-132): X^T (F D S - S D F) X
             hfk_launch_flat_to_full(n, Fflat, W.Ffull, c->stream);
@@ -445,7 +446,13 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
                 if (D_out) HFK_HIP(hipMemcpyAsync(D_out, Dflat, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
                 if (C_out) HFK_HIP(hipMemcpyAsync(C_out, W.C, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
                 if (eps_out) HFK_HIP(hipMemcpyAsync(eps_out, W.w, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+                // keep (D, F(D)) of the last build for hfk_gradient
+                if (!c->d_scf_D) HFK_TRY(dev_alloc(&c->d_scf_D, nf));
+                if (!c->d_scf_F) HFK_TRY(dev_alloc(&c->d_scf_F, nf));
+                HFK_HIP(hipMemcpyAsync(c->d_scf_D, Dprev, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream));
+                HFK_HIP(hipMemcpyAsync(c->d_scf_F, Fflat, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream));
                 HFK_HIP(hipStreamSynchronize(c->stream));
+                c->scf_cached = true;
                 return HFK_OK;
             }
         }

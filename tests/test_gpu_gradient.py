@@ -220,3 +220,23 @@ def test_gradient_autograd_vs_cpu_fd(require_gpu):
     _say("  E gpu %.12f cpu %.12f; max |dE/dR gpu - fd cpu| %.3e" % (E.item(), e_cpu, np.max(np.abs(R.grad.numpy() - fd))))
     assert abs(E.item() - e_cpu) < 1e-8
     assert np.max(np.abs(R.grad.numpy() - fd)) < 2e-6
+
+
+def test_gradient_after_scf_uses_cached_fock(require_gpu):
+    """After a converged scfHF the context forms F(D) for W from its last Fock
+    build plus G(D - D0); the gradient equals the one from a fresh context
+    (full Fock build) at the same density."""
+    _say("case: (H2O)2 6-31g gradient after SCF vs fresh context")
+    mol = systems.water_cluster(2, "6-31g")
+    with hfk.Context(mol) as ctx:
+        r = ctx.scfHF(converge=1e-9, max_steps=100, diis_start=float("inf"), diis_size=6)
+        g_cached = ctx.gradient(r.getDensity)
+        rng = np.random.default_rng(11)
+        D2 = r.getDensity + _rand_sym_flat(rng, mol.nbf, 1e-3)
+        g_cached2 = ctx.gradient(D2)
+    with hfk.Context(mol) as ctx:
+        g_full = ctx.gradient(r.getDensity)
+        g_full2 = ctx.gradient(D2)
+    err = max(np.max(np.abs(g_cached - g_full)), np.max(np.abs(g_cached2 - g_full2)))
+    _say("  max |cached - full| = %.3e" % err)
+    assert err < 1e-10
