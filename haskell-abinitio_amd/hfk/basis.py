@@ -12,8 +12,9 @@ Mirrors the reference's ingestion semantics:
 * CGF order per shell follows FetchBasis.hs:129-137: SP -> S,Px,Py,Pz;
   P -> Px,Py,Pz; D -> Dxx,Dxy,Dxz,Dyy,Dyz,Dzz.
 
-Unlike ParserBasis.hs:214-216 (which stores 1.0 as every SP exponent), SP
-blocks keep their exponent.
+Unlike ParserBasis.hs:116-118 (`triple` packs each SP primitive as
+(c_s, c_p, 1), so every SP exponent becomes 1.0), SP blocks keep their
+exponent; sp_exponent_bug=True reproduces the reference's behaviour.
 """
 import json
 import math
@@ -112,13 +113,15 @@ def load_basis_sets():
         return json.load(f)
 
 
-def element_cgfs(shells, d_order=SHELL_FUNCS["D"]):
-    """createCGF (FetchBasis.hs:128-137): shells -> list of (funtype, prims)."""
+def element_cgfs(shells, d_order=SHELL_FUNCS["D"], sp_exponent_bug=False):
+    """createCGF (FetchBasis.hs:128-137): shells -> list of (funtype, prims).
+    sp_exponent_bug: every SP primitive gets exponent 1.0, as the reference's
+    parser stores it (ParserBasis.hs:116-118)."""
     cgfs = []
     for kind, prims in shells:
         if kind == "SP":
-            s = [(cs, e) for cs, cp, e in prims]
-            p = [(cp, e) for cs, cp, e in prims]
+            s = [(cs, 1.0 if sp_exponent_bug else e) for cs, cp, e in prims]
+            p = [(cp, 1.0 if sp_exponent_bug else e) for cs, cp, e in prims]
             cgfs.append(("S", norm_coeff(s, "S")))
             for ft in SHELL_FUNCS["P"]:
                 cgfs.append((ft, norm_coeff(p, ft)))
@@ -194,11 +197,11 @@ class Molecule:
         return Molecule(atoms, self.charge)
 
 
-def build_molecule(symbols, coords_bohr, basis_name, charge=0.0, d_order=SHELL_FUNCS["D"]):
+def build_molecule(symbols, coords_bohr, basis_name, charge=0.0, d_order=SHELL_FUNCS["D"], sp_exponent_bug=False):
     sets = load_basis_sets()[basis_name.lower()]
     atoms = []
     for sym, xyz in zip(symbols, coords_bohr):
-        cgfs = element_cgfs([(k, [tuple(p) for p in prims]) for k, prims in sets[sym]], d_order)
+        cgfs = element_cgfs([(k, [tuple(p) for p in prims]) for k, prims in sets[sym]], d_order, sp_exponent_bug)
         atoms.append((sym, tuple(float(v) for v in xyz), cgfs))
     return Molecule(atoms, charge)
 

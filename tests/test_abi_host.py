@@ -49,6 +49,21 @@ def test_basis_parser_keeps_sp_exponents():
     assert kind == "SP" and prims[0] == (-0.1193324, 0.0689991, 7.8682724)
 
 
+def test_sp_exponent_bug_flag():
+    """sp_exponent_bug=True reproduces ParserBasis.hs:116-118 (every SP
+    primitive gets exponent 1.0); S and P shells are untouched."""
+    shells = [("S", [(0.5, 10.0)]), ("SP", [(-0.12, 0.07, 7.87), (-0.16, 0.32, 1.88)])]
+    good = basis.element_cgfs(shells)
+    bug = basis.element_cgfs(shells, sp_exponent_bug=True)
+    assert [ft for ft, _ in good] == [ft for ft, _ in bug] == ["S", "S", "Px", "Py", "Pz"]
+    assert good[0] == bug[0]
+    for (_, pg), (_, pb) in zip(good[1:], bug[1:]):
+        assert [e for _, e in pg] == [7.87, 1.88] and [e for _, e in pb] == [1.0, 1.0]
+    m = basis.build_molecule(["O"], [(0.0, 0.0, 0.0)], "6-31g", sp_exponent_bug=True)
+    # two SP shells of 3 and 1 primitives, stored once per S, Px, Py, Pz function
+    assert m.nbf == 9 and np.count_nonzero(m.expo == 1.0) == (3 + 1) * 4
+
+
 def test_normalisation_matches_reference_formula():
     # normCoeff (NormalizeBasis.hs:52-59): c / sqrt(prod (2k-1)!! / (4e)^L * (pi/2e)^1.5)
     for ft, prod in (("S", 1.0), ("Px", 1.0), ("Dxx", 3.0), ("Dxy", 1.0)):
