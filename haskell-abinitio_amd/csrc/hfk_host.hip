@@ -958,6 +958,12 @@ extern "C" int hfk_eri_flat(hfk_ctx* c, double* out, size_t out_len)
 {
     if (!c || !out) return hfk_set_error(HFK_EINVAL, "hfk_eri_flat: null argument");
     std::lock_guard<std::mutex> g(c->mu);
+    return hfk_ctx_eri_flat(c, out, out_len);
+}
+
+// calcIntegrals into a host array (the caller holds the context lock).
+int hfk_ctx_eri_flat(hfk_ctx* c, double* out, size_t out_len)
+{
     const long long n = c->n, m = n * (n + 1) / 2, len = m * (m + 1) / 2;
     if ((long long)out_len != len)
         return hfk_set_error(HFK_EINVAL, "hfk_eri_flat: out_len %zu, need %lld", out_len, len);

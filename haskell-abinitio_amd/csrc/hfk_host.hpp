@@ -183,6 +183,8 @@ int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op);
 int hfk_ctx_plan(hfk_ctx* c);
 void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a);
 int hfk_ctx_collect_times(hfk_ctx* c);
+// calcIntegrals into a host array of out_len = M(M+1)/2 (caller holds the lock)
+int hfk_ctx_eri_flat(hfk_ctx* c, double* out, size_t out_len);
 // extra_cut > 0 also skips shell quartets with Schwarz product below it
 int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat,
                         double extra_cut = 0.0);

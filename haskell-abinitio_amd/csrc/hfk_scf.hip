@@ -257,6 +257,102 @@ extern "C" int hfk_unitary(hfk_ctx* c, const double* X, const double* F_flat, do
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// The reference's index maps exactly as written, including their behaviour
+// for indices outside a matrix of the given dimension, which the reference's
+// Harris guess produces (NumericalTools/LinearAlgebra.hs:194-283).
+namespace refidx {
+typedef long long ll;
+// sum (takeZero w (iterate pred dim)): takeZero 0 = [0], take of a negative count = []
+static ll calc_elem(ll dim, ll w)
+{
+    ll s = 0;
+    for (ll t = 0; t < w; ++t) s += dim - t;
+    return s;
+}
+static ll dim2_to_flat(ll dim, ll x, ll y)   // indexDIM2toFlat
+{
+    return x <= y ? calc_elem(dim, x) + (y - x) : calc_elem(dim, y) + (x - y);
+}
+static bool flat_to_dim2(ll dim, ll idx, ll& x, ll& y)   // indexFlat2DIM2; false if it would not end
+{
+    ll r = idx, k = 0;
+    while (!(r < dim - k)) {
+        if (dim - k <= 0) return false;
+        r -= dim - k;
+        ++k;
+    }
+    x = k;
+    y = idx + x - calc_elem(dim, x);
+    return true;
+}
+static ll sum_index_triang(ll n) { return (n * n + n) / 2; }   // n(n+1) >= 0: div = floor
+static ll sum_first_index(ll n)
+{
+    ll s = n + sum_index_triang(n - 1);
+    for (ll k = 1; k <= n - 1; ++k) s += sum_index_triang(n) - k;
+    return s;
+}
+static ll four_index_to_flat(ll n, const ll q[4])   // fourIndex2Flat
+{
+    const ll i = q[0], j = q[1], k = q[2], l = q[3];
+    ll a0 = 0;
+    for (ll m = 0; m <= i - 1; ++m) a0 += sum_first_index(n - m);   // totalFirstIndex
+    ll pos;
+    if (i == j && i == k) {
+        pos = l - i;
+    } else if (i == j) {
+        pos = (n - i) + dim2_to_flat(n - (i + 1), k - (i + 1), l - (i + 1));
+    } else {
+        ll s = 0;
+        for (ll m = 1; m <= (j - 1) - i; ++m) s += sum_index_triang(n - i) - m;
+        pos = (n - i) + sum_index_triang((n - 1) - i) + s + (dim2_to_flat(n, k, l) - dim2_to_flat(n, i, j));
+    }
+    return a0 + pos;
+}
+static void sort_keys(ll i, ll j, ll k, ll l, ll q[4])   // sortKeys (HartreeFock.hs:153-156)
+{
+    const ll a0 = std::min(i, j), a1 = std::max(i, j), b0 = std::min(k, l), b1 = std::max(k, l);
+    if (a0 < b0 || (a0 == b0 && a1 <= b1)) { q[0] = a0; q[1] = a1; q[2] = b0; q[3] = b1; }
+    else { q[0] = b0; q[1] = b1; q[2] = a0; q[3] = a1; }
+}
+}   // namespace refidx
+
+// harrisFunctional / computeGuessIntegrals as written (HartreeFock.hs:127-149):
+// G_idx = 2 sum_{k<occ} [eri(i,j,k,k) - 1/2 eri(i,k,k,j)] with (i,j) =
+// indexFlat2DIM2 M idx (M, the flat length, as the dimension) and the flat
+// positions taken by fourIndex2Flat with occ as the basis size.  The reference
+// indexes with `!`, so an index outside the integral array aborts it; here that
+// is HFK_EINVAL.  Host arithmetic on the flat ERI array (small n only).
+static int reference_harris_g(hfk_ctx* c, int occ, std::vector<double>& G)
+{
+    const long long n = c->n, M = n * (n + 1) / 2, len = M * (M + 1) / 2;
+    if (len > (1LL << 26))
+        return hfk_set_error(HFK_EINVAL, "hfk_scf: HFK_GUESS_REFERENCE reads the flat ERI array (%lld entries); "
+                                         "use it for small systems only", len);
+    std::vector<double> eri((size_t)len);
+    HFK_TRY(hfk_ctx_eri_flat(c, eri.data(), eri.size()));
+    G.assign((size_t)M, 0.0);
+    for (long long idx = 0; idx < M; ++idx) {
+        long long i, j;
+        if (!refidx::flat_to_dim2(M, idx, i, j))
+            return hfk_set_error(HFK_EINVAL, "hfk_scf: reference Harris guess: indexFlat2DIM2 does not end");
+        double s = 0.0;
+        for (long long k = 0; k < occ; ++k) {
+            long long q1[4], q2[4];
+            refidx::sort_keys(i, j, k, k, q1);
+            refidx::sort_keys(i, k, k, j, q2);
+            const long long f1 = refidx::four_index_to_flat(occ, q1), f2 = refidx::four_index_to_flat(occ, q2);
+            if (f1 < 0 || f1 >= len || f2 < 0 || f2 >= len)
+                return hfk_set_error(HFK_EINVAL, "hfk_scf: reference Harris guess indexes the integral array at %lld / %lld "
+                                                 "(length %lld); the reference aborts here", f1, f2, len);
+            s += eri[(size_t)f1] - 0.5 * eri[(size_t)f2];
+        }
+        G[(size_t)idx] = 2.0 * s;
+    }
+    return HFK_OK;
+}
+
 // scfHF + scfDIIS (HartreeFock.hs:70-123)
 extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res, double* C_out, double* eps_out,
                        double* D_out, double* F_out)
@@ -275,7 +371,8 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
     for (double z : c->Z) zsum += z;
     const int occ = (int)floor((zsum - opt.charge) / 2.0);   // HartreeFock.hs:75-76
     if (occ <= 0 || occ > n) return hfk_set_error(HFK_EINVAL, "hfk_scf: occupied=%d for n=%d", occ, n);
-    if (opt.guess != HFK_GUESS_CORE && opt.guess != HFK_GUESS_INTENDED && opt.guess != HFK_GUESS_DENSITY)
+    if (opt.guess != HFK_GUESS_CORE && opt.guess != HFK_GUESS_INTENDED && opt.guess != HFK_GUESS_DENSITY &&
+        opt.guess != HFK_GUESS_REFERENCE)
         return hfk_set_error(HFK_EINVAL, "hfk_scf: unknown guess %d", opt.guess);
     if (opt.guess == HFK_GUESS_DENSITY && !opt.guess_density)
         return hfk_set_error(HFK_EINVAL, "hfk_scf: HFK_GUESS_DENSITY needs guess_density");
@@ -323,6 +420,13 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
         // initial density
         if (opt.guess == HFK_GUESS_CORE) {
             HFK_TRY(diagonal_hf(W, W.H, occ, Dflat));
+        } else if (opt.guess == HFK_GUESS_REFERENCE) {
+            std::vector<double> G;
+            HFK_TRY(reference_harris_g(c, occ, G));
+            HFK_HIP(hipMemcpyAsync(Fflat, G.data(), sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+            HFK_HIP(hipStreamSynchronize(c->stream));   // G is pageable and local
+            hfk_launch_axpby((long long)nf, 1.0, W.H, 1.0, Fflat, c->stream);   // fockGuess = fcore + guess
+            HFK_TRY(diagonal_hf(W, Fflat, occ, Dflat));
         } else if (opt.guess == HFK_GUESS_DENSITY) {
             HFK_HIP(hipMemcpyAsync(Dflat, opt.guess_density, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
         } else {

@@ -28,6 +28,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (HFK_BOYS_ACCURATE, HFK_BOYS_REFERENCE, HFK_GUESS_CORE, HFK_GUESS_DENSITY, HFK_GUESS_INTENDED,
+                   HFK_GUESS_REFERENCE,
                    HFK_SCREEN_NONE, HFK_SCREEN_REFERENCE, HfkError, check)
 from .basis import Molecule, build_molecule
 from . import guess as _guess
@@ -35,7 +36,7 @@ from . import guess as _guess
 __all__ = [
     "Context", "HFData", "HfkError", "Molecule", "build_molecule", "options",
     "HFK_BOYS_ACCURATE", "HFK_BOYS_REFERENCE", "HFK_SCREEN_NONE", "HFK_SCREEN_REFERENCE",
-    "HFK_GUESS_CORE", "HFK_GUESS_DENSITY", "HFK_GUESS_INTENDED", "flat_len", "eri_flat_len",
+    "HFK_GUESS_CORE", "HFK_GUESS_DENSITY", "HFK_GUESS_INTENDED", "HFK_GUESS_REFERENCE", "flat_len", "eri_flat_len",
 ]
 
 # HFData (GlobalTypes.hs:122-128) plus run statistics

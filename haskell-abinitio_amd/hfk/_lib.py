@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(HERE, "libhfk.so")
 
 HFK_SCREEN_REFERENCE, HFK_SCREEN_NONE = 0, 1
 HFK_BOYS_REFERENCE, HFK_BOYS_ACCURATE = 0, 1
-HFK_GUESS_CORE, HFK_GUESS_INTENDED, HFK_GUESS_DENSITY = 0, 1, 2
+HFK_GUESS_CORE, HFK_GUESS_INTENDED, HFK_GUESS_DENSITY, HFK_GUESS_REFERENCE = 0, 1, 2, 3
 
 # every symbol include/hfk.h declares
 EXPORTS = (

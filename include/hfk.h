@@ -129,6 +129,10 @@ int hfk_hcore(hfk_ctx *ctx, double *H_flat);
 #define HFK_GUESS_DENSITY 2    /* new: start from hfk_scf_options.guess_density        */
 #define HFK_GUESS_INTENDED 1   /* Harris step with D = 2 on the first occ diagonal AOs,
                                   correctly indexed (HartreeFock.hs:128-149 intent)     */
+#define HFK_GUESS_REFERENCE 3  /* harrisFunctional as written (HartreeFock.hs:127-149):
+                                  flat length as the matrix dimension, occupied as the
+                                  basis size; HFK_EINVAL where the reference's bounds-
+                                  checked indexing aborts (from (H2O)_8 up)             */
 
 typedef struct hfk_scf_options {
     int32_t guess;             /* HFK_GUESS_*                                         */

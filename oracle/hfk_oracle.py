@@ -290,8 +290,62 @@ def gauss_elem(A, b):
     return x
 
 
+# --- the reference's index maps exactly as written (LinearAlgebra.hs:194-283),
+# including indices outside the matrix, as the reference's Harris guess uses them
+def ref_calc_elem(dim, w):
+    """sum (takeZero w (iterate pred dim)): takeZero 0 = [0]; take of a negative count = []."""
+    return sum(dim - t for t in range(w)) if w > 0 else 0
+
+
+def ref_dim2_to_flat(dim, x, y):
+    """indexDIM2toFlat (LinearAlgebra.hs:250-253)."""
+    return ref_calc_elem(dim, x) + (y - x) if x <= y else ref_calc_elem(dim, y) + (x - y)
+
+
+def ref_flat_to_dim2(dim, idx):
+    """indexFlat2DIM2 (LinearAlgebra.hs:240-246)."""
+    x, n = idx, 0
+    while not x < dim - n:
+        x, n = x - (dim - n), n + 1
+    return n, idx + n - ref_calc_elem(dim, n)
+
+
+def ref_four_index_to_flat(n, q):
+    """fourIndex2Flat with totalFirstIndex/sumFirstIndex/selectPosition (LinearAlgebra.hs:255-283)."""
+    tri = lambda m: (m * m + m) // 2
+    first = lambda m: m + tri(m - 1) + sum(tri(m) - k for k in range(1, m))
+    i, j, k, l = q
+    a0 = sum(first(n - m) for m in range(i))
+    if i == j == k:
+        pos = l - i
+    elif i == j:
+        pos = (n - i) + ref_dim2_to_flat(n - (i + 1), k - (i + 1), l - (i + 1))
+    else:
+        pos = (n - i) + tri(n - 1 - i) + sum(tri(n - i) - m for m in range(1, j - i)) + \
+            (ref_dim2_to_flat(n, k, l) - ref_dim2_to_flat(n, i, j))
+    return a0 + pos
+
+
+def reference_harris_g(eri, nflat, occ):
+    """computeGuessIntegrals as written (HartreeFock.hs:137-149): the flat length
+    as the matrix dimension and occ as the basis size.  Raises IndexError where
+    the reference's bounds-checked `!` aborts."""
+    G = np.zeros(nflat)
+    for idx in range(nflat):
+        i, j = ref_flat_to_dim2(nflat, idx)
+        s = 0.0
+        for k in range(occ):
+            f1 = ref_four_index_to_flat(occ, sort_keys(i, j, k, k))
+            f2 = ref_four_index_to_flat(occ, sort_keys(i, k, k, j))
+            if not (0 <= f1 < len(eri) and 0 <= f2 < len(eri)):
+                raise IndexError("reference Harris guess indexes the integral array at %d / %d" % (f1, f2))
+            s += eri[f1] - 0.5 * eri[f2]
+        G[idx] = 2.0 * s
+    return G
+
+
 def scf(mol, fock_G, H_flat, S_flat, guess="core", max_steps=30, ndiis=5, converge=1e-7,
-        diis_start=0.1):
+        diis_start=0.1, eri=None):
     """scfHF/scfDIIS.  fock_G(D_flat) -> G_flat.  Returns dict(energy, steps, C, eps, D, F)."""
     n = mol.nbf
     occ = mol.occupied
@@ -300,6 +354,8 @@ def scf(mol, fock_G, H_flat, S_flat, guess="core", max_steps=30, ndiis=5, conver
     X = symm_ortho(S)
     if guess == "core":
         _, D, _ = diagonal_hf(H_flat, X, occ)
+    elif guess == "reference":   # harrisFunctional (HartreeFock.hs:127-134) as written
+        _, D, _ = diagonal_hf(H_flat + reference_harris_g(eri, len(H_flat), occ), X, occ)
     else:
         D0 = np.zeros(n * (n + 1) // 2)
         for i in range(occ):

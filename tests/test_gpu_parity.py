@@ -104,3 +104,22 @@ def test_water_dimer_scf(require_gpu):
         r = ctx.scfHF()
     assert abs(r.getEnergy - ref["energy"]) < 1e-8
     assert r.steps == ref["steps"]
+
+
+def test_water_sto3g_scf_reference_harris_guess(require_gpu, water):
+    """scfHF from harrisFunctional as written (HFK_GUESS_REFERENCE) against
+    the oracle driver with the same guess: same energy, same step count.  For
+    (H2O)_8 the guess reads outside the integral array, where the reference
+    aborts: an error, no result."""
+    print("case: water sto-3g SCF, reference Harris guess", flush=True)
+    ref_eri, _ = O.eri_flat(water)
+    ref = O.scf(water, lambda D: O.gmatrix(water.nbf, ref_eri, D), O.hcore(water), O.overlap(water),
+                guess="reference", eri=ref_eri)
+    with hfk.Context(water) as ctx:
+        r = ctx.scfHF(guess=hfk.HFK_GUESS_REFERENCE)
+    print("  E gpu %.12f oracle %.12f; steps %d / %d" % (r.getEnergy, ref["energy"], r.steps, ref["steps"]), flush=True)
+    assert abs(r.getEnergy - ref["energy"]) < 1e-8
+    assert r.steps == ref["steps"]
+    with hfk.Context(systems.water_cluster(8, "6-31g")) as ctx:
+        with pytest.raises(hfk.HfkError):
+            ctx.scfHF(guess=hfk.HFK_GUESS_REFERENCE)

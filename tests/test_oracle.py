@@ -240,3 +240,38 @@ def test_gauss_elem_reference_example():
     q1 = [[4.0, -2.0, 1.0], [-2.0, 4.0, -2.0], [1.0, -2.0, 4.0]]
     x = O.gauss_elem(q1, [11.0, -16.0, 17.0])
     np.testing.assert_allclose(x, [1, -2, 3], atol=1e-14)
+
+
+def test_reference_index_maps_on_valid_indices():
+    """The literal restatements of indexFlat2DIM2 / indexDIM2toFlat /
+    fourIndex2Flat used by the reference Harris guess agree, on indices inside
+    the matrix, with the C restatement and the calcIntegrals enumeration."""
+    for dim in range(1, 9):
+        for idx in range(dim * (dim + 1) // 2):
+            i, j = O.ref_flat_to_dim2(dim, idx)
+            assert 0 <= i <= j < dim and O.ref_dim2_to_flat(dim, i, j) == idx
+    for n in range(1, 6):
+        r = 0
+        for i in range(n):
+            for j in range(i, n):
+                for k in range(n):
+                    for l in range(k, n):
+                        if (i, j) > (k, l):
+                            continue
+                        q = [i, j, k, l]
+                        assert O.ref_four_index_to_flat(n, q) == O.four_index_ref(n, i, j, k, l) == O.flat_rank(n, i, j, k, l)
+
+
+def test_reference_harris_guess_index_range():
+    """computeGuessIntegrals as written: in range for water/STO-3G (n=7,
+    occ=5), outside the integral array for (H2O)_8/6-31G (n=104, occ=40),
+    where the reference's bounds-checked `!` aborts."""
+    n, occ = 7, 5
+    M = n * (n + 1) // 2
+    eri = np.arange(M * (M + 1) // 2, dtype=float)
+    G = O.reference_harris_g(eri, M, occ)
+    assert G.shape == (M,) and np.all(np.isfinite(G))
+    n, occ = 104, 40
+    M = n * (n + 1) // 2
+    with pytest.raises(IndexError):
+        O.reference_harris_g(np.zeros(M * (M + 1) // 2), M, occ)
