@@ -63,7 +63,7 @@ def cpu_baseline(mol, target_s=12.0):
                       "contracted4Centers, reference screen + Boys table, OpenMP)" % (count, m * (m + 1) // 2, dt)}
 
 
-def mixed_vs_fp64(workload, thr):
+def mixed_vs_fp64(workload, thr, sweep=(1e-6, 1e-5, 1e-3, 1e-2)):
     """SCF wall time FP64 vs mixed (FP32 for shell quartets whose Schwarz
     bound is below thr) on the same molecule; |dE|, orbital-energy error and
     one Fock build each."""
@@ -96,6 +96,26 @@ def mixed_vs_fp64(workload, thr):
     res["max_dEps"] = float(np.max(np.abs(runs["mixed"].getOrbE - runs["fp64"].getOrbE)))
     res["fock_speedup"] = res["fp64"]["fock_ms"] / res["mixed"]["fock_ms"]
     res["scf_speedup"] = res["fp64"]["scf_s"] / res["mixed"]["scf_s"]
+    # threshold sweep: error and speed of the mixed SCF against FP64
+    res["sweep"] = []
+    for t in sweep:
+        with hfk.Context(mol, hfk.options(fp32_threshold=t)) as ctx:
+            try:
+                r = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60)
+            except hfk.HfkError as e:
+                res["sweep"].append({"fp32_threshold": t, "error": str(e)})
+                log("bench: %s sweep thr %g: %s" % (workload, t, e))
+                continue
+            ctx.calcGmatrix(r.getDensity)
+            fock_ms = ctx.stats()["last_eri_ms"]
+            cs = ctx.class_stats()
+        q32 = sum(c["shell_quartets"] for c in cs if c["prec"] == 1)
+        q64 = sum(c["shell_quartets"] for c in cs if c["prec"] == 0)
+        res["sweep"].append({"fp32_threshold": t, "steps": r.steps, "dE": r.getEnergy - runs["fp64"].getEnergy,
+                             "max_dEps": float(np.max(np.abs(r.getOrbE - runs["fp64"].getOrbE))),
+                             "fock_ms": fock_ms, "fp32_quartet_fraction": q32 / max(1, q32 + q64)})
+        log("bench: %s sweep thr %g dE %.2e fock %.2fms fp32 %.1f%%" % (workload, t, res["sweep"][-1]["dE"], fock_ms,
+                                                                        100.0 * q32 / max(1, q32 + q64)))
     return res
 
 
