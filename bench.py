@@ -245,10 +245,18 @@ def main():
         try:
             r = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60)
             scf = {"energy": r.getEnergy, "steps": r.steps, "wall_s": r.seconds + ctx_s, "scf_s": r.seconds,
+                   "max_error": r.max_error, "last_step_dE": r.delta_energy,
+                   "converged_by": "reference rule max|e| < 1e-7 (both rules: see energy_rule)",
                    "fock_s": r.fock_seconds, "precision": "fp64", "guess": "core",
                    "diis": "6 vectors from step 1 (reference switch-on at 0.1 never triggers here)"}
             D = r.getDensity
             log("bench: scf E=%.10f steps=%d %.3fs" % (r.getEnergy, r.steps, r.seconds))
+            # the same SCF stopped by both rules: max|e| < 1e-7 and |dE| < 1e-8 Eh
+            re = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60, energy_converge=1e-8)
+            scf["energy_rule"] = {"rule": "max|e| < 1e-7 and |E_k - E_k-1| < 1e-8 Eh", "energy": re.getEnergy,
+                                  "steps": re.steps, "scf_s": re.seconds, "last_step_dE": re.delta_energy,
+                                  "dE_vs_reference_rule": re.getEnergy - r.getEnergy}
+            log("bench: scf (energy rule) E=%.10f steps=%d dE=%.2e" % (re.getEnergy, re.steps, re.delta_energy))
         except hfk.HfkError as e:
             scf = {"error": str(e)}
             log("bench: scf failed: %s" % e)

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+#include <string.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -197,6 +198,7 @@ extern "C" void hfk_default_scf_options(hfk_scf_options* o)
     o->incremental_tau = 0.0;   // the reference rebuilds F from D every step
     o->rebuild_every = 16;
     o->pad2 = 0;
+    o->energy_converge = 0.0;   // the reference tests max|e| only
     o->converge = 1.0e-7;   // HartreeFock.hs:113
     o->charge = 0.0;
 }
@@ -358,6 +360,7 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
                        double* D_out, double* F_out)
 {
     if (!c || !res) return hfk_set_error(HFK_EINVAL, "hfk_scf: null argument");
+    memset(res, 0, sizeof *res);
     if (c->Z.empty()) return hfk_set_error(HFK_EINVAL, "hfk_scf: basis has no nuclei");
     hfk_scf_options opt;
     if (o) opt = *o;
@@ -536,6 +539,7 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
             hfk_launch_axpby((long long)nn, 1.0, W.T2, 1.0, W.T1, c->stream);   // H + F
             hfk_launch_reduce((long long)nn, Dfull, W.T1, 0, W.part, W.scal + 2, c->stream);
             const double etot = 0.5 * read_scalar(c, W.scal + 2) + vnn;
+            res->delta_energy = (step == 0) ? INFINITY : etot - res->energy;
             res->energy = etot;
             res->e_nuc = vnn;
             res->max_error = maxe;
@@ -545,7 +549,8 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
                 fprintf(stderr, "hfk_scf: step %d E %.10f max|e| %.3e%s\n", step + 1, etot, maxe, diis_on ? " diis" : "");
                 fflush(stderr);
             }
-            if (step != 0 && maxe < opt.converge) {
+            const bool energy_ok = opt.energy_converge <= 0.0 || std::fabs(res->delta_energy) < opt.energy_converge;
+            if (step != 0 && maxe < opt.converge && energy_ok) {
                 if (F_out) HFK_HIP(hipMemcpyAsync(F_out, Fuse, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
                 if (D_out) HFK_HIP(hipMemcpyAsync(D_out, Dflat, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
                 if (C_out) HFK_HIP(hipMemcpyAsync(C_out, W.C, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));

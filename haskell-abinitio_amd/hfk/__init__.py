@@ -40,7 +40,8 @@ __all__ = [
 ]
 
 # HFData (GlobalTypes.hs:122-128) plus run statistics
-HFData = namedtuple("HFData", "getFock getCoeff getDensity getOrbE getEnergy steps max_error seconds fock_seconds")
+HFData = namedtuple("HFData", "getFock getCoeff getDensity getOrbE getEnergy steps max_error seconds fock_seconds "
+                               "delta_energy")
 
 _dp = C.POINTER(C.c_double)
 
@@ -169,14 +170,16 @@ class Context:
         return H
 
     def scfHF(self, charge=None, guess=HFK_GUESS_CORE, max_steps=30, diis_size=5, converge=1e-7, diis_start=0.1,
-              verbose=False, guess_density=None, incremental_tau=0.0, rebuild_every=16):
+              verbose=False, guess_density=None, incremental_tau=0.0, rebuild_every=16, energy_converge=0.0):
         """scfHF.  Defaults are the reference's (30 steps, 5 DIIS vectors,
         DIIS switched on once max|e| < 0.1, converged at max|e| < 1e-7);
         diis_start=float('inf') switches DIIS on from the second step.
         guess_density: flat starting density (overrides guess).
         incremental_tau > 0: Fock builds from the density change, skipping
         shell quartets with Q_ab Q_cd max|dD| < tau, with a full build every
-        rebuild_every steps (the reference rebuilds every step)."""
+        rebuild_every steps (the reference rebuilds every step).
+        energy_converge > 0: stop only when the last step's energy change is
+        also below it (the reference tests max|e| alone)."""
         o = _lib.HfkScfOptions()
         _lib.lib().hfk_default_scf_options(C.byref(o))
         o.guess = guess
@@ -193,6 +196,7 @@ class Context:
         o.verbose = 1 if verbose else 0
         o.incremental_tau = float(incremental_tau)
         o.rebuild_every = int(rebuild_every)
+        o.energy_converge = float(energy_converge)
         o.charge = self.mol.charge if charge is None else float(charge)
         r = _lib.HfkScfResult()
         n = self.n
@@ -201,7 +205,7 @@ class Context:
         D = np.zeros(flat_len(n))
         F = np.zeros(flat_len(n))
         check(_lib.lib().hfk_scf(self._h, C.byref(o), C.byref(r), _ptr(Cm), _ptr(eps), _ptr(D), _ptr(F)))
-        return HFData(F, Cm, D, eps, r.energy, r.steps, r.max_error, r.seconds, r.fock_seconds)
+        return HFData(F, Cm, D, eps, r.energy, r.steps, r.max_error, r.seconds, r.fock_seconds, r.delta_energy)
 
     def fragment_guess(self, **scf_kw):
         """Block-diagonal starting density from SCF runs on each bonded

@@ -45,6 +45,7 @@ class HfkScfOptions(C.Structure):
         ("diis_start", C.c_double), ("converge", C.c_double), ("charge", C.c_double),
         ("verbose", C.c_int32), ("pad", C.c_int32), ("guess_density", C.POINTER(C.c_double)),
         ("incremental_tau", C.c_double), ("rebuild_every", C.c_int32), ("pad2", C.c_int32),
+        ("energy_converge", C.c_double),
     ]
 
 
@@ -52,7 +53,7 @@ class HfkScfResult(C.Structure):
     _fields_ = [
         ("energy", C.c_double), ("e_nuc", C.c_double), ("max_error", C.c_double),
         ("steps", C.c_int32), ("occupied", C.c_int32), ("seconds", C.c_double),
-        ("fock_seconds", C.c_double),
+        ("fock_seconds", C.c_double), ("delta_energy", C.c_double),
     ]
 
 

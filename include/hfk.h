@@ -150,6 +150,8 @@ typedef struct hfk_scf_options {
     int32_t rebuild_every;     /* new: with incremental_tau, a full build every this
                                   many steps (<= 1: every step)                        */
     int32_t pad2;
+    double energy_converge;    /* new: 0 = the reference's rule alone; > 0: stop only
+                                  when also |E_k - E_{k-1}| < energy_converge          */
 } hfk_scf_options;
 
 typedef struct hfk_scf_result {
@@ -160,6 +162,9 @@ typedef struct hfk_scf_result {
     int32_t occupied;
     double seconds;            /* wall time of the SCF call                             */
     double fock_seconds;       /* time in Fock builds                                   */
+    double delta_energy;       /* new: E_k - E_{k-1} of the last step k (+inf after a
+                                  single step); for an energy-change criterion beside
+                                  the reference's max|e| rule                          */
 } hfk_scf_result;
 
 void hfk_default_scf_options(hfk_scf_options *o);

@@ -63,6 +63,34 @@ def test_water_sto3g_scf(require_gpu, water):
     assert np.max(np.abs(np.sort(r.getOrbE) - np.sort(ref["eps"]))) < 1e-6
 
 
+def test_scf_delta_energy_is_last_step_change(require_gpu, water):
+    """delta_energy after k steps equals E_k - E_{k-1}, read from the result
+    of runs stopped after k-1 and k steps (converge = 0 never converges, so
+    hfk_scf returns HFK_ESCF with the last step's figures filled in)."""
+    _say("case: water sto-3g scf delta_energy")
+    import ctypes as C
+    from hfk import _lib
+
+    def run(steps):
+        o = _lib.HfkScfOptions()
+        _lib.lib().hfk_default_scf_options(C.byref(o))
+        o.max_steps = steps
+        o.converge = 0.0
+        r = _lib.HfkScfResult()
+        rc = _lib.lib().hfk_scf(ctx._h, C.byref(o), C.byref(r), None, None, None, None)
+        assert rc != 0 and r.steps == steps
+        return r.energy, r.delta_energy
+
+    with hfk.Context(water) as ctx:
+        e1, d1 = run(1)
+        e2, d2 = run(2)
+        e3, d3 = run(3)
+    assert d1 == float("inf")
+    assert abs(d2 - (e2 - e1)) < 1e-12
+    assert abs(d3 - (e3 - e2)) < 1e-12
+    assert abs(d3) > 0.0
+
+
 def test_water_dimer_631g_eri_flat(require_gpu):
     _say("case: (H2O)2 6-31g eri_flat")
     mol = systems.water_cluster(2, "6-31g")
@@ -104,6 +132,24 @@ def test_water_dimer_scf(require_gpu):
         r = ctx.scfHF()
     assert abs(r.getEnergy - ref["energy"]) < 1e-8
     assert r.steps == ref["steps"]
+
+
+def test_scf_energy_converge_rule(require_gpu):
+    """energy_converge adds |E_k - E_k-1| < bound to the max|e| rule: the run
+    stops no earlier than the reference rule alone, its last step meets the
+    bound, and its energy is within the bound of a tightly converged one."""
+    _say("case: (H2O)2 6-31g scf energy_converge")
+    mol = systems.water_cluster(2, "6-31g")
+    kw = dict(diis_start=float("inf"), diis_size=6, max_steps=60)
+    with hfk.Context(mol) as ctx:
+        r0 = ctx.scfHF(**kw)
+        r1 = ctx.scfHF(energy_converge=1e-8, **kw)
+        rt = ctx.scfHF(converge=1e-10, **kw)
+    print("  steps %d / %d / %d; dE %.2e / %.2e" % (r0.steps, r1.steps, rt.steps, r0.delta_energy,
+                                                     r1.delta_energy), flush=True)
+    assert r1.steps >= r0.steps
+    assert abs(r1.delta_energy) < 1e-8 and r1.max_error < 1e-7
+    assert abs(r1.getEnergy - rt.getEnergy) < 1e-8
 
 
 def test_water_sto3g_scf_reference_harris_guess(require_gpu, water):
