@@ -63,6 +63,50 @@ def cpu_baseline(mol, target_s=12.0):
                       "contracted4Centers, reference screen + Boys table, OpenMP)" % (count, m * (m + 1) // 2, dt)}
 
 
+def cpu_extra(sample_s=4.0):
+    """The oracle beyond the quartet-rate sample: end-to-end SCF (ERIs, then
+    the DIIS loop on the flat ERI array, the reference's calcIntegrals + fock
+    scheme) for water/STO-3G and (H2O)_8/6-31G, and, for (H2O)_32 and _64,
+    the time to evaluate every canonical quartet once (one integral-direct
+    Fock build) extrapolated from a uniform sample."""
+    import hfk_oracle as O
+    from hfk import systems
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.boys_table()
+    out = {"cores": threads, "kind": "port", "end_to_end": [], "fock_build_extrapolated": []}
+    for name, mol, kw in (("water STO-3G, reference SCF settings", systems.water("sto-3g"), {}),
+                          ("(H2O)_8 6-31G, DIIS 6 vectors from step 1", systems.water_cluster(8, "6-31g"),
+                           dict(ndiis=6, diis_start=float("inf"), max_steps=60))):
+        t0 = time.perf_counter()
+        S, H = O.overlap(mol), O.hcore(mol)
+        flat, _kept = O.eri_flat(mol)
+        t1 = time.perf_counter()
+        r = O.scf(mol, lambda D: O.gmatrix(mol.nbf, flat, D), H, S, **kw)
+        t2 = time.perf_counter()
+        out["end_to_end"].append({"workload": name, "energy": r["energy"], "steps": r["steps"],
+                                  "eri_s": t1 - t0, "scf_loop_s": t2 - t1, "total_s": t2 - t0})
+        log("bench: cpu e2e %s %.2fs" % (name, t2 - t0))
+    for nw in (32, 64):
+        mol = systems.water_cluster(nw, "6-31g")
+        m = mol.nbf * (mol.nbf + 1) // 2
+        total = m * (m + 1) // 2
+        probe = O.random_canonical_quartets(mol.nbf, 2000, seed=1)
+        t = time.perf_counter()
+        O.eri_list(mol, probe)
+        rate0 = len(probe) / max(time.perf_counter() - t, 1e-6)
+        count = int(min(5e7, max(2e4, rate0 * sample_s)))
+        sample = O.random_canonical_quartets(mol.nbf, count, seed=2)
+        t = time.perf_counter()
+        O.eri_list(mol, sample)
+        dt = time.perf_counter() - t
+        out["fock_build_extrapolated"].append({
+            "workload": "(H2O)_%d 6-31G" % nw, "canonical_quartets": total, "sample": count, "sample_s": dt,
+            "quartets_per_s": count / dt, "fock_build_s": total * dt / count,
+            "label": "extrapolated from a uniform sample of canonical quartets"})
+        log("bench: cpu extrapolated (H2O)_%d build %.0fs" % (nw, total * dt / count))
+    return out
+
+
 def mixed_vs_fp64(workload, thr, sweep=(1e-6, 1e-5, 1e-3, 1e-2)):
     """SCF wall time FP64 vs mixed (FP32 for shell quartets whose Schwarz
     bound is below thr) on the same molecule; |dE|, orbital-energy error and
@@ -208,6 +252,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="h2o8", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-extra", action="store_true",
+                    help="also time the oracle end to end (water, (H2O)_8) and extrapolate (H2O)_32/_64 builds")
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-workload", default="h2o32", choices=sorted(WORKLOADS))
@@ -357,6 +403,8 @@ def main():
     if not args.no_cpu:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(mol)
+        if args.cpu_extra:
+            out["cpu_baseline"]["extra"] = cpu_extra()
     print(json.dumps(out), flush=True)
 
 

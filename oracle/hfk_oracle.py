@@ -175,7 +175,10 @@ def eri_list(mol, quartets, exact=False, screen=True):
 
 
 def random_canonical_quartets(n, count, seed=0):
-    """Uniform sample of canonical quartets i<=j, k<=l, (i,j)<=(k,l)."""
+    """Sample, with replacement, of canonical quartets i<=j, k<=l,
+    (i,j)<=(k,l): two pair indices drawn uniformly and ordered, so quartets
+    with (i,j) == (k,l) (a fraction 1/M of the list) have half the weight of
+    the others."""
     rng = np.random.default_rng(seed)
     iu, ju = np.triu_indices(n)
     m = iu.size
