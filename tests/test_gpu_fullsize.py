@@ -98,3 +98,48 @@ def test_cluster_gmatrix_linear_in_density(require_gpu, nwat):
     err = float(np.max(np.abs(g12 - (g1 - 2.0 * g2))))
     _say("  max|G(D1-2D2) - G(D1) + 2G(D2)| = %.2e (scale %.2e)" % (err, scale))
     assert err < 1e-12 * scale
+
+
+def test_h2o32_gradient_fixed_density_fd(require_gpu):
+    """hfk_gradient at (H2O)_32 size against central differences of
+    E(R) = sum D H(R) + 1/2 sum D G(D; R) - sum W S(R) + V_nn(R) at a fixed
+    non-converged D and W, on four coordinates, and translation invariance
+    over all 96 atoms.  Accurate Boys function (the reference's table is
+    piecewise, so its energy is not differentiable across grid cells) and
+    the reference's primitive screen at 1e-20 (a primitive quartet that
+    crosses it moves E by < 1e-20)."""
+    _say("case: (H2O)32 6-31g fixed-density gradient vs finite differences")
+    opts = dict(screen_mode=hfk.HFK_SCREEN_REFERENCE, screen_tau=1e-20, boys_mode=hfk.HFK_BOYS_ACCURATE)
+    mol = systems.water_cluster(32, "6-31g")
+    n = mol.nbf
+    rng = np.random.default_rng(32)
+    D = O.full_to_flat(_density(n, 5) + np.eye(n) * 0.5)
+    W = O.full_to_flat(_density(n, 6))
+    with hfk.Context(mol, hfk.options(**opts)) as ctx:
+        g = ctx.gradient(D, W)
+
+    def energy(R):
+        m = mol.with_coords(R)
+        with hfk.Context(m, hfk.options(**opts)) as c:
+            H = O.flat_to_full(c.hcore(), n)
+            G = O.flat_to_full(c.calcGmatrix(D), n)
+            S = O.flat_to_full(c.mtxOverlap(), n)
+        Df, Wf = O.flat_to_full(D, n), O.flat_to_full(W, n)
+        return float((Df * H).sum() + 0.5 * (Df * G).sum() - (Wf * S).sum() + m.nuclear_repulsion())
+
+    h = 1e-4
+    coords = [(0, 0), (mol.natom - 1, 2)] + [(int(a), int(k)) for a, k in
+                                               zip(rng.integers(1, mol.natom - 1, 2), rng.integers(0, 3, 2))]
+    scale = max(1.0, float(np.max(np.abs(g))))
+    for a, k in coords:
+        R = mol.R.copy()
+        R[a, k] += h
+        ep = energy(R)
+        R[a, k] -= 2 * h
+        em = energy(R)
+        fd = (ep - em) / (2 * h)
+        _say("  atom %d axis %d: analytic % .10f fd % .10f |d| %.2e" % (a, k, g[a, k], fd, abs(g[a, k] - fd)))
+        assert abs(g[a, k] - fd) < 2e-7 * scale
+    resid = float(np.max(np.abs(g.sum(axis=0))))
+    _say("  max|sum_A g_A| = %.2e (max|g| %.2e)" % (resid, scale))
+    assert resid < 1e-9 * scale
