@@ -35,24 +35,24 @@ void hfk_linalg_destroy(hfk_ctx* c)
 
 // Symmetric eigen-solve (EigenValues.hs:29-31 + VectorTools.hs:27-31):
 // eigenvalues ascending; on return row k of A is the k-th eigenvector.
-int hfk_dev_syev(hfk_ctx* c, int n, double* A, double* w)
+// E (n doubles) and info (one int) are device workspace.  With info_host
+// (pinned) the status is copied there asynchronously and the caller checks
+// it after its next stream synchronisation; without, this call waits.
+int hfk_dev_syev(hfk_ctx* c, int n, double* A, double* w, double* E, int* info, int* info_host)
 {
     HFK_TRY(hfk_linalg_init(c));
-    double* work = nullptr;
-    int* info = nullptr;
-    HFK_TRY(dev_alloc(&work, (size_t)n));
-    HFK_TRY(dev_alloc(&info, 1));
     // A is symmetric, so row-major == column-major on input; dsyevd leaves
     // eigenvectors in columns (column-major) == rows in our row-major view.
     rocblas_status st = rocsolver_dsyevd((rocblas_handle)c->blas, rocblas_evect_original, rocblas_fill_upper, n, A, n,
-                                         w, work, info);
-    int hinfo = 0;
-    hipError_t e = hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    dev_free(c, work);
-    dev_free(c, info);
+                                         w, E, info);
     if (st != rocblas_status_success) return hfk_set_error(HFK_EHIP, "rocsolver_dsyevd failed (%d)", (int)st);
-    if (e != hipSuccess) return hfk_set_error(HFK_EHIP, "syev info copy: %s", hipGetErrorString(e));
+    if (info_host) {
+        HFK_HIP(hipMemcpyAsync(info_host, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        return HFK_OK;
+    }
+    int hinfo = 0;
+    HFK_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HFK_HIP(hipStreamSynchronize(c->stream));
     if (hinfo != 0) return hfk_set_error(HFK_ENUMERIC, "rocsolver_dsyevd info=%d", hinfo);
     return HFK_OK;
 }
@@ -66,7 +66,16 @@ struct Work {
     double *T1 = nullptr, *T2 = nullptr, *T3 = nullptr, *Ev = nullptr, *w = nullptr;
     double *Ffull = nullptr, *C = nullptr;
     double *part = nullptr, *scal = nullptr;
+    double* ev_e = nullptr;    // eigen-solve workspace (n)
+    int* ev_info = nullptr;
+    // pinned host mirror of scal, and the status of the last eigen-solve,
+    // copied asynchronously and checked after the next synchronisation
+    double* hscal = nullptr;
+    int* hinfo = nullptr;
 };
+
+constexpr int SCAL_LEN = 64;
+constexpr int SCAL_DOTS = 8;   // scal[SCAL_DOTS + k]: trace(e_cur^T e_k) for DIIS slot k
 
 int work_alloc(hfk_ctx* c, Work& W)
 {
@@ -84,15 +93,37 @@ int work_alloc(hfk_ctx* c, Work& W)
     HFK_TRY(dev_alloc(&W.Ffull, nn));
     HFK_TRY(dev_alloc(&W.C, nn));
     HFK_TRY(dev_alloc(&W.part, 1024));
-    HFK_TRY(dev_alloc(&W.scal, 64));
+    HFK_TRY(dev_alloc(&W.scal, SCAL_LEN));
+    HFK_TRY(dev_alloc(&W.ev_e, (size_t)c->n));
+    HFK_TRY(dev_alloc(&W.ev_info, 1));
+    HFK_HIP(hipHostMalloc((void**)&W.hscal, sizeof(double) * SCAL_LEN, hipHostMallocDefault));
+    HFK_HIP(hipHostMalloc((void**)&W.hinfo, sizeof(int), hipHostMallocDefault));
+    *W.hinfo = 0;
     return HFK_OK;
 }
 
 void work_free(Work& W)
 {
     hfk_ctx* c = W.c;
-    double* ps[] = {W.S, W.X, W.H, W.T1, W.T2, W.T3, W.Ev, W.w, W.Ffull, W.C, W.part, W.scal};
+    hipStreamSynchronize(c->stream);
+    double* ps[] = {W.S, W.X, W.H, W.T1, W.T2, W.T3, W.Ev, W.w, W.Ffull, W.C, W.part, W.scal, W.ev_e};
     for (double* p : ps) dev_free(c, p);
+    dev_free(c, W.ev_info);
+    if (W.hscal) hipHostFree(W.hscal);
+    if (W.hinfo) hipHostFree(W.hinfo);
+    W.hscal = nullptr;
+    W.hinfo = nullptr;
+}
+
+// Wait for the stream and copy scal[first, first+count) to W.hscal; also
+// checks the status of the eigen-solves queued since the last call.
+int sync_read(Work& W, int first, int count)
+{
+    hfk_ctx* c = W.c;
+    HFK_HIP(hipMemcpyAsync(W.hscal + first, W.scal + first, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
+    HFK_HIP(hipStreamSynchronize(c->stream));
+    if (*W.hinfo != 0) return hfk_set_error(HFK_ENUMERIC, "rocsolver_dsyevd info=%d", *W.hinfo);
+    return HFK_OK;
 }
 
 void gemm(hfk_ctx* c, int m, int n, int k, double alpha, const double* A, int lda, int ta, const double* B, int ldb,
@@ -107,7 +138,7 @@ int symm_ortho_dev(Work& W, const double* S, double* X)
     hfk_ctx* c = W.c;
     const int n = W.n;
     HFK_HIP(hipMemcpyAsync(W.Ev, S, sizeof(double) * n * n, hipMemcpyDeviceToDevice, c->stream));
-    HFK_TRY(hfk_dev_syev(c, n, W.Ev, W.w));
+    HFK_TRY(hfk_dev_syev(c, n, W.Ev, W.w, W.ev_e, W.ev_info, nullptr));
     // rows of Ev are eigenvectors u_k; X = sum_k u_k s_k^-1/2 u_k^T = Ev^T diag(s^-1/2) Ev
     HFK_HIP(hipMemcpyAsync(W.T1, W.Ev, sizeof(double) * n * n, hipMemcpyDeviceToDevice, c->stream));
     hfk_launch_scale_cols(n, W.T1, W.w, 0 | 4, c->stream);   // scale row k by s_k^-1/2
@@ -138,20 +169,12 @@ int diagonal_hf(Work& W, const double* Fflat, int occ, double* Dflat)
     const int n = W.n;
     hfk_launch_flat_to_full(n, Fflat, W.Ffull, c->stream);
     unitary_dev(W, W.X, W.Ffull, W.Ev);
-    HFK_TRY(hfk_dev_syev(c, n, W.Ev, W.w));
+    HFK_TRY(hfk_dev_syev(c, n, W.Ev, W.w, W.ev_e, W.ev_info, W.hinfo));
     gemm(c, n, n, n, 1.0, W.X, n, 0, W.Ev, n, 1, 0.0, W.C, n);   // C = X C', C' columns = eigenvectors
     density_dev(W, W.C, occ, W.T1);
     hfk_launch_full_to_flat(n, W.T1, Dflat, c->stream);
     HFK_HIP(hipGetLastError());
     return HFK_OK;
-}
-
-double read_scalar(hfk_ctx* c, const double* d)
-{
-    double v = 0.0;
-    hipMemcpyAsync(&v, d, sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    hipStreamSynchronize(c->stream);
-    return v;
 }
 
 // gaussElem (NumericalTools/GaussElimination.hs:42-112): elimination without
@@ -380,6 +403,8 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
     if (opt.guess == HFK_GUESS_DENSITY && !opt.guess_density)
         return hfk_set_error(HFK_EINVAL, "hfk_scf: HFK_GUESS_DENSITY needs guess_density");
     if (!(opt.incremental_tau >= 0.0)) return hfk_set_error(HFK_EINVAL, "hfk_scf: incremental_tau must be >= 0");
+    if (opt.diis_size < 1 || opt.diis_size > SCAL_LEN - SCAL_DOTS)
+        return hfk_set_error(HFK_EINVAL, "hfk_scf: diis_size %d outside [1, %d]", opt.diis_size, SCAL_LEN - SCAL_DOTS);
     // nuclearRep (HartreeFock.hs:300-305)
     double vnn = 0.0;
     for (int a = 0; a < c->natom; ++a)
@@ -415,6 +440,7 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
     }
     std::vector<double*> free_e(errs), free_f(focks);
     std::deque<int> hist;   // indices into errs/focks, newest first
+    std::vector<double> Bdot((size_t)std::max(opt.diis_size, 1) * std::max(opt.diis_size, 1), 0.0);
     auto run = [&]() -> int {
         HFK_TRY(hfk_ctx_hcore_dev(c, W.H));
         HFK_TRY(hfk_ctx_overlap_dev(c, c->d_flatA));
@@ -452,7 +478,8 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
                 HFK_HIP(hipMemcpyAsync(dD, Dflat, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream));
                 hfk_launch_axpby((long long)nf, -1.0, Dprev, 1.0, dD, c->stream);
                 hfk_launch_reduce((long long)nf, dD, nullptr, 1, W.part, W.scal + 3, c->stream);
-                const double dmax = read_scalar(c, W.scal + 3);
+                HFK_TRY(sync_read(W, 3, 1));
+                const double dmax = W.hscal[3];
                 const double cut = dmax > 0.0 ? opt.incremental_tau / dmax : INFINITY;
                 HFK_TRY(hfk_ctx_fock_device(c, Fflat, dD, Fflat, cut));
                 ++since_full;
@@ -482,7 +509,16 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
             gemm(c, n, n, n, 1.0, W.X, n, 1, W.T2, n, 0, 0.0, ecur, n);
             HFK_HIP(hipMemcpyAsync(focks[cur], Fflat, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream));
             hfk_launch_reduce((long long)nn, ecur, nullptr, 1, W.part, W.scal, c->stream);
-            const double maxe = read_scalar(c, W.scal);
+            // the DIIS matrix entries of the current error with itself and the
+            // stored ones (trace(e_i^T e_j)); entries between stored errors were
+            // computed when the newer of the two was current.  One wait for all.
+            hfk_launch_reduce((long long)nn, ecur, ecur, 0, W.part, W.scal + SCAL_DOTS + cur, c->stream);
+            for (int h : hist) hfk_launch_reduce((long long)nn, ecur, errs[h], 0, W.part, W.scal + SCAL_DOTS + h, c->stream);
+            HFK_TRY(sync_read(W, 0, SCAL_DOTS + opt.diis_size));
+            const double maxe = W.hscal[0];
+            Bdot[(size_t)cur * opt.diis_size + cur] = W.hscal[SCAL_DOTS + cur];
+            for (int h : hist)
+                Bdot[(size_t)cur * opt.diis_size + h] = Bdot[(size_t)h * opt.diis_size + cur] = W.hscal[SCAL_DOTS + h];
             if (std::isnan(maxe)) return hfk_set_error(HFK_ENUMERIC, "hfk_scf: NaN in the DIIS error at step %d", step);
             // diisDriver (DIIS.hs:55-65)
             const double* Fuse = Fflat;
@@ -507,9 +543,7 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
                     std::vector<double> B((size_t)(m + 1) * (m + 1), 0.0), rhs(m + 1, 0.0);
                     for (int i = 0; i < m; ++i)
                         for (int j = 0; j <= i; ++j) {
-                            hfk_launch_reduce((long long)nn, errs[idx[i]], errs[idx[j]], 0, W.part, W.scal + 1,
-                                              c->stream);
-                            const double v = read_scalar(c, W.scal + 1);   // trace(e_i^T e_j)
+                            const double v = Bdot[(size_t)idx[i] * opt.diis_size + idx[j]];   // trace(e_i^T e_j)
                             B[(size_t)i * (m + 1) + j] = v;
                             B[(size_t)j * (m + 1) + i] = v;
                         }
@@ -538,7 +572,8 @@ extern "C" int hfk_scf(hfk_ctx* c, const hfk_scf_options* o, hfk_scf_result* res
             hfk_launch_flat_to_full(n, W.H, W.T2, c->stream);
             hfk_launch_axpby((long long)nn, 1.0, W.T2, 1.0, W.T1, c->stream);   // H + F
             hfk_launch_reduce((long long)nn, Dfull, W.T1, 0, W.part, W.scal + 2, c->stream);
-            const double etot = 0.5 * read_scalar(c, W.scal + 2) + vnn;
+            HFK_TRY(sync_read(W, 2, 1));
+            const double etot = 0.5 * W.hscal[2] + vnn;
             res->delta_energy = (step == 0) ? INFINITY : etot - res->energy;
             res->energy = etot;
             res->e_nuc = vnn;
