@@ -289,8 +289,17 @@ def main():
     D = None
     if not args.no_scf:
         try:
+            # first SCF of the process (pays the HIP/rocSOLVER one-time start-up),
+            # then the same SCF again on a new context: the figures reported
+            r0 = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60)
+            cold_s = r0.seconds + ctx_s
+            ctx.close()
+            t_ctx = time.perf_counter()
+            ctx = hfk.Context(mol)
+            ctx_s = time.perf_counter() - t_ctx
             r = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60)
             scf = {"energy": r.getEnergy, "steps": r.steps, "wall_s": r.seconds + ctx_s, "scf_s": r.seconds,
+                   "ctx_s": ctx_s, "first_in_process_wall_s": cold_s,
                    "max_error": r.max_error, "last_step_dE": r.delta_energy,
                    "converged_by": "reference rule max|e| < 1e-7 (both rules: see energy_rule)",
                    "fock_s": r.fock_seconds, "precision": "fp64", "guess": "core",

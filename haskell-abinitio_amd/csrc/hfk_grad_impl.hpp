@@ -14,8 +14,18 @@
 
 namespace {
 
+// Waves per SIMD the register allocation must leave room for.  The total of
+// VGPRs and AGPRs sets occupancy (512 per lane per SIMD); (ps|ps) and (pp|ss)
+// need a little over 256, which alone would leave one wave per SIMD.
+template <int LA, int LB, int LC, int LD>
+constexpr int grad_min_waves()
+{
+    return (LA + LB + LC + LD == 2) ? 2 : 1;
+}
+
 template <int LA, int LB, int LC, int LD, int BOYS, int MODE>
-__global__ void __launch_bounds__(256) k_grad_tiles(QuartetArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_min_waves<LA, LB, LC, LD>())))
+k_grad_tiles(QuartetArgs a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NKET = NC * ND, NBRA = NA * NB;
