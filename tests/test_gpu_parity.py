@@ -169,3 +169,28 @@ def test_water_sto3g_scf_reference_harris_guess(require_gpu, water):
     with hfk.Context(systems.water_cluster(8, "6-31g")) as ctx:
         with pytest.raises(hfk.HfkError):
             ctx.scfHF(guess=hfk.HFK_GUESS_REFERENCE)
+
+
+def test_textbook_h2_and_heh_through_the_abi(require_gpu):
+    """Szabo & Ostlund, STO-3G, exact Boys function and no screen (the
+    textbook values are for the exact integrals): H2 at R = 1.4 bohr,
+    zeta = 1.24 (S12, four ERIs, E = -1.1167) and HeH+ at R = 1.4632 bohr
+    (E = -2.860662; printed to 6 d.p., checked to 5e-6)."""
+    _say("case: textbook H2 and HeH+ STO-3G")
+    exact = hfk.options(screen_mode=hfk.HFK_SCREEN_NONE, boys_mode=hfk.HFK_BOYS_ACCURATE)
+    h2 = systems.h2_szabo()
+    with hfk.Context(h2, exact) as ctx:
+        S = ctx.mtxOverlap()
+        flat = ctx.calcIntegrals()
+        r = ctx.scfHF()
+    g = lambda *ix: O.lib().ob_flat_get(2, O._dp(np.ascontiguousarray(flat)), *ix)
+    assert abs(S[1] - 0.6593) < 5e-5
+    assert abs(g(0, 0, 0, 0) - 0.7746) < 5e-5
+    assert abs(g(0, 0, 1, 1) - 0.5697) < 5e-5
+    assert abs(g(1, 0, 1, 0) - 0.2970) < 5e-5
+    assert abs(g(1, 0, 0, 0) - 0.4441) < 5e-5
+    assert abs(r.getEnergy - (-1.1167)) < 5e-5
+    with hfk.Context(systems.heh_szabo(), exact) as ctx:
+        r2 = ctx.scfHF()
+    _say("  E(H2) %.6f  E(HeH+) %.6f" % (r.getEnergy, r2.getEnergy))
+    assert abs(r2.getEnergy - (-2.860662)) < 5e-6
