@@ -616,9 +616,13 @@ int hfk_ctx_plan(hfk_ctx* c)
         const long long cmerge = std::max<long long>(chunk, std::min<long long>(1024, (merged_kets + target - 1) / target));
         std::vector<int4> tasks;
         std::vector<int2> runs;
+        std::vector<long long> cost;   // estimated primitive quartets per task
+        long long cost_acc = 0;
         auto push = [&](int bp, long long r0, int w, int len) {
             if (r0 > INT32_MAX || (long long)runs.size() > INT32_MAX) return false;
             tasks.push_back(make_int4(bp, (int)r0, (int)runs.size(), w));
+            cost.push_back(cost_acc);
+            cost_acc = 0;
             e.task_len_max = std::max(e.task_len_max, len);
             return true;
         };
@@ -635,6 +639,7 @@ int hfk_ctx_plan(hfk_ctx* c)
                 while (x < x1) {
                     const long long take = std::min<long long>(x1 - x, cmerge - len);
                     runs.push_back(make_int2((int)x, (int)(x + take)));
+                    cost_acc += take * nbp * nkp;
                     x += take;
                     len += (int)take;
                     if (len == cmerge) {
@@ -667,6 +672,7 @@ int hfk_ctx_plan(hfk_ctx* c)
                             const long long j1 = std::min<long long>(k1, j0 + chunk);
                             const long long rr = (long long)runs.size();
                             runs.push_back(make_int2((int)j0, (int)j1));
+                            cost_acc = (j1 - j0) * (long long)(b1 - b0) * (q1 - q0);
                             if (!push(bl[i], rr, w, (int)(j1 - j0))) return hfk_set_error(HFK_EUNSUP, "task list too long");
                         }
                         e.bra_stage = std::max(e.bra_stage, b1 - b0);
@@ -675,6 +681,25 @@ int hfk_ctx_plan(hfk_ctx* c)
             }
         }
         if ((long long)kl.size() > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "pair class too long");
+        // costliest tasks first: workgroups start roughly in index order, so
+        // the launch ends on short tasks and its tail is short.  A stable
+        // bucket sort on the cost's quarter-octave (4 log2 cost, from its two
+        // bits below the leading one), linear in the number of tasks
+        // (millions for the large clusters).
+        {
+            auto bucket = [](long long v) {
+                if (v <= 0) return 0;
+                const int b = 63 - __builtin_clzll((unsigned long long)v);
+                const int sub = b >= 2 ? (int)((v >> (b - 2)) & 3) : (int)((v << (2 - b)) & 3);
+                return 4 * b + sub;   // 0 .. 255
+            };
+            std::vector<size_t> start(257, 0);
+            for (long long v : cost) ++start[255 - bucket(v) + 1];
+            for (int b = 1; b <= 256; ++b) start[b] += start[b - 1];
+            std::vector<int4> sorted(tasks.size());
+            for (size_t t = 0; t < tasks.size(); ++t) sorted[start[255 - bucket(cost[t])]++] = tasks[t];
+            tasks.swap(sorted);
+        }
         e.ntask = (long long)tasks.size();
         e.d_tasks = nullptr;
         e.d_runs = nullptr;
