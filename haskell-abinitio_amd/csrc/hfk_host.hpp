@@ -197,5 +197,5 @@ int hfk_linalg_init(hfk_ctx* c);
 void hfk_linalg_destroy(hfk_ctx* c);
 // symmetric eigen-solve on device: A (n x n row-major, overwritten by
 // eigenvectors as ROWS: A[k][:] = k-th eigenvector), w ascending.
-// E (n) and info (1) are device workspace; info_host (pinned) or NULL: see hfk_scf.hip
+// E (n) and info (2) are device workspace; info_host (pinned) or NULL: see hfk_scf.hip
 int hfk_dev_syev(hfk_ctx* c, int n, double* A, double* w, double* E, int* info, int* info_host);

@@ -35,17 +35,25 @@ void hfk_linalg_destroy(hfk_ctx* c)
 
 // Symmetric eigen-solve (EigenValues.hs:29-31 + VectorTools.hs:27-31):
 // eigenvalues ascending; on return row k of A is the k-th eigenvector.
-// E (n doubles) and info (one int) are device workspace.  With info_host
+// E (n doubles) and info (two ints) are device workspace.  With info_host
 // (pinned) the status is copied there asynchronously and the caller checks
 // it after its next stream synchronisation; without, this call waits.
+// Up to n = 32 (molecular fragments) rocSOLVER's Jacobi solver is the
+// faster one (0.15 against 0.33 ms at n = 13), above it divide and conquer
+// (1.6 against 2.8 ms for Jacobi-D&C at n = 104); both sort ascending.
 int hfk_dev_syev(hfk_ctx* c, int n, double* A, double* w, double* E, int* info, int* info_host)
 {
     HFK_TRY(hfk_linalg_init(c));
-    // A is symmetric, so row-major == column-major on input; dsyevd leaves
+    // A is symmetric, so row-major == column-major on input; the solvers leave
     // eigenvectors in columns (column-major) == rows in our row-major view.
-    rocblas_status st = rocsolver_dsyevd((rocblas_handle)c->blas, rocblas_evect_original, rocblas_fill_upper, n, A, n,
-                                         w, E, info);
-    if (st != rocblas_status_success) return hfk_set_error(HFK_EHIP, "rocsolver_dsyevd failed (%d)", (int)st);
+    rocblas_status st;
+    if (n <= 32)
+        st = rocsolver_dsyevj((rocblas_handle)c->blas, rocblas_esort_ascending, rocblas_evect_original,
+                              rocblas_fill_upper, n, A, n, 0.0 /* machine precision */, E /* residual */, 100,
+                              info + 1 /* sweeps */, w, info);
+    else
+        st = rocsolver_dsyevd((rocblas_handle)c->blas, rocblas_evect_original, rocblas_fill_upper, n, A, n, w, E, info);
+    if (st != rocblas_status_success) return hfk_set_error(HFK_EHIP, "rocsolver eigen-solve failed (%d)", (int)st);
     if (info_host) {
         HFK_HIP(hipMemcpyAsync(info_host, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         return HFK_OK;
@@ -53,7 +61,7 @@ int hfk_dev_syev(hfk_ctx* c, int n, double* A, double* w, double* E, int* info, 
     int hinfo = 0;
     HFK_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HFK_HIP(hipStreamSynchronize(c->stream));
-    if (hinfo != 0) return hfk_set_error(HFK_ENUMERIC, "rocsolver_dsyevd info=%d", hinfo);
+    if (hinfo != 0) return hfk_set_error(HFK_ENUMERIC, "rocsolver eigen-solve info=%d", hinfo);
     return HFK_OK;
 }
 
@@ -95,7 +103,7 @@ int work_alloc(hfk_ctx* c, Work& W)
     HFK_TRY(dev_alloc(&W.part, 1024));
     HFK_TRY(dev_alloc(&W.scal, SCAL_LEN));
     HFK_TRY(dev_alloc(&W.ev_e, (size_t)c->n));
-    HFK_TRY(dev_alloc(&W.ev_info, 1));
+    HFK_TRY(dev_alloc(&W.ev_info, 2));
     HFK_HIP(hipHostMalloc((void**)&W.hscal, sizeof(double) * SCAL_LEN, hipHostMallocDefault));
     HFK_HIP(hipHostMalloc((void**)&W.hinfo, sizeof(int), hipHostMallocDefault));
     *W.hinfo = 0;
@@ -122,7 +130,7 @@ int sync_read(Work& W, int first, int count)
     hfk_ctx* c = W.c;
     HFK_HIP(hipMemcpyAsync(W.hscal + first, W.scal + first, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
     HFK_HIP(hipStreamSynchronize(c->stream));
-    if (*W.hinfo != 0) return hfk_set_error(HFK_ENUMERIC, "rocsolver_dsyevd info=%d", *W.hinfo);
+    if (*W.hinfo != 0) return hfk_set_error(HFK_ENUMERIC, "rocsolver eigen-solve info=%d", *W.hinfo);
     return HFK_OK;
 }
 
