@@ -241,6 +241,7 @@ int hfk_ctx_init_device(hfk_ctx* c)
     }
     HFK_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     if (const char* v = getenv("HFK_CONCURRENT")) c->concurrent = atoi(v) != 0;
+    HFK_HIP(hipHostMalloc((void**)&c->h_pin, sizeof(double) * HFK_PIN_DOUBLES + sizeof(int), hipHostMallocDefault));
     // Boys tables
     std::vector<double> glx(128), glw(128);
     gl256(glx.data(), glw.data());
@@ -341,6 +342,8 @@ void hfk_ctx_free_device(hfk_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void* p : c->allocs) hipFree(p);
     c->allocs.clear();
+    if (c->h_pin) hipHostFree(c->h_pin);
+    c->h_pin = nullptr;
     for (hipEvent_t ev : c->events) hipEventDestroy(ev);
     c->events.clear();
     if (c->ev0) hipEventDestroy(c->ev0);

@@ -89,6 +89,8 @@ struct hfk_ctx {
     bool timed_serial = false;  // the last timed run launched serially
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void* blas = nullptr;                 // rocblas_handle
+    double* h_pin = nullptr;              // pinned host: HFK_PIN_DOUBLES scalars, then one int status
+#define HFK_PIN_DOUBLES 64
     std::vector<void*> allocs;
     double *d_glx = nullptr, *d_glw = nullptr;
     double *d_boys_ref64 = nullptr, *d_boys_acc64 = nullptr;

@@ -76,8 +76,9 @@ struct Work {
     double *part = nullptr, *scal = nullptr;
     double* ev_e = nullptr;    // eigen-solve workspace (n)
     int* ev_info = nullptr;
-    // pinned host mirror of scal, and the status of the last eigen-solve,
-    // copied asynchronously and checked after the next synchronisation
+    // pinned host mirror of scal (the context's buffer), and the status of
+    // the last eigen-solve, copied asynchronously and checked after the next
+    // synchronisation
     double* hscal = nullptr;
     int* hinfo = nullptr;
 };
@@ -104,8 +105,9 @@ int work_alloc(hfk_ctx* c, Work& W)
     HFK_TRY(dev_alloc(&W.scal, SCAL_LEN));
     HFK_TRY(dev_alloc(&W.ev_e, (size_t)c->n));
     HFK_TRY(dev_alloc(&W.ev_info, 2));
-    HFK_HIP(hipHostMalloc((void**)&W.hscal, sizeof(double) * SCAL_LEN, hipHostMallocDefault));
-    HFK_HIP(hipHostMalloc((void**)&W.hinfo, sizeof(int), hipHostMallocDefault));
+    static_assert(SCAL_LEN <= HFK_PIN_DOUBLES, "pinned scalar buffer");
+    W.hscal = c->h_pin;   // the context's (calls on a context are serialised)
+    W.hinfo = (int*)(c->h_pin + HFK_PIN_DOUBLES);
     *W.hinfo = 0;
     return HFK_OK;
 }
@@ -117,8 +119,6 @@ void work_free(Work& W)
     double* ps[] = {W.S, W.X, W.H, W.T1, W.T2, W.T3, W.Ev, W.w, W.Ffull, W.C, W.part, W.scal, W.ev_e};
     for (double* p : ps) dev_free(c, p);
     dev_free(c, W.ev_info);
-    if (W.hscal) hipHostFree(W.hscal);
-    if (W.hinfo) hipHostFree(W.hinfo);
     W.hscal = nullptr;
     W.hinfo = nullptr;
 }
