@@ -101,6 +101,29 @@ def test_water_dimer_631g_eri_flat(require_gpu):
     assert err < 1e-12, err
 
 
+def test_eri_fixtures_water_and_water8(require_gpu):
+    """The flat ERI export against the committed fixtures
+    (tests/golden/eri_golden.json): all water/STO-3G ERIs and 10^4 seeded
+    (H2O)_8/6-31G quartets, to 1e-12."""
+    import json
+    import os
+    _say("case: ERI fixtures water sto-3g, (H2O)8 6-31g sample")
+    eg = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "eri_golden.json")))
+    with hfk.Context(systems.water("sto-3g")) as ctx:
+        got = ctx.calcIntegrals()
+    ref = np.array([float(v) for v in eg["water_sto3g"]["flat"]])
+    assert np.max(np.abs(got - ref)) < 1e-12
+    s = eg["h2o8_631g"]
+    mol = systems.water_cluster(8, "6-31g")
+    with hfk.Context(mol) as ctx:
+        flat = ctx.calcIntegrals()
+    q = np.array(s["quartets"]).reshape(-1, 4)
+    idx = np.array([O.flat_rank(mol.nbf, *map(int, r)) for r in q])
+    err = np.max(np.abs(flat[idx] - np.array([float(v) for v in s["values"]])))
+    _say("  (H2O)8 sample: max|d| %.2e over %d quartets" % (err, len(q)))
+    assert err < 1e-12
+
+
 @pytest.fixture(scope="module")
 def water8():
     mol = systems.water_cluster(8, "6-31g")

@@ -275,3 +275,27 @@ def test_reference_harris_guess_index_range():
     M = n * (n + 1) // 2
     with pytest.raises(IndexError):
         O.reference_harris_g(np.zeros(M * (M + 1) // 2), M, occ)
+
+
+def test_oracle_matches_its_committed_fixtures():
+    """Regression pins (tests/golden/make_eri_golden.py): the Boys table's
+    SHA-256 and sampled nodes; every water/STO-3G ERI and the number of
+    primitive quartets the reference screen keeps; 10^4 seeded (H2O)_8/6-31G
+    ERIs.  Bit for bit: the oracle is deterministic."""
+    import hashlib
+    bt = json.load(open(os.path.join(GOLD, "boys_table.json")))
+    t = np.ascontiguousarray(O.boys_table(), dtype="<f8")
+    assert hashlib.sha256(t.tobytes()).hexdigest() == bt["sha256"]
+    grid = t.reshape(bt["shape"])
+    for m, i, v in bt["samples"]:
+        assert grid[m, i] == float(v)
+    eg = json.load(open(os.path.join(GOLD, "eri_golden.json")))
+    water = systems.water("sto-3g")
+    flat, kept = O.eri_flat(water)
+    assert kept == eg["water_sto3g"]["kept_primitive_quartets"]
+    np.testing.assert_array_equal(flat, np.array([float(v) for v in eg["water_sto3g"]["flat"]]))
+    s = eg["h2o8_631g"]
+    mol = systems.water_cluster(8, "6-31g")
+    q = np.array(s["quartets"], dtype=np.int32).reshape(-1, 4)
+    np.testing.assert_array_equal(q, O.random_canonical_quartets(mol.nbf, len(q), seed=s["seed"]))
+    np.testing.assert_array_equal(O.eri_list(mol, q), np.array([float(v) for v in s["values"]]))
