@@ -217,3 +217,16 @@ def test_textbook_h2_and_heh_through_the_abi(require_gpu):
         r2 = ctx.scfHF()
     _say("  E(H2) %.6f  E(HeH+) %.6f" % (r.getEnergy, r2.getEnergy))
     assert abs(r2.getEnergy - (-2.860662)) < 5e-6
+
+
+def test_too_many_primitives_is_an_error(require_gpu):
+    """A shell pair holds at most 255 primitive pairs (tasks carry 8-bit
+    primitive ranges): 16 x 16 is refused at context creation, 15 x 15 works."""
+    _say("case: primitive-count limit")
+    def h2(nprim):
+        prims = [(1.0, 0.1 * 1.7 ** k) for k in range(nprim)]
+        return systems.custom_molecule([("H", (0, 0, 0), [("S", prims)]), ("H", (0, 0, 1.4), [("S", prims)])])
+    with pytest.raises(hfk.HfkError):
+        hfk.Context(h2(16))
+    with hfk.Context(h2(15)) as ctx:
+        assert np.all(np.isfinite(ctx.calcIntegrals()))
