@@ -602,16 +602,16 @@ int hfk_ctx_plan(hfk_ctx* c)
             for (const auto& sg : segs)
                 est_prim += (long long)c->h_pair_npp[bl[i]] * c->h_pair_npp[kl[sg.first]] * (staircase(i, sg) - sg.first);
         // Primitive ranges: a lane's share of a task is about per_lane primitive
-        // quartets, sized from the launch's total so that small launches still
-        // fill the GPU (~1.3e5 lanes of work; with the costliest-first task
-        // order this beat 2.6e5: fewer, longer pieces and fewer partial
-        // digestions) and large ones keep quartets whole.  Bra primitives are
-        // wave-uniform (LDS broadcast), so a lane keeps as many of them as fit
-        // and the ket range is split.  Segments that need no split join the
-        // long multi-run tasks.
+        // quartets (at most 128), sized from the launch's total so that small
+        // launches still fill the GPU (~1.3e5 lanes of work; with the
+        // costliest-first task order this beat 2.6e5: fewer, longer pieces and
+        // fewer partial digestions) and large ones keep quartets whole.  Bra
+        // primitives are wave-uniform (LDS broadcast), so a lane keeps as many
+        // of them as fit and the ket range is split.  Segments that need no
+        // split join the long multi-run tasks.
         const int L = e.la + e.lb + e.lc + e.ld;
         const long long floor_l = std::max(4, 32 >> L);   // cheap classes: amortise the digestion
-        const int per_lane = (int)std::max(floor_l, std::min(256LL, est_prim / 131072));
+        const int per_lane = (int)std::max(floor_l, std::min(128LL, est_prim / 131072));
         const bool single = coop || hfk_class_grad_coop(e.la, e.lb, e.lc, e.ld);   // cooperative kernels read these
         auto whole = [&](int nbp, int nkp) { return !single && nbp <= HFK_BRA_STAGE && nbp * nkp <= per_lane; };
         long long merged_kets = 0;
