@@ -230,3 +230,15 @@ def test_too_many_primitives_is_an_error(require_gpu):
         hfk.Context(h2(16))
     with hfk.Context(h2(15)) as ctx:
         assert np.all(np.isfinite(ctx.calcIntegrals()))
+
+
+def test_scf_option_limits(require_gpu, water):
+    """diis_size outside [1, 56] and a negative incremental_tau are refused
+    before any work (the DIIS matrix row of the current error is read back
+    with max|e| in one 64-double transfer)."""
+    _say("case: scf option limits")
+    with hfk.Context(water) as ctx:
+        for bad in (dict(diis_size=0), dict(diis_size=57), dict(incremental_tau=-1.0)):
+            with pytest.raises(hfk.HfkError):
+                ctx.scfHF(**bad)
+        assert abs(ctx.scfHF(diis_size=56).getEnergy - ctx.scfHF().getEnergy) < 1e-8
