@@ -68,3 +68,22 @@ def test_gemm_larger(require_gpu):
         S = _spd(n, 6)
         Xs = c.symmOrtho(S)
         np.testing.assert_allclose(Xs.T @ S @ Xs, np.eye(n), atol=1e-11)
+
+
+@pytest.mark.parametrize("system", ["h2", "water", "h2o8", "h2o16"])
+def test_symm_ortho_sizes(require_gpu, system):
+    """symmOrtho on both sides of the eigen-solver switch: rocSOLVER's Jacobi
+    solver up to n = 32 (H2: 2, water/STO-3G: 7) and divide and conquer above
+    ((H2O)_8: 104, (H2O)_16: 208).  X = S^-1/2 is unique, so it is compared
+    with numpy's directly; S = I (one eigenvalue, n-fold degenerate) gives X = I."""
+    mol = {"h2": lambda: systems.h2_szabo(), "water": lambda: systems.water("sto-3g"),
+           "h2o8": lambda: systems.water_cluster(8, "6-31g"), "h2o16": lambda: systems.water_cluster(16, "6-31g")}[system]()
+    with hfk.Context(mol) as c:
+        n = c.n
+        print("case: symmOrtho %s n=%d" % (system, n), flush=True)
+        S = _spd(n, 8) / n
+        X = c.symmOrtho(S)
+        ref = O.symm_ortho(S)
+        assert np.max(np.abs(X - ref)) < 1e-11 * np.max(np.abs(ref))
+        np.testing.assert_allclose(X.T @ S @ X, np.eye(n), atol=1e-11)
+        np.testing.assert_allclose(c.symmOrtho(np.eye(n)), np.eye(n), atol=1e-13)
