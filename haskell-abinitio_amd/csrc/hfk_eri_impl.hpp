@@ -186,12 +186,11 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
     // lanes' atomics fall on consecutive addresses; tasks resident at the same
     // time share ket runs, so workgroups spread them over a few copies
     double* __restrict__ Jk = a.Jk + (size_t)(blockIdx.x % (unsigned)a.jk_copies) * a.nket * NKET;
+    // (lanes take the kets in run order: a wave meets one primitive-pair
+    // count except at run boundaries)
     const int len = task_ket_count(a, task);
-    const int rot = (int)((unsigned)task.x * 37u % (unsigned)(len > 0 ? len : 1));
     for (int it = tid; it < len; it += nthr) {
-        int jj = it + rot;
-        if (jj >= len) jj -= len;
-        const int pos = task_ket_pos(a, task, jj);
+        const int pos = task_ket_pos(a, task, it);
         const int kp = a.ket_idx[pos];
         if (!quartet_selected(a, bp, kp)) continue;
         T Vt[NBRA * NKET];
