@@ -20,12 +20,13 @@ Boys (accurate)             Mamedov reference values (Boys.hs:27-39, Eq(3)),
 primitive ERI (MD)          (ss|ss) closed form; independent Obara-Saika code
                             (tests/os_eri.py) at 1e-13; 8-fold symmetry;
                             translation and rotation covariance
-contracted ERI + screen     the screen is a restatement of
-                            IntegralsEvaluation.hs:305-337; pinned by the
-                            water/STO-3G case where it is active (the O 1s x H
-                            pair, SURVEY analysis) and by counting screened
-                            primitive quartets; parity unpinned beyond that
-                            (the reference cannot be run: no GHC)
+contracted ERI + screen     the screen restates IntegralsEvaluation.hs:
+                            303-337; twoTermsERI against hand-derived per-axis
+                            closed forms (s, p, d pairs; one-centre p-s pairs
+                            are negative, sqrt gives NaN, the quartet is
+                            kept); s-s pairs = 1/4 of the exact (ab|ab); the
+                            water/STO-3G kept count (32 274 of 32 886)
+                            recomputed from the closed forms alone
 flat layout                 literal restatement of fourIndex2Flat
                             (LinearAlgebra.hs:266-283) vs the enumeration of
                             calcIntegrals (HartreeFock.hs:166-181), n <= 12
@@ -278,13 +279,15 @@ def gauss_elem(A, b):
     A = np.array(A, dtype=np.float64)
     b = np.array(b, dtype=np.float64)
     m = A.shape[0]
-    for k in range(m):
-        A2, b2 = A.copy(), b.copy()
-        for i in range(k + 1, m):
-            alpha = A[i, k] / A[k, k]
-            A2[i, k:] = A[i, k:] - alpha * A[k, k:]
-            b2[i] = b[i] - alpha * b[k]
-        A, b = A2, b2
+    # a zero pivot makes the rows non-finite; the check below turns that into None
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for k in range(m):
+            A2, b2 = A.copy(), b.copy()
+            for i in range(k + 1, m):
+                alpha = A[i, k] / A[k, k]
+                A2[i, k:] = A[i, k:] - alpha * A[k, k:]
+                b2[i] = b[i] - alpha * b[k]
+            A, b = A2, b2
     if not all(np.any(np.abs(A[i]) > 1e-6) for i in range(m)) or not np.all(np.isfinite(A)):
         return None
     x = np.zeros(m)

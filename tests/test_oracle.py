@@ -205,6 +205,110 @@ def test_reference_screen_is_active_for_water():
     assert 1e-8 < d < 1e-6
 
 
+def _two_terms_closed_form(A, la, c1, a, B, lb, c2, b):
+    """twoTermsERI (IntegralsEvaluation.hs:318-337) written out by hand:
+    2 c1^2 c2^2 (pi/2p)^(5/2) (sum_tuv E_tuv)(sum_tuv (-1)^(t+u+v) E_tuv).
+    E_tuv = E^x_t E^y_u E^z_v and E^x_0 of (0,0) is exp(-mu X_AB^2), so both
+    sums are exp(-mu AB^2) times products of per-axis sums S+- = sum_t
+    (+-1)^t E^ij_t / E^00_0.  From E^{i+1,j}_t = E^ij_{t-1}/2p + X_PA E^ij_t
+    + (t+1) E^ij_{t+1} (and the same in j with X_PB), with h = 1/2p:
+    (0,0) 1; (1,0) X_PA +- h; (2,0) (X_PA +- h)^2 + h; (1,1)
+    (X_PA +- h)(X_PB +- h) + h; (0,j) as (j,0) with X_PB."""
+    p = a + b
+    h = 0.5 / p
+    P = (a * np.asarray(A) + b * np.asarray(B)) / p
+    prod = 1.0
+    for x in range(3):
+        i, j = la[x], lb[x]
+        xa, xb = P[x] - A[x], P[x] - B[x]
+        if (i, j) == (0, 0):
+            sp = sm = 1.0
+        elif i + j == 1:
+            xx = xa if i else xb
+            sp, sm = xx + h, xx - h
+        elif (i, j) in ((2, 0), (0, 2)):
+            xx = xa if i else xb
+            sp, sm = (xx + h) ** 2 + h, (xx - h) ** 2 + h
+        elif (i, j) == (1, 1):
+            sp, sm = (xa + h) * (xb + h) + h, (xa - h) * (xb - h) + h
+        else:
+            raise ValueError("closed forms cover i + j <= 2 per axis")
+        prod *= sp * sm
+    ab2 = float(np.sum((np.asarray(A) - np.asarray(B)) ** 2))
+    return 2.0 * c1 ** 2 * c2 ** 2 * (math.pi / (2 * p)) ** 2.5 * math.exp(-2.0 * a * b / p * ab2) * prod
+
+
+def _two_terms(A, la, c1, a, B, lb, c2, b):
+    import ctypes as C
+    ia, ib = (np.ascontiguousarray(l, dtype=np.int32) for l in (la, lb))
+    ra, rb = (np.ascontiguousarray(r, dtype=np.float64) for r in (A, B))
+    return O.lib().ob_two_terms(O._dp(ra), ia.ctypes.data_as(C.POINTER(C.c_int)), c1, a,
+                                O._dp(rb), ib.ctypes.data_as(C.POINTER(C.c_int)), c2, b)
+
+
+def test_screen_pair_quantity_closed_forms():
+    """The oracle's twoTermsERI against the hand-derived per-axis closed
+    forms, for s, p and d pairs with i + j <= 2 per axis, on distinct and
+    coincident centres (one-centre p-s pairs come out negative, so the
+    reference's sqrt gives NaN and `not (NaN < 1e-8)` keeps the quartet).
+    For an s-s pair it is a quarter of the exact (ab|ab) of the independent
+    Obara-Saika code: the reference's constant 2 (pi/2p)^(5/2) against the
+    true 2 pi^(5/2) / (p^2 sqrt(2p))."""
+    import os_eri
+    rng = np.random.default_rng(318)
+    shapes = [((0, 0, 0), (0, 0, 0)), ((1, 0, 0), (0, 0, 0)), ((0, 0, 0), (0, 1, 0)), ((0, 0, 1), (0, 0, 1)),
+              ((1, 0, 0), (0, 1, 0)), ((2, 0, 0), (0, 0, 0)), ((0, 0, 0), (0, 2, 0)), ((1, 1, 0), (0, 0, 0)),
+              ((1, 0, 1), (0, 1, 0)), ((0, 1, 1), (1, 0, 0))]
+    negative = 0
+    for la, lb in shapes:
+        for same_centre in (False, True):
+            A = rng.standard_normal(3)
+            B = A.copy() if same_centre else A + rng.standard_normal(3) * 0.8
+            a, b = rng.uniform(0.2, 8.0, 2)
+            c1, c2 = rng.uniform(0.3, 2.0, 2)
+            ref = _two_terms_closed_form(A, la, c1, a, B, lb, c2, b)
+            got = _two_terms(A, la, c1, a, B, lb, c2, b)
+            assert abs(got - ref) <= 1e-13 * max(abs(ref), 1e-300), (la, lb, same_centre, got, ref)
+            negative += ref < 0
+    assert negative > 0
+    A, B = [0.1, -0.3, 0.2], [0.5, 0.4, -0.6]
+    a, b, c1, c2 = 1.3, 0.45, 0.7, 1.1
+    exact = os_eri.prim_eri(A, (0, 0, 0), a, B, (0, 0, 0), b, A, (0, 0, 0), a, B, (0, 0, 0), b)
+    assert abs(_two_terms(A, (0, 0, 0), c1, a, B, (0, 0, 0), c2, b) - 0.25 * c1 ** 2 * c2 ** 2 * exact) < 1e-14
+
+
+def test_screen_keeps_32274_water_quartets_by_closed_forms():
+    """The reference screen's decision, recomputed from the closed forms
+    alone for every primitive quartet of the 406 canonical water/STO-3G
+    quartets: keep unless sqrt(q_ab) sqrt(q_cd) < 1e-8, NaN kept
+    (IntegralsEvaluation.hs:303-316).  Matches the oracle's count."""
+    mol = systems.water("sto-3g")
+    n = mol.nbf
+    po = mol.prim_off
+    q = {}
+    for i in range(n):
+        for j in range(n):
+            for g1 in range(po[i], po[i + 1]):
+                for g2 in range(po[j], po[j + 1]):
+                    v = _two_terms_closed_form(mol.center[i], mol.lxyz[i], mol.coef[g1], mol.expo[g1],
+                                               mol.center[j], mol.lxyz[j], mol.coef[g2], mol.expo[g2])
+                    q[g1, g2] = math.sqrt(v) if v >= 0 else float("nan")
+    kept = nan_kept = 0
+    pairs = [(i, j) for i in range(n) for j in range(i, n)]
+    for x, (i, j) in enumerate(pairs):
+        for (k, l) in pairs[x:]:
+            for g1 in range(po[i], po[i + 1]):
+                for g2 in range(po[j], po[j + 1]):
+                    for g3 in range(po[k], po[k + 1]):
+                        for g4 in range(po[l], po[l + 1]):
+                            val = q[g1, g2] * q[g3, g4]
+                            if not (val < 1e-8):
+                                kept += 1
+                                nan_kept += val != val
+    assert kept == O.eri_flat(mol)[1] == 32274
+    assert nan_kept > 0
+
+
 def test_textbook_h2_and_heh():
     """Szabo & Ostlund, STO-3G: H2 (R=1.4, zeta=1.24) and HeH+ (R=1.4632)."""
     h2 = systems.h2_szabo()
@@ -260,6 +364,30 @@ def test_reference_index_maps_on_valid_indices():
                             continue
                         q = [i, j, k, l]
                         assert O.ref_four_index_to_flat(n, q) == O.four_index_ref(n, i, j, k, l) == O.flat_rank(n, i, j, k, l)
+
+
+def test_reference_index_maps_outside_the_matrix_by_hand():
+    """Values of the reference's maps at the arguments the Harris guess hands
+    them for water/STO-3G (flat length 28 as the matrix dimension, occupied =
+    5 as the basis size), evaluated by hand from LinearAlgebra.hs:194-283:
+
+    indexFlat2DIM2 28 idx: idx < 28 - 0 gives x = 0, y = idx - calcElem 0
+      = idx - sum [0] = idx, so every row index is 0.
+    fourIndex2Flat 5 [0,0,0,27]: a0 = 0; i = j = k, so l - i = 27.
+    fourIndex2Flat 5 [0,0,3,3]: i = j only: (5 - 0) + indexDIM2toFlat 4 2 2
+      = 5 + (4 + 3) = 12.
+    fourIndex2Flat 5 [0,27,3,3]: 5 + sumIndexTriang 4 (= 10)
+      + sum_{m=1..26} (15 - m) (= 39) + (indexDIM2toFlat 5 3 3 (= 12)
+      - indexDIM2toFlat 5 0 27 (= 27)) = 39.
+    fourIndex2Flat 5 [0,3,3,27]: 5 + 10 + (14 + 13) + ((12 + 24) - 3) = 75.
+    fourIndex2Flat 5 [1,1,1,9]: a0 = sumFirstIndex 5 = 5 + 10 + (14 + 13 +
+      12 + 11) = 65; i = j = k, so 65 + (9 - 1) = 73."""
+    assert [O.ref_flat_to_dim2(28, idx) for idx in range(28)] == [(0, idx) for idx in range(28)]
+    for q, want in (([0, 0, 0, 27], 27), ([0, 0, 3, 3], 12), ([0, 27, 3, 3], 39), ([0, 3, 3, 27], 75),
+                    ([1, 1, 1, 9], 73)):
+        assert O.ref_four_index_to_flat(5, q) == want, (q, want)
+    assert O.sort_keys(0, 27, 0, 0) == [0, 0, 0, 27]   # [0,27] > [0,0], so the pairs swap
+    assert O.sort_keys(0, 3, 3, 27) == [0, 3, 3, 27]
 
 
 def test_reference_harris_guess_index_range():
