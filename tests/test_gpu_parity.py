@@ -242,3 +242,36 @@ def test_scf_option_limits(require_gpu, water):
             with pytest.raises(hfk.HfkError):
                 ctx.scfHF(**bad)
         assert abs(ctx.scfHF(diis_size=56).getEnergy - ctx.scfHF().getEnergy) < 1e-8
+
+
+def test_one_function_helium_closed_forms(require_gpu):
+    """The smallest input: He with one STO-3G s function (zeta = 1.6875; nbf =
+    1, occupied = 1, one atom away from the origin).  One orbital fixes the
+    density at D = 2/S, so with H = h11, J = (11|11): G(D) = J/S,
+    E = 2H/S + J/S^2 (no nuclear repulsion), eps = (H + J/S)/S, and a lone
+    atom's gradient is zero.  An empty basis is refused at context creation."""
+    import ctypes as C
+    from hfk import _lib
+    _say("case: He one function, closed forms; empty basis refused")
+    mol = systems.custom_molecule([("He", (0.3, -0.2, 0.1), [("S", systems.sto3g_s(1.6875))])])
+    ref_eri, _ = O.eri_flat(mol)
+    with hfk.Context(mol) as ctx:
+        S = float(ctx.mtxOverlap()[0])
+        H = float(ctx.hcore()[0])
+        flat = ctx.calcIntegrals()
+        J = float(flat[0])
+        G = ctx.calcGmatrix(np.array([2.0 / S]))
+        r = ctx.scfHF()
+        g = ctx.gradient(r.getDensity)
+    _say("  S %.15f H %.12f J %.12f E %.12f steps %d" % (S, H, J, r.getEnergy, r.steps))
+    assert flat.shape == (1,) and abs(J - ref_eri[0]) < 1e-12
+    assert abs(S - O.overlap(mol)[0]) < 1e-13 and abs(H - O.hcore(mol)[0]) < 1e-11
+    assert abs(G[0] - J / S) < 1e-13
+    assert abs(r.getEnergy - (2 * H / S + J / S ** 2)) < 1e-10
+    assert abs(float(np.ravel(r.getOrbE)[0]) - (H + J / S) / S) < 1e-10
+    assert abs(float(np.ravel(r.getDensity)[0]) - 2.0 / S) < 1e-12
+    assert g.shape == (1, 3) and np.max(np.abs(g)) < 1e-12
+    b = _lib.HfkBasis()
+    b.nbf = 0
+    h = C.c_void_p()
+    assert _lib.lib().hfk_create(C.byref(b), None, C.byref(h)) == -1 and not h.value   # HFK_EINVAL
