@@ -32,7 +32,8 @@ def test_default_options_are_reference():
 
 
 def test_flops_model_anchors():
-    # SURVEY 8(d) anchor values of the Head-Gordon-Pople count
+    # Head-Gordon-Pople flop counts: VRR per primitive quartet (f_prim), HRR per
+    # contracted quartet (f_hrr); (ss|ss) 43 and (ps|ss) 72 price the headline kernels
     assert flops.f_prim(0, 0, 0, 0) == 43
     assert flops.f_prim(1, 0, 0, 0) == 72
     assert (flops.f_prim(1, 1, 0, 0), flops.f_hrr(1, 1, 0, 0)) == (153, 18)
