@@ -1,17 +1,19 @@
 """Benchmark: integral-direct RHF Fock build (ERI engine + J/K digestion) on
 one MI355X.
 
-A step is one full (non-incremental) Fock build G(D) of the workload with
-the density already resident in HBM.  The headline value is ERI quartets/s:
-canonical basis-function quartets held by the shell quartets that survive
-the screen, divided by the ERI-kernel wall time of one Fock build (HIP
-events around the class launches on the library's stream), averaged over
-the timed steps.
+Workload (default): (H2O)_64/6-31G, 832 basis functions, the largest of the
+water-cluster configurations.  A step is one full (non-incremental) Fock
+build G(D) with the converged density already resident in HBM.  The
+headline value is ERI quartets/s: canonical basis-function quartets held by
+the shell quartets that survive the screen, divided by the ERI-kernel wall
+time of one Fock build (HIP events around the class launches on the
+library's stream), averaged over the timed steps.
 
-Prints one JSON line (rank 0).  Extra fields carry the SCF wall time, the
-per-class launch breakdown, the roofline of the dominant kernel and the CPU
+Prints one JSON line (rank 0).  Extra fields carry the per-class launch
+breakdown, the roofline of the dominant kernel, the same record for
+(H2O)_8/6-31G, the mixed-precision and gradient records and the CPU
 baseline (the oracle's reference algorithm on the host cores, on a bounded
-sample of the same workload).
+sample of canonical quartets of the same molecule).
 """
 import argparse
 import json
@@ -251,16 +253,114 @@ def target_bench(nwat=64, thr=1e-4, tau=1e-12):
     return res
 
 
+def fock_rate(ctx, D, steps, warmup, workload):
+    """Time `steps` full Fock builds G(D) with D resident in HBM: canonical
+    function quartets of the surviving shell quartets / ERI-kernel time
+    (HIP events on the library's stream around the class launches), the
+    per-class breakdown and the roofline of the dominant launch."""
+    import torch
+    from hfk.flops import launch_flops
+    n = ctx.n
+    nf = n * (n + 1) // 2
+    dD = torch.tensor(D, dtype=torch.float64, device="cuda")
+    dG = torch.empty(nf, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        ctx.fock_dev(None, dD.data_ptr(), dG.data_ptr())
+    # the first statistics request counts each class's surviving quartets once
+    # (one k_count_tasks launch per class); keep that out of the timed steps
+    ctx.stats()
+    torch.cuda.synchronize()
+    eri_ms, per_launch = [], None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.fock_dev(None, dD.data_ptr(), dG.data_ptr())
+        st = ctx.stats()
+        eri_ms.append(st["last_eri_ms"])
+        cs = ctx.class_stats()
+        if per_launch is None:
+            per_launch = [dict(c, ms_sum=0.0) for c in cs]
+        for acc, c in zip(per_launch, cs):
+            acc["ms_sum"] += c["ms"]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = ctx.stats()
+    fq = st["func_quartets"]
+    eri_avg = sum(eri_ms) / len(eri_ms)
+    m = n * (n + 1) // 2
+    classes = []
+    total_flops = 0
+    for c in per_launch:
+        ms = c["ms_sum"] / steps
+        fl = launch_flops(c)
+        total_flops += fl
+        classes.append({"class": "(%s%s|%s%s)" % tuple("spdf"[c[k]] for k in ("la", "lb", "lc", "ld")),
+                        "prec": "fp32" if c["prec"] else "fp64", "ms": round(ms, 4), "solo": bool(c["solo"]),
+                        "tasks": c["tasks"], "shell_quartets": c["shell_quartets"], "prim_quartets": c["prim_quartets"],
+                        "func_quartets": c["func_quartets"], "alg_flops": fl,
+                        "alg_tflops": round(fl / (ms * 1e-3) / 1e12, 3) if ms > 0 else None})
+    # the dominant kernel: the longest launch that had the GPU to itself (the
+    # library runs it first and alone; the other launches overlap each other)
+    dom = max([x for x in classes if x["solo"]] or classes, key=lambda x: x["ms"])
+    achieved = dom["alg_flops"] / (dom["ms"] * 1e-3) / 1e12
+    peak = PEAK_FP32 if dom["prec"] == "fp32" else PEAK_FP64
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # FETCH_SIZE/WRITE_SIZE summary of the same workload (calibrated).
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "jk_tiles_%s_traffic.json" % workload)
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath))["traffic"].get("%s %s" % (dom["class"], dom["prec"]))
+    return {
+        "value": fq / (eri_avg * 1e-3), "eri_ms": eri_avg, "ms_per_step": (t1 - t0) * 1e3 / steps,
+        "canonical_quartets": m * (m + 1) // 2, "surviving_func_quartets": fq, "shell_pairs": st["nshell_pair"],
+        "unscreened_rate": m * (m + 1) // 2 / (eri_avg * 1e-3),
+        "alg_tflops_all_classes": total_flops / (eri_avg * 1e-3) / 1e12,
+        "classes": classes,
+        "roofline": {"bound": "fp64_valu" if dom["prec"] == "fp64" else "fp32_valu", "achieved": achieved,
+                     "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
+                     "kernel": "k_jk_tiles %s %s (ERI + fused J/K digestion)" % (dom["class"], dom["prec"]),
+                     "launch_ms": dom["ms"], "alg_flops_per_launch": dom["alg_flops"],
+                     "note": "vector-ALU bound: MD ERI arithmetic per primitive quartet; peak is the MI355X "
+                             "FP64 vector figure; achieved = Head-Gordon-Pople flops of the launch's surviving "
+                             "quartets / its HIP-event time"},
+    }
+
+
+def headline_density(mol, big):
+    """Converged density of the workload: from the fragment guess with
+    incremental builds for the large clusters, else from the core guess."""
+    import hfk
+    t0 = time.perf_counter()
+    with hfk.Context(mol) as ctx:
+        ctx_s = time.perf_counter() - t0
+        if big:
+            t1 = time.perf_counter()
+            Dg = ctx.fragment_guess()
+            guess_s = time.perf_counter() - t1
+            r = ctx.scfHF(guess_density=Dg, diis_start=float("inf"), diis_size=8, max_steps=100,
+                          incremental_tau=1e-12)
+            scf = {"guess": "fragment", "guess_s": guess_s, "incremental_tau": 1e-12}
+        else:
+            r = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60)
+            scf = {"guess": "core"}
+    scf.update({"energy": r.getEnergy, "steps": r.steps, "scf_s": r.seconds, "ctx_s": ctx_s,
+                "max_error": r.max_error, "last_step_dE": _finite(r.delta_energy), "fock_s": r.fock_seconds,
+                "precision": "fp64"})
+    log("bench: scf E=%.10f steps=%d %.3fs" % (r.getEnergy, r.steps, r.seconds))
+    return r.getDensity, scf
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="h2o8", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="h2o64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-extra", action="store_true",
                     help="also time the oracle end to end (water, (H2O)_8) and extrapolate (H2O)_32/_64 builds")
-    ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--no-scf", action="store_true", help="time the builds on a random density (no SCF first)")
+    ap.add_argument("--no-small", action="store_true", help="skip the (H2O)_8 Fock-build record")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-workload", default="h2o32", choices=sorted(WORKLOADS))
     ap.add_argument("--mixed-thr", type=float, default=1e-4)
@@ -280,7 +380,6 @@ def main():
     import torch
     import hfk
     from hfk import systems
-    from hfk.flops import launch_flops
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -288,36 +387,10 @@ def main():
     mol = systems.water_cluster(nwat, "6-31g")
     n = mol.nbf
     log("bench: workload %s nbf=%d" % (args.workload, n))
-    t_ctx = time.perf_counter()
-    ctx = hfk.Context(mol)
-    ctx_s = time.perf_counter() - t_ctx
-    scf = None
-    D = None
+    scf, D = None, None
     if not args.no_scf:
         try:
-            # first SCF of the process (pays the HIP/rocSOLVER one-time start-up),
-            # then the same SCF again on a new context: the figures reported
-            r0 = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60)
-            cold_s = r0.seconds + ctx_s
-            ctx.close()
-            t_ctx = time.perf_counter()
-            ctx = hfk.Context(mol)
-            ctx_s = time.perf_counter() - t_ctx
-            r = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60)
-            scf = {"energy": r.getEnergy, "steps": r.steps, "wall_s": r.seconds + ctx_s, "scf_s": r.seconds,
-                   "ctx_s": ctx_s, "first_in_process_wall_s": cold_s,
-                   "max_error": r.max_error, "last_step_dE": _finite(r.delta_energy),
-                   "converged_by": "reference rule max|e| < 1e-7 (both rules: see energy_rule)",
-                   "fock_s": r.fock_seconds, "precision": "fp64", "guess": "core",
-                   "diis": "6 vectors from step 1 (reference switch-on at 0.1 never triggers here)"}
-            D = r.getDensity
-            log("bench: scf E=%.10f steps=%d %.3fs" % (r.getEnergy, r.steps, r.seconds))
-            # the same SCF stopped by both rules: max|e| < 1e-7 and |dE| < 1e-8 Eh
-            r_e = ctx.scfHF(diis_start=float("inf"), diis_size=6, max_steps=60, energy_converge=1e-8)
-            scf["energy_rule"] = {"rule": "max|e| < 1e-7 and |E_k - E_k-1| < 1e-8 Eh", "energy": r_e.getEnergy,
-                                  "steps": r_e.steps, "scf_s": r_e.seconds, "last_step_dE": _finite(r_e.delta_energy),
-                                  "dE_vs_reference_rule": r_e.getEnergy - r.getEnergy}
-            log("bench: scf (energy rule) E=%.10f steps=%d dE=%.2e" % (r_e.getEnergy, r_e.steps, r_e.delta_energy))
+            D, scf = headline_density(mol, nwat >= 32)
         except hfk.HfkError as e:
             scf = {"error": str(e)}
             log("bench: scf failed: %s" % e)
@@ -332,86 +405,47 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    nf = n * (n + 1) // 2
-    dD = torch.tensor(D, dtype=torch.float64, device="cuda")
-    dG = torch.empty(nf, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        ctx.fock_dev(None, dD.data_ptr(), dG.data_ptr())
-    # the first statistics request counts each class's surviving quartets once
-    # (one k_count_tasks launch per class); keep that out of the timed steps
-    ctx.stats()
-    torch.cuda.synchronize()
-    eri_ms, per_launch = [], None
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.fock_dev(None, dD.data_ptr(), dG.data_ptr())
-        st = ctx.stats()
-        eri_ms.append(st["last_eri_ms"])
-        cs = ctx.class_stats()
-        if per_launch is None:
-            per_launch = [dict(c, ms_sum=0.0) for c in cs]
-        for acc, c in zip(per_launch, cs):
-            acc["ms_sum"] += c["ms"]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    st = ctx.stats()
-    fq = st["func_quartets"]
-    eri_avg = sum(eri_ms) / len(eri_ms)
-    value = fq / (eri_avg * 1e-3)
-    m = n * (n + 1) // 2
-    classes = []
-    total_flops = 0
-    for c in per_launch:
-        ms = c["ms_sum"] / args.steps
-        fl = launch_flops(c)
-        total_flops += fl
-        classes.append({"class": "(%s%s|%s%s)" % tuple("spdf"[c[k]] for k in ("la", "lb", "lc", "ld")),
-                        "prec": "fp32" if c["prec"] else "fp64", "ms": round(ms, 4), "solo": bool(c["solo"]),
-                        "shell_quartets": c["shell_quartets"], "prim_quartets": c["prim_quartets"],
-                        "func_quartets": c["func_quartets"], "alg_tflops": round(fl / (ms * 1e-3) / 1e12, 3) if ms > 0 else None})
-    # the dominant kernel: the longest launch that had the GPU to itself (the
-    # library runs it first and alone; the other launches overlap each other)
-    dom = max([x for x in classes if x["solo"]] or classes, key=lambda x: x["ms"])
-    dom_flops = [launch_flops(c) for c in per_launch if
-                 "(%s%s|%s%s)" % tuple("spdf"[c[k]] for k in ("la", "lb", "lc", "ld")) == dom["class"]
-                 and ("fp32" if c["prec"] else "fp64") == dom["prec"]][0]
-    achieved = dom_flops / (dom["ms"] * 1e-3) / 1e12
-    peak = PEAK_FP32 if dom["prec"] == "fp32" else PEAK_FP64
-    # HBM bytes per launch of the dominant kernel, from the committed
-    # FETCH_SIZE/WRITE_SIZE summary of the same workload (calibrated).
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "jk_tiles_%s_traffic.json" % args.workload)
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath))["traffic"].get("%s %s" % (dom["class"], dom["prec"]))
+    t_ctx = time.perf_counter()
+    ctx = hfk.Context(mol)
+    ctx_s = time.perf_counter() - t_ctx
+    rec = fock_rate(ctx, D, args.steps, args.warmup, args.workload)
+    ctx.close()
+    log("bench: %s Fock build %.2f ms, %.3e quartets/s, dominant %s %.1f%% of FP64 peak"
+        % (args.workload, rec["eri_ms"], rec["value"], rec["roofline"]["kernel"], 100 * rec["roofline"]["frac"]))
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": rec["value"],
         "unit": "quartets/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": rec["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded water cluster geometry; converged SCF density of it)",
-        "config": {"workload": desc, "nbf": n, "screen": "reference per-primitive 1e-8",
-                   "boys": "reference table", "canonical_quartets": m * (m + 1) // 2,
-                   "surviving_func_quartets": fq, "shell_pairs": st["nshell_pair"]},
-        "eri_ms": eri_avg,
-        "unscreened_rate": m * (m + 1) // 2 / (eri_avg * 1e-3),
-        "alg_tflops_all_classes": total_flops / (eri_avg * 1e-3) / 1e12,
-        "classes": classes,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic,
-                     "kernel": "k_jk_tiles %s %s (ERI + fused J/K)" % (dom["class"], dom["prec"]),
-                     "note": "FP64 vector-ALU kernel; peak is MI355X FP64 (vector = matrix figure)"},
+        "config": {"workload": desc + "; one full integral-direct Fock build G(D) per step, all FP64",
+                   "nbf": n, "screen": "reference per-primitive 1e-8", "boys": "reference table",
+                   "canonical_quartets": rec["canonical_quartets"],
+                   "surviving_func_quartets": rec["surviving_func_quartets"], "shell_pairs": rec["shell_pairs"]},
+        "eri_ms": rec["eri_ms"],
+        "ctx_s": ctx_s,
+        "unscreened_rate": rec["unscreened_rate"],
+        "alg_tflops_all_classes": rec["alg_tflops_all_classes"],
+        "classes": rec["classes"],
+        "roofline": rec["roofline"],
         "scf": scf,
     }
-    ctx.close()
+    if args.workload != "h2o8" and not args.no_small:
+        m8 = systems.water_cluster(8, "6-31g")
+        D8, scf8 = headline_density(m8, False)
+        with hfk.Context(m8) as c8:
+            r8 = fock_rate(c8, D8, 20, 3, "h2o8")
+        out["h2o8"] = {"workload": WORKLOADS["h2o8"][1], "value": r8["value"], "eri_ms": r8["eri_ms"],
+                       "ms_per_step": r8["ms_per_step"], "roofline": r8["roofline"], "classes": r8["classes"],
+                       "scf": scf8}
+        log("bench: h2o8 Fock build %.3f ms, %.3e quartets/s" % (r8["eri_ms"], r8["value"]))
     if not args.no_mixed:
         out["mixed_vs_fp64"] = mixed_vs_fp64(args.mixed_workload, args.mixed_thr)
     if not args.no_grad:

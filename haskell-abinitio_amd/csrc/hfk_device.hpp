@@ -56,7 +56,7 @@ __device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 // F_m(x) for m = 0..L.
 //  BOYS == 0: the reference's 6-term Taylor step from the 0.1-spaced node
 //             table, asymptotic formula evaluated at the node above x_k = 100
-//             (TableBoys.hs:45-51,65-69; Boys.hs:155-157).
+//             (TableBoys.hs:45-51,65-69; Boys.hs:85-87).
 //  BOYS == 1: accurate: 8-term Taylor about the nearest 0.1 node (|dx| <= 0.05)
 //             of an accurate table for x < 50; F_0 = sqrt(pi/x)/2 and upward
 //             recursion above.

@@ -96,7 +96,7 @@ void gl256(double* x, double* w)
     }
 }
 
-// facOdd (Boys.hs:159-164) with Int (64-bit wrapping) factorials.
+// facOdd (Boys.hs:89-94) with Int (64-bit wrapping) factorials (fac, Boys.hs:97-100).
 double hs_facodd(int i)
 {
     if (i < 2) return 1.0;
@@ -261,7 +261,7 @@ int hfk_ctx_init_device(hfk_ctx* c)
     c->bt.acc64 = c->d_boys_acc64;
     c->bt.acc32 = c->d_boys_acc32;
     for (int m = 0; m < 24; ++m) {
-        // asymptBoysF m x = (facOdd (floor (2m-1)) / 2^(m+1)) * sqrt (pi / x^(2m+1))  (Boys.hs:155-157)
+        // asymptBoysF m x = (facOdd (floor (2m-1)) / 2^(m+1)) * sqrt (pi / x^(2m+1))  (Boys.hs:85-87)
         c->bt.asym[m] = hs_facodd((int)floor(2.0 * m - 1.0)) / pow(2.0, m + 1.0);
     }
     // raw basis on device (one-electron integrals)
@@ -377,9 +377,10 @@ int hfk_ctx_build_pairs(hfk_ctx* c)
             pB.push_back(B);
             pp0.push_back((int)npp_tot);
             const int k = c->shells[A].nprim * c->shells[B].nprim;
-            // tasks carry primitive-pair ranges in 8-bit fields
-            if (k > 255)
-                return hfk_set_error(HFK_EUNSUP, "shell pair with %d primitive pairs (at most 255: %d x %d primitives)", k,
+            // tasks carry primitive-pair ranges in 8-bit fields, where 255
+            // marks "all of the ket's pairs": at most 254 per pair
+            if (k > 254)
+                return hfk_set_error(HFK_EUNSUP, "shell pair with %d primitive pairs (at most 254: %d x %d primitives)", k,
                                      c->shells[A].nprim, c->shells[B].nprim);
             npp.push_back(k);
             q0.push_back((int)nq_tot);

@@ -14,8 +14,8 @@ namespace {
 // Boys tables
 // ------------------------------------------------------------------
 
-// Reference node: boysF m x (NumericalTools/Boys.hs:124-136) with kummer
-// by 256-point Gauss-Legendre (Boys.hs:176-188, 233-234).  glx/glw are the
+// Reference node: boysF m x (NumericalTools/Boys.hs:54-66) with kummer
+// by 256-point Gauss-Legendre (Boys.hs:106-118, 155-157, 163-165).  glx/glw are the
 // 128 positive nodes/weights in ascending order.
 __device__ double boys_ref_node(int m, double x, const double* glx, const double* glw)
 {

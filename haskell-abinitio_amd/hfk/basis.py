@@ -207,12 +207,28 @@ def build_molecule(symbols, coords_bohr, basis_name, charge=0.0, d_order=SHELL_F
 
 
 def parse_xyz(text):
-    """parserXYZ (ParsecTools/ParseXYZ.hs:33-43): Angstrom -> bohr (Units.hs:13)."""
-    lines = [ln for ln in text.splitlines() if ln.strip()]
-    n = int(lines[0].split()[0])
+    """parserXYZ (ParsecTools/ParseXYZ.hs:33-43): Angstrom -> bohr (Units.hs:13).
+
+    As in the reference: leading whitespace, the atom count, the rest of
+    that line and then exactly one more line (the comment, which may be
+    empty) are consumed; then n atoms follow, each a symbol of ASCII letters
+    and three numbers, with any whitespace (blank lines included) between
+    them (ParseUtils.hs:41-57).  Fewer than n atoms raise ValueError."""
+    s = text.lstrip()
+    head, _, rest = s.partition("\n")
+    tok = head.split()
+    if not tok or not tok[0].isdigit():
+        raise ValueError("parse_xyz: no atom count on the first line")
+    n = int(tok[0])
+    _comment, _, body = rest.partition("\n")
+    t = body.split()
+    if len(t) < 4 * n:
+        raise ValueError("parse_xyz: %d atoms declared, %d found" % (n, len(t) // 4))
     syms, xyz = [], []
-    for ln in lines[2:2 + n]:
-        t = ln.split()
-        syms.append(t[0])
-        xyz.append([float(v) / 0.529177 for v in t[1:4]])
+    for k in range(n):
+        sym = t[4 * k]
+        if not (sym.isascii() and sym.isalpha()):
+            raise ValueError("parse_xyz: atom %d has symbol %r" % (k, sym))
+        syms.append(sym)
+        xyz.append([float(v) / 0.529177 for v in t[4 * k + 1:4 * k + 4]])
     return syms, xyz

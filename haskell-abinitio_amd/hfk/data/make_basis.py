@@ -7,7 +7,7 @@ Run in a container that has the reference tree mounted:
 The text files are EMSL Basis Set Exchange exports (data/basis_text/*.txt in
 the reference).  Only the elements the workloads use are kept.  SP blocks are
 parsed with their real exponent (the reference's cereal binaries carry the
-exponent 1.0 for every SP primitive, ParsecTools/ParserBasis.hs:214-216, so
+exponent 1.0 for every SP primitive, ParsecTools/ParserBasis.hs:116-118, so
 they are not used).
 """
 import json

@@ -107,7 +107,7 @@ void ob_gl256(double *x, double *w)
     memcpy(w, gl_w, sizeof gl_w);
 }
 
-/* fac with Int (64-bit, wrapping) arithmetic: Boys.hs:167-170 */
+/* fac with Int (64-bit, wrapping) arithmetic: Boys.hs:97-100 */
 static int64_t hs_fac(int i)
 {
     uint64_t acc = 1;
@@ -115,7 +115,7 @@ static int64_t hs_fac(int i)
     return (int64_t)acc;
 }
 
-/* facOdd: Boys.hs:159-164 */
+/* facOdd: Boys.hs:89-94 */
 static double hs_facodd(int i)
 {
     if (i < 2) return 1.0;             /* compare i 2 == LT (odd i only) */
@@ -123,15 +123,15 @@ static double hs_facodd(int i)
     return (double)hs_fac(2 * k) / (hs_powi(2.0, k) * (double)hs_fac(k));
 }
 
-/* asymptBoysF: Boys.hs:155-157 */
+/* asymptBoysF: Boys.hs:85-87 */
 double ob_boys_asympt(double m, double x)
 {
     double fn = hs_facodd((int)floor(2.0 * m - 1.0));
     return (fn / pow(2.0, m + 1.0)) * sqrt(M_PI / pow(x, 2.0 * m + 1.0));
 }
 
-/* boysF: Boys.hs:124-136 with kummer Boys.hs:176-188 (err=0.1 selects
- * nIntegrate256, Boys.hs:233-234) */
+/* boysF: Boys.hs:54-66 with kummer Boys.hs:106-118 (err=0.1 selects
+ * nIntegrate256, Boys.hs:155-157,163-165) */
 double ob_boys_node(double m, double x)
 {
     if (m == 0.0) {

@@ -412,3 +412,44 @@ def scf(mol, fock_G, H_flat, S_flat, guess="core", max_steps=30, ndiis=5, conver
             return dict(energy=etot, steps=step + 1, C=Cm, eps=eps, D=Dn, F=Fuse, max_error=maxe)
         D = Dn
     raise RuntimeError("SCF maxium steps exceeded")
+
+
+def variational_energy(mol, D_flat, H_flat, F_flat):
+    """1/2 sum_ij D_ij (H_ij + F_ij) + V_nn from flat upper triangles
+    (variationalE, HartreeFock.hs:277-287, with F = F(D) rebuilt from the
+    density itself: second order in the density error)."""
+    n = mol.nbf
+    Df, Hf, Ff = flat_to_full(D_flat, n), flat_to_full(H_flat, n), flat_to_full(F_flat, n)
+    return 0.5 * float(np.sum(Df * (Hf + Ff))) + mol.nuclear_repulsion()
+
+
+def scf_energy_exact(mol, converge=1e-10):
+    """FP64 SCF energy of mol with the accurate Boys function and no screen:
+    flat ERIs (calcIntegrals), DIIS from the second step, then the
+    variational energy of the converged density with F(D) rebuilt."""
+    S, H = overlap(mol), hcore(mol, exact=True)
+    flat, _ = eri_flat(mol, exact=True, screen=False)
+    n = mol.nbf
+    r = scf(mol, lambda D: gmatrix(n, flat, D), H, S, max_steps=100, ndiis=6, converge=converge,
+            diis_start=float("inf"))
+    D = r["D"]
+    return variational_energy(mol, D, H, H + gmatrix(n, flat, D)), r
+
+
+def fd_gradient(mol, h=1e-4, converge=1e-10):
+    """FP64 CPU reference gradient dE/dR (natom x 3, Eh/bohr): central
+    differences at +-h bohr of scf_energy_exact along every coordinate.  The
+    reference has no working gradient (HartreeFock/Derivatives.hs does not
+    compile), so this is the oracle's gradient: the derivative of its own
+    SCF energy, whose pieces (S, T, V, ERIs, the SCF driver) are pinned
+    separately."""
+    g = np.zeros((mol.natom, 3))
+    for a in range(mol.natom):
+        for k in range(3):
+            R = mol.R.copy()
+            R[a, k] += h
+            ep, _ = scf_energy_exact(mol.with_coords(R), converge)
+            R[a, k] -= 2 * h
+            em, _ = scf_energy_exact(mol.with_coords(R), converge)
+            g[a, k] = (ep - em) / (2 * h)
+    return g

@@ -106,3 +106,20 @@ def test_no_gpu_fallback_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
     with pytest.raises(RuntimeError):
         _lib.lib()
+
+
+def test_parse_xyz_skips_exactly_two_lines():
+    """parserXYZ consumes the count line and exactly one comment line, which
+    may be empty (ParseXYZ.hs:33-43); coordinates are Angstrom -> bohr."""
+    atoms = "O 0.0 0.0 0.1173\nH 0.0 0.7572 -0.4692\nH 0.0 -0.7572 -0.4692\n"
+    for comment in ("", "water, comment line", "   "):
+        syms, xyz = basis.parse_xyz("3\n%s\n%s" % (comment, atoms))
+        assert syms == ["O", "H", "H"]
+        assert abs(xyz[1][1] - 0.7572 / 0.529177) < 1e-14 and abs(xyz[0][2] - 0.1173 / 0.529177) < 1e-14
+    # leading blank lines before the count and blank lines between atoms are whitespace
+    syms, _ = basis.parse_xyz("\n\n  3\ncomment\nO 0 0 0\n\nH 0 0 1\nH 0 1 0")
+    assert syms == ["O", "H", "H"]
+    with pytest.raises(ValueError):
+        basis.parse_xyz("3\n\nO 0 0 0\nH 0 0 1\n")
+    with pytest.raises(ValueError):
+        basis.parse_xyz("comment only\n")

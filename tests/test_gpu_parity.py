@@ -220,8 +220,11 @@ def test_textbook_h2_and_heh_through_the_abi(require_gpu):
 
 
 def test_too_many_primitives_is_an_error(require_gpu):
-    """A shell pair holds at most 255 primitive pairs (tasks carry 8-bit
-    primitive ranges): 16 x 16 is refused at context creation, 15 x 15 works."""
+    """A shell pair holds at most 254 primitive pairs (tasks carry 8-bit
+    primitive ranges; 255 marks a whole ket): 16 x 16 is refused at context
+    creation, 15 x 15 works.  A pair of 255 = 15 x 17 = 5 x 51 = 3 x 85
+    primitive pairs cannot occur alone: the larger shell's own pair (289 or
+    more) is refused first."""
     _say("case: primitive-count limit")
     def h2(nprim):
         prims = [(1.0, 0.1 * 1.7 ** k) for k in range(nprim)]

@@ -35,7 +35,7 @@ def test_boys_table_f0_nodes_are_erf_closed_form():
 
 def test_boys_table_deviation_profile():
     """The reference's m>=1 nodes come from a 256-point Gauss-Legendre
-    quadrature of t^(m-1/2) e^(-xt) (Boys.hs:130-136,176-188); its error
+    quadrature of t^(m-1/2) e^(-xt) (Boys.hs:60-66,106-118); its error
     is largest for m=1 (integrable t^(1/2) singularity).  Independent
     accurate values: mpmath (via tests/os_eri.boys)."""
     import os_eri
@@ -427,3 +427,134 @@ def test_oracle_matches_its_committed_fixtures():
     q = np.array(s["quartets"], dtype=np.int32).reshape(-1, 4)
     np.testing.assert_array_equal(q, O.random_canonical_quartets(mol.nbf, len(q), seed=s["seed"]))
     np.testing.assert_array_equal(O.eri_list(mol, q), np.array([float(v) for v in s["values"]]))
+
+
+def test_oracle_fd_gradient_properties():
+    """The oracle's FP64 gradient (central differences of its exact-Boys,
+    unscreened SCF energy) on water/6-31G: translation invariant, zero
+    normal to the molecular plane (x), the two H atoms mirror images in y,
+    and unchanged to 1e-7 when the step is halved."""
+    from hfk import systems
+    mol = systems.water("6-31g")
+    g = O.fd_gradient(mol, h=1e-4)
+    assert np.max(np.abs(g.sum(axis=0))) < 1e-7
+    assert np.max(np.abs(g[:, 0])) < 1e-7
+    assert abs(g[1, 1] + g[2, 1]) < 1e-7 and abs(g[1, 2] - g[2, 2]) < 1e-7
+    assert np.max(np.abs(g)) > 1e-3   # not at the equilibrium geometry
+    g2 = O.fd_gradient(mol, h=5e-5)
+    assert np.max(np.abs(g - g2)) < 1e-7
+
+
+def _onee_pairs(mol):
+    """Function pairs covering s-p, p-p, s-d, p-d and d-d on one centre and
+    on two centres (water/6-31G*: O 1s 2s 2p 3s 3p 6d, H 1s 2s)."""
+    L = mol.lxyz.sum(axis=1)
+    at = mol.atom_of_bf
+    pairs = []
+    for li, lj in ((0, 1), (1, 1), (0, 2), (1, 2), (2, 2), (0, 0)):
+        for same in (True, False):
+            cand = [(i, j) for i in range(mol.nbf) for j in range(i, mol.nbf)
+                    if {L[i], L[j]} == {li, lj} and (at[i] == at[j]) == same]
+            pairs += cand[:: max(1, len(cand) // 4)][:4]
+    return pairs
+
+
+def test_oracle_overlap_kinetic_p_d_against_quadrature():
+    """mtxOverlap (IntegralsEvaluation.hs:169-221) and the kinetic part of
+    hcore (tab, :263-280) for p and d functions against Gauss-Hermite
+    quadrature of the same primitives (tests/os_onee.py, no recursion in
+    common).  T is hcore with every nuclear charge set to zero.  The
+    reference forces S_ii = 1; the true self-overlaps of the normalised
+    contractions differ from 1 by less than 1e-5."""
+    import copy
+    import os_onee
+    from hfk import systems
+    mol = systems.water("6-31g*")
+    n = mol.nbf
+    S = O.flat_to_full(O.overlap(mol), n)
+    m0 = copy.copy(mol)
+    m0.Z = np.zeros_like(mol.Z)
+    T = O.flat_to_full(O.hcore(m0, exact=True), n)
+    pairs = _onee_pairs(mol)
+    assert len(pairs) >= 30
+    for i, j in pairs:
+        s_ref = os_onee.contracted(mol, i, j, "S")
+        t_ref = os_onee.contracted(mol, i, j, "T")
+        if i != j:
+            assert abs(S[i, j] - s_ref) < 1e-13, (i, j, S[i, j], s_ref)
+        assert abs(T[i, j] - t_ref) < 1e-11 * max(1.0, abs(t_ref)), (i, j, T[i, j], t_ref)
+    for i in range(n):
+        assert S[i, i] == 1.0
+        assert abs(os_onee.contracted(mol, i, i, "S") - 1.0) < 1e-5
+
+
+def test_oracle_nuclear_attraction_p_d_against_quadrature():
+    """The nuclear-attraction part of hcore (vijHermite / mcMurchie,
+    IntegralsEvaluation.hs:385-414) for p and d functions against the
+    Gaussian-transform quadrature of tests/os_onee.py: with only the first
+    H nucleus charged (Z = 1), hcore - T = -V_H."""
+    import copy
+    import os_onee
+    from hfk import systems
+    mol = systems.water("6-31g*")
+    n = mol.nbf
+    m0 = copy.copy(mol)
+    m0.Z = np.zeros_like(mol.Z)
+    mh = copy.copy(mol)
+    mh.Z = np.zeros_like(mol.Z)
+    mh.Z[1] = 1.0
+    V = -(O.flat_to_full(O.hcore(mh, exact=True), n) - O.flat_to_full(O.hcore(m0, exact=True), n))
+    for i, j in _onee_pairs(mol)[::2]:
+        v_ref = os_onee.contracted(mol, i, j, "V", C=mol.R[1])
+        assert abs(V[i, j] - v_ref) < 1e-10 * max(1.0, abs(v_ref)), (i, j, V[i, j], v_ref)
+
+
+def test_quadrature_one_electron_closed_forms():
+    """The quadrature code itself against closed forms for normalised
+    primitives on one centre: <s|s> = 1, <s|T|s> = 3a/2, <px|T|px> = 5a/2,
+    <dxy|T|dxy> = 7a/2, <dxx|T|dxx> = 13a/6 (x^2 carries an s admixture:
+    7a/6 along x plus a/2 along y and z); and <s_a|1/r|s_a> = 2 sqrt(2a/pi) (nucleus at the
+    centre), <s_a|1/r_C|s_a> = erf(sqrt(2a) R)/R (nucleus at distance R)."""
+    import math
+    import os_onee
+    a, O3 = 0.8, (0.0, 0.0, 0.0)
+
+    def nrm(l):
+        return 1.0 / math.sqrt(os_onee.prim_overlap(O3, l, a, O3, l, a))
+
+    for l, t in (((0, 0, 0), 1.5), ((1, 0, 0), 2.5), ((1, 1, 0), 3.5), ((2, 0, 0), 13.0 / 6.0)):
+        c = nrm(l) ** 2
+        assert abs(os_onee.prim_kinetic(O3, l, a, O3, l, a) * c - t * a) < 1e-13
+    c = nrm((0, 0, 0)) ** 2
+    assert abs(os_onee.prim_nuclear(O3, (0, 0, 0), a, O3, (0, 0, 0), a, O3) * c - 2 * math.sqrt(2 * a / math.pi)) < 1e-12
+    R = 1.7
+    v = os_onee.prim_nuclear(O3, (0, 0, 0), a, O3, (0, 0, 0), a, (0.0, 0.0, R)) * c
+    assert abs(v - math.erf(math.sqrt(2 * a) * R) / R) < 1e-12
+
+
+def test_water_sto3g_published_energy():
+    """Water/STO-3G at the geometry of T. D. Crawford's Hartree-Fock
+    programming project #3 (bohr): published E_nuc = 8.002367061810450 and
+    E_SCF = -74.942079928192 Eh (exact integrals, normalised contractions).
+    With each contraction scaled to unit self-overlap (the quadrature of
+    tests/os_onee.py; the stored STO-3G coefficients are normalised to
+    ~3e-8) the oracle's exact-Boys, unscreened SCF reproduces the printed
+    value.  This pins S, T, V for s and p functions, the ERIs and the SCF
+    driver together."""
+    import copy
+    import math
+    import os_onee
+    from hfk import basis
+    xyz = [(0.0, -0.143225816552, 0.0), (1.638036840407, 1.136548822547, 0.0), (-1.638036840407, 1.136548822547, 0.0)]
+    mol = basis.build_molecule(["O", "H", "H"], xyz, "sto-3g")
+    assert abs(mol.nuclear_repulsion() - 8.002367061810450) < 1e-11
+    e_stored, _ = O.scf_energy_exact(mol)
+    mol = copy.deepcopy(mol)
+    for i in range(mol.nbf):
+        s = os_onee.contracted(mol, i, i, "S")
+        assert abs(s - 1.0) < 1e-7
+        mol.coef[mol.prim_off[i]:mol.prim_off[i + 1]] /= math.sqrt(s)
+    e, r = O.scf_energy_exact(mol, converge=1e-11)
+    assert abs(e - (-74.942079928192)) < 1e-10
+    # the reference's convention (coefficients as stored, S_ii := 1) moves E by the self-overlap defect only
+    assert abs(e_stored - e) < 1e-6
