@@ -693,10 +693,131 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
             }
         }
         return;
+    } else if constexpr (OP == HFK_OP_JK) {
+        // J/K: a task is one run of kets with explicit primitive ranges
+        // (uniform over the workgroup), walked S kets at a time.  One wave,
+        // so the LDS accumulators (K' rows of the bra shells, the slots'
+        // J'_cd sums) take the lanes' adds in a fixed hardware order; the
+        // results go to the task's slots with plain stores.
+        const long long tix = a.task0 + blockIdx.x;
+        const int4 task = a.tasks[blockIdx.x];
+        const int len = task_ket_count(a, task);
+        const int k0 = (task.w >> 8) & 0xff, k1 = task.w & 0xff;
+        // uniform over the workgroup, before any barrier; this kernel needs an
+        // explicit ket primitive range (the planner never gives it k1 = all)
+        if (k1 == 0xff) return;
+        const int bp = task.x;
+        const int b0 = (task.w >> 24) & 0xff, b1 = (task.w >> 16) & 0xff;
+        const int A = a.pair_A[bp], B = a.pair_B[bp];
+        const int ia = a.sh_bfp[A * HFK_MAXCOMP + ca], ib = a.sh_bfp[B * HFK_MAXCOMP + cb];
+        const double sab = active ? a.sh_scale[A * HFK_MAXCOMP + ca] * a.sh_scale[B * HFK_MAXCOMP + cb] : 0.0;
+        const double* __restrict__ D = a.D;
+        const double dab = D[(size_t)ia * ld + ib];
+        const int ldk = a.ncols + 1;
+        double* krow = dyn;                    // [NA+NB][ldk]
+        double* sred = dyn + (NA + NB) * ldk;  // [64] per-lane J'_ab
+        for (int t = lane; t < (NA + NB) * ldk; t += 64) krow[t] = 0.0;
+        for (int t = lane; t < SH::S * NKET; t += 64) (&sm.jcd[0][0])[t] = 0.0;
+        // bra tables once per task when its primitive range fits the staging area
+        const bool staged = (b1 - b0) <= SH::NBP;
+        if (staged) coop_stage_bra(sm, a.pp + (size_t)a.pair_pp0[bp] * PP_STRIDE, b0, b1 - b0);
+        __syncthreads();
+        const KetRun run = a.runs[task.y];
+        const int* __restrict__ colmap = a.colmap;
+        double jab = 0.0;
+        for (int g0 = 0; g0 < len; g0 += SH::S) {
+            const int it = g0 + slot;
+            int kp = -1;
+            if (active && it < len) {
+                kp = a.ket_idx[run.x + it];
+                if (!quartet_selected(a, bp, kp)) kp = -1;
+            }
+            int C = 0, Dd = 0;
+            int ic[NC], id[ND], cc[NC], cdc[ND];
+            double sc[NC], sd[ND];
+            double f = 0.0;
+            if (kp >= 0) {
+                C = a.pair_A[kp];
+                Dd = a.pair_B[kp];
+                f = 1.0;
+                if (A == B) f *= 0.5;
+                if (C == Dd) f *= 0.5;
+                if (bp == kp) f *= 0.5;
+            }
+#pragma unroll
+            for (int x = 0; x < NC; ++x) {
+                ic[x] = kp >= 0 ? a.sh_bfp[C * HFK_MAXCOMP + x] : a.n;
+                sc[x] = kp >= 0 ? a.sh_scale[C * HFK_MAXCOMP + x] : 0.0;
+                cc[x] = colmap[ic[x]];
+            }
+#pragma unroll
+            for (int x = 0; x < ND; ++x) {
+                id[x] = kp >= 0 ? a.sh_bfp[Dd * HFK_MAXCOMP + x] : a.n;
+                sd[x] = kp >= 0 ? a.sh_scale[Dd * HFK_MAXCOMP + x] : 0.0;
+                cdc[x] = colmap[id[x]];
+            }
+            double V[NKET];
+            coop_group<LA, LB, LC, LD, BOYS, MODE, OP>(a, sm, bp, kp, b0, b1, k0, k1, staged, V, nullptr, nullptr,
+                                                       nullptr, nullptr);
+            if (kp >= 0) {
+                const size_t rowa = (size_t)ca * ldk, rowb = (size_t)(NA + cb) * ldk;
+                double js = 0.0;
+#pragma unroll
+                for (int cd = 0; cd < NKET; ++cd) {
+                    const int c = cd / ND, d = cd - (cd / ND) * ND;
+                    V[cd] *= f * sab * sc[c] * sd[d];
+                    js += V[cd] * D[(size_t)ic[c] * ld + id[d]];
+                    atomic_add_lds(&sm.jcd[slot][cd], 2.0 * V[cd] * dab);
+                }
+                jab += 2.0 * js;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    double s1 = 0.0, s3 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) {
+                        s1 += V[c * ND + d] * D[(size_t)ib * ld + id[d]];
+                        s3 += V[c * ND + d] * D[(size_t)ia * ld + id[d]];
+                    }
+                    atomic_add_lds(&krow[rowa + cc[c]], s1);
+                    atomic_add_lds(&krow[rowb + cc[c]], s3);
+                }
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    double s2 = 0.0, s4 = 0.0;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        s2 += V[c * ND + d] * D[(size_t)ib * ld + ic[c]];
+                        s4 += V[c * ND + d] * D[(size_t)ia * ld + ic[c]];
+                    }
+                    atomic_add_lds(&krow[rowa + cdc[d]], s2);
+                    atomic_add_lds(&krow[rowb + cdc[d]], s4);
+                }
+            }
+            __syncthreads();
+            // every ket of the task rewrites its J'_cd entry (zero when skipped)
+            if (active && it < len) {
+                for (int z = li; z < NKET; z += NBRA) {
+                    a.jcd[(run.jcd + it) * NKET + z] = sm.jcd[slot][z];
+                    sm.jcd[slot][z] = 0.0;
+                }
+            }
+            __syncthreads();
+        }
+        // J'_ab: the slots' partial sums in slot order
+        sred[lane] = active ? jab : 0.0;
+        __syncthreads();
+        if (lane < NBRA) {
+            double v = 0.0;
+            for (int sl = 0; sl < SH::S; ++sl) v += sred[sl * NBRA + lane];
+            a.jab[(size_t)a.jab_slot[tix] * NBRA + lane] = v;
+        }
+        double* __restrict__ ks = a.kslot + (size_t)tix * (NA + NB) * a.ncols;
+        for (int r = 0; r < NA + NB; ++r)
+            for (int col = lane; col < a.ncols; col += 64) ks[(size_t)r * a.ncols + col] = krow[r * ldk + col];
     } else {
         const int4 task = a.tasks[blockIdx.x];
         // a long task (planned for the per-lane kernel) is cut into slices of
-        // 4 S kets along grid y; every slice digests independently
+        // 4 S kets along grid y; every slice adds into the gradient independently
         const int kslice = 4 * SH::S;
         const int kbeg = (int)blockIdx.y * kslice;   // kets of the task, in run order
         const int ktot = task_ket_count(a, task);
@@ -712,24 +833,21 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
         const double sab = active ? a.sh_scale[A * HFK_MAXCOMP + ca] * a.sh_scale[B * HFK_MAXCOMP + cb] : 0.0;
         const double* __restrict__ D = a.D;
         const double dab = D[(size_t)ia * ld + ib];
-        const bool lds = (OP == HFK_OP_JK) ? a.lds_rows != 0 : a.grad_lds != 0;
-        const int ndyn = (OP == HFK_OP_JK) ? (NA + NB) * ld : 3 * a.natom;
+        const bool lds = a.grad_lds != 0;
+        const int ndyn = 3 * a.natom;
         if (lds)
             for (int t = lane; t < ndyn; t += 64) dyn[t] = 0.0;
-        for (int t = lane; t < SH::S * NKET; t += 64) (&sm.jcd[0][0])[t] = 0.0;
         // bra tables once per task when its primitive range fits the staging area
         const bool staged = (b1 - b0) <= SH::NBP;
         if (staged) coop_stage_bra(sm, a.pp + (size_t)a.pair_pp0[bp] * PP_STRIDE, b0, b1 - b0);
         __syncthreads();
-        double jab = 0.0;
         double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
         const int len = kend - kbeg;
         for (int g0 = 0; g0 < len; g0 += SH::S) {
             const int it = g0 + slot;
-            int kp = -1, kpos = -1;
+            int kp = -1;
             if (active && it < len) {
-                kpos = task_ket_pos(a, task, kbeg + it);
-                kp = a.ket_idx[kpos];
+                kp = a.ket_idx[task_ket_pos(a, task, kbeg + it)];
                 if (!quartet_selected(a, bp, kp)) kp = -1;
             }
             int C = 0, Dd = 0;
@@ -754,135 +872,78 @@ __global__ void __launch_bounds__(64) k_coop(QuartetArgs a)
                 id[x] = kp >= 0 ? a.sh_bfp[Dd * HFK_MAXCOMP + x] : a.n;
                 sd[x] = kp >= 0 ? a.sh_scale[Dd * HFK_MAXCOMP + x] : 0.0;
             }
-            if constexpr (OP == HFK_OP_JK) {
-                double V[NKET];
-                coop_group<LA, LB, LC, LD, BOYS, MODE, OP>(a, sm, bp, kp, b0, b1, k0, k1, staged, V, nullptr,
-                                                           nullptr, nullptr, nullptr);
-                if (kp >= 0) {
-                    double* __restrict__ K = lds ? dyn : a.K;
-                    const size_t rowa = lds ? (size_t)ca * ld : (size_t)ia * ld;
-                    const size_t rowb = lds ? (size_t)(NA + cb) * ld : (size_t)ib * ld;
-                    double js = 0.0;
+            // two-particle density of the lane's bra row: 4 f s_abcd Gamma
+            double Gw[NKET];
 #pragma unroll
-                    for (int cd = 0; cd < NKET; ++cd) {
-                        const int c = cd / ND, d = cd - (cd / ND) * ND;
-                        V[cd] *= f * sab * sc[c] * sd[d];
-                        js += V[cd] * D[(size_t)ic[c] * ld + id[d]];
-                        atomic_add_f64(&sm.jcd[slot][cd], 2.0 * V[cd] * dab);
-                    }
-                    jab += 2.0 * js;
+            for (int cd = 0; cd < NKET; ++cd) {
+                const int c = cd / ND, d = cd - (cd / ND) * ND;
+                const double dcd = D[(size_t)ic[c] * ld + id[d]];
+                const double dac = D[(size_t)ia * ld + ic[c]], dbd = D[(size_t)ib * ld + id[d]];
+                const double dad = D[(size_t)ia * ld + id[d]], dbc = D[(size_t)ib * ld + ic[c]];
+                Gw[cd] = (4.0 * f * sab * sc[c] * sd[d]) * (dab * dcd - 0.25 * (dac * dbd + dad * dbc));
+            }
+            double ga[3], gb[3], gc[3];
+            coop_group<LA, LB, LC, LD, BOYS, MODE, OP>(a, sm, bp, kp, b0, b1, k0, k1, staged, nullptr, Gw, ga, gb,
+                                                       gc);
+            if (kp >= 0) {
+                double* acc = lds ? dyn : a.grad;
+                const int atC = a.sh_atom[C], atD = a.sh_atom[Dd];
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) {
-                        double s1 = 0.0, s3 = 0.0;
-#pragma unroll
-                        for (int d = 0; d < ND; ++d) {
-                            s1 += V[c * ND + d] * D[(size_t)ib * ld + id[d]];
-                            s3 += V[c * ND + d] * D[(size_t)ia * ld + id[d]];
-                        }
-                        atomic_add_f64(&K[rowa + ic[c]], s1);
-                        atomic_add_f64(&K[rowb + ic[c]], s3);
-                    }
-#pragma unroll
-                    for (int d = 0; d < ND; ++d) {
-                        double s2 = 0.0, s4 = 0.0;
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) {
-                            s2 += V[c * ND + d] * D[(size_t)ib * ld + ic[c]];
-                            s4 += V[c * ND + d] * D[(size_t)ia * ld + ic[c]];
-                        }
-                        atomic_add_f64(&K[rowa + id[d]], s2);
-                        atomic_add_f64(&K[rowb + id[d]], s4);
-                    }
-                }
-                __syncthreads();
-                if (kp >= 0) {
-                    for (int z = li; z < NKET; z += NBRA) {
-                        const double v = sm.jcd[slot][z];
-                        if (v != 0.0)
-                            atomic_add_f64(&a.Jk[((size_t)(blockIdx.x % (unsigned)a.jk_copies) * a.nket + kpos) * NKET + z], v);
-                        sm.jcd[slot][z] = 0.0;
-                    }
-                }
-            } else {
-                // two-particle density of the lane's bra row: 4 f s_abcd Gamma
-                double Gw[NKET];
-#pragma unroll
-                for (int cd = 0; cd < NKET; ++cd) {
-                    const int c = cd / ND, d = cd - (cd / ND) * ND;
-                    const double dcd = D[(size_t)ic[c] * ld + id[d]];
-                    const double dac = D[(size_t)ia * ld + ic[c]], dbd = D[(size_t)ib * ld + id[d]];
-                    const double dad = D[(size_t)ia * ld + id[d]], dbc = D[(size_t)ib * ld + ic[c]];
-                    Gw[cd] = (4.0 * f * sab * sc[c] * sd[d]) * (dab * dcd - 0.25 * (dac * dbd + dad * dbc));
-                }
-                double ga[3], gb[3], gc[3];
-                coop_group<LA, LB, LC, LD, BOYS, MODE, OP>(a, sm, bp, kp, b0, b1, k0, k1, staged, nullptr, Gw, ga, gb,
-                                                           gc);
-                if (kp >= 0) {
-                    double* acc = lds ? dyn : a.grad;
-                    const int atC = a.sh_atom[C], atD = a.sh_atom[Dd];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        gA[k] += ga[k];
-                        gB[k] += gb[k];
-                        atomic_add_f64(&acc[3 * atC + k], gc[k]);
-                        atomic_add_f64(&acc[3 * atD + k], -((ga[k] + gb[k]) + gc[k]));
-                    }
+                for (int k = 0; k < 3; ++k) {
+                    gA[k] += ga[k];
+                    gB[k] += gb[k];
+                    atomic_add_f64(&acc[3 * atC + k], gc[k]);
+                    atomic_add_f64(&acc[3 * atD + k], -((ga[k] + gb[k]) + gc[k]));
                 }
             }
         }
-        if constexpr (OP == HFK_OP_JK) {
-            if (active && jab != 0.0) atomic_add_f64(&a.J[(size_t)ia * ld + ib], jab);
-            if (lds) {
-                __syncthreads();
-                for (int r = 0; r < NA + NB; ++r) {
-                    const int row = (r < NA) ? a.sh_bfp[A * HFK_MAXCOMP + r] : a.sh_bfp[B * HFK_MAXCOMP + r - NA];
-                    double* __restrict__ Kr = a.K + (size_t)row * ld;
-                    const double* sr = dyn + r * ld;
-                    for (int col = lane; col < ld; col += 64) {
-                        const double v = sr[col];
-                        if (v != 0.0) atomic_add_f64(&Kr[col], v);
-                    }
-                }
-            }
-        } else {
-            double* acc = lds ? dyn : a.grad;
-            const int atA = a.sh_atom[A], atB = a.sh_atom[B];
+        double* acc = lds ? dyn : a.grad;
+        const int atA = a.sh_atom[A], atB = a.sh_atom[B];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                double u = gA[k], v = gB[k];
+        for (int k = 0; k < 3; ++k) {
+            double u = gA[k], v = gB[k];
 #pragma unroll
-                for (int off = WAVE / 2; off > 0; off >>= 1) {
-                    u += __shfl_xor(u, off, WAVE);
-                    v += __shfl_xor(v, off, WAVE);
-                }
-                if (lane == 0) {
-                    atomic_add_f64(&acc[3 * atA + k], u);
-                    atomic_add_f64(&acc[3 * atB + k], v);
-                }
+            for (int off = WAVE / 2; off > 0; off >>= 1) {
+                u += __shfl_xor(u, off, WAVE);
+                v += __shfl_xor(v, off, WAVE);
             }
-            if (lds) {
-                __syncthreads();
-                for (int t = lane; t < 3 * a.natom; t += 64)
-                    if (dyn[t] != 0.0) atomic_add_f64(&a.grad[t], dyn[t]);
+            if (lane == 0) {
+                atomic_add_f64(&acc[3 * atA + k], u);
+                atomic_add_f64(&acc[3 * atB + k], v);
             }
+        }
+        if (lds) {
+            __syncthreads();
+            for (int t = lane; t < 3 * a.natom; t += 64)
+                if (dyn[t] != 0.0) atomic_add_f64(&a.grad[t], dyn[t]);
         }
     }
 }
 
+// 0, or -3 when the K' rows do not fit in LDS next to the static tables
 template <int LA, int LB, int LC, int LD, int BOYS, int MODE, int OP>
-void launch_coop(const QuartetArgs& a, hipStream_t s)
+int launch_coop(const QuartetArgs& a, hipStream_t s)
 {
-    constexpr int NA = ncart(LA), NB = ncart(LB);
-    if constexpr (OP == HFK_OP_JK || OP == HFK_OP_GRAD) {
-        if (a.ntask <= 0) return;
+    if constexpr (OP == HFK_OP_JK) {
+        if (a.ntask <= 0) return 0;
+        const size_t shm = hfk_coop_jk_lds_bytes(LA, LB, a.ncols);
+        if (shm + sizeof(CoopLds<CoopShape<LA, LB, LC, LD, OP>>) > 64 * 1024) return -3;
+        long long done = 0;
+        while (done < a.ntask) {
+            const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);
+            QuartetArgs b = a;
+            b.tasks = a.tasks + done;
+            b.task0 = a.task0 + done;
+            hipLaunchKernelGGL((k_coop<LA, LB, LC, LD, BOYS, MODE, OP>), dim3((unsigned)cnt), dim3(64), shm, s, b);
+            done += cnt;
+        }
+    } else if constexpr (OP == HFK_OP_GRAD) {
+        if (a.ntask <= 0) return 0;
         using SH = CoopShape<LA, LB, LC, LD, OP>;
         QuartetArgs base = a;
-        size_t shm = 0;
-        if (OP == HFK_OP_JK && a.lds_rows) shm = sizeof(double) * (NA + NB) * (size_t)(a.n + 1);
-        if (OP == HFK_OP_GRAD && a.grad_lds) shm = sizeof(double) * 3 * (size_t)a.natom;
+        size_t shm = a.grad_lds ? sizeof(double) * 3 * (size_t)a.natom : 0;
         if (shm + sizeof(CoopLds<SH>) > 64 * 1024) {   // stage nothing: global atomics
             shm = 0;
-            base.lds_rows = 0;
             base.grad_lds = 0;
         }
         long long done = 0;
@@ -896,22 +957,22 @@ void launch_coop(const QuartetArgs& a, hipStream_t s)
             done += cnt;
         }
     } else {
-        if (a.nquart <= 0) return;
+        if (a.nquart <= 0) return 0;
         const long long g = std::min<long long>(a.nquart, 65536);
         hipLaunchKernelGGL((k_coop<LA, LB, LC, LD, BOYS, MODE, OP>), dim3((unsigned)g), dim3(64), 0, s, a);
     }
+    return 0;
 }
 
 template <int LA, int LB, int LC, int LD, int OP>
-void launch_coop_modes(int boys, int mode, const QuartetArgs& a, hipStream_t s)
+int launch_coop_modes(int boys, int mode, const QuartetArgs& a, hipStream_t s)
 {
     if (boys == 0) {
-        if (mode == 0) launch_coop<LA, LB, LC, LD, 0, 0, OP>(a, s);
-        else launch_coop<LA, LB, LC, LD, 0, 1, OP>(a, s);
-    } else {
-        if (mode == 0) launch_coop<LA, LB, LC, LD, 1, 0, OP>(a, s);
-        else launch_coop<LA, LB, LC, LD, 1, 1, OP>(a, s);
+        if (mode == 0) return launch_coop<LA, LB, LC, LD, 0, 0, OP>(a, s);
+        return launch_coop<LA, LB, LC, LD, 0, 1, OP>(a, s);
     }
+    if (mode == 0) return launch_coop<LA, LB, LC, LD, 1, 0, OP>(a, s);
+    return launch_coop<LA, LB, LC, LD, 1, 1, OP>(a, s);
 }
 
 // op: HFK_OP_JK / HFK_OP_GRAD (mode from the plan), HFK_OP_FLAT (screened
@@ -921,10 +982,10 @@ int dispatch_coop(int op, int prec, int boys, int mode, const QuartetArgs& a, hi
 {
     if (prec != 0) return -1;
     switch (op) {
-    case HFK_OP_JK: launch_coop_modes<LA, LB, LC, LD, HFK_OP_JK>(boys, mode, a, s); return 0;
-    case HFK_OP_GRAD: launch_coop_modes<LA, LB, LC, LD, HFK_OP_GRAD>(boys, mode, a, s); return 0;
-    case HFK_OP_FLAT: launch_coop_modes<LA, LB, LC, LD, HFK_OP_FLAT>(boys, a.qtau > 0.0 ? 1 : 0, a, s); return 0;
-    case HFK_OP_DIAG: launch_coop_modes<LA, LB, LC, LD, HFK_OP_DIAG>(boys, 0, a, s); return 0;
+    case HFK_OP_JK: return launch_coop_modes<LA, LB, LC, LD, HFK_OP_JK>(boys, mode, a, s);
+    case HFK_OP_GRAD: return launch_coop_modes<LA, LB, LC, LD, HFK_OP_GRAD>(boys, mode, a, s);
+    case HFK_OP_FLAT: return launch_coop_modes<LA, LB, LC, LD, HFK_OP_FLAT>(boys, a.qtau > 0.0 ? 1 : 0, a, s);
+    case HFK_OP_DIAG: return launch_coop_modes<LA, LB, LC, LD, HFK_OP_DIAG>(boys, 0, a, s);
     default: return -1;
     }
 }

@@ -113,8 +113,12 @@ __device__ __forceinline__ void boys_eval(T x, const BoysTables& bt, T* F)
     }
 }
 
-// Atomic add of a double into global memory with the hardware FP64 atomic.
+// Atomic add of a double with the hardware FP64 atomic (ds_add_f64 on an
+// LDS address, the L2 atomic on a global one).
 __device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
+// The same, for LDS accumulators of one workgroup (the J/K kernels use no
+// global FP atomics).
+__device__ __forceinline__ void atomic_add_lds(double* p, double v) { unsafeAtomicAdd(p, v); }
 
 // floor(x / 0.1) of the reference (TableBoys.hs:47-48) without a division:
 // x*10 and x/0.1 differ by less than 2^-51 relative, so their floors can

@@ -117,43 +117,51 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 // ---------------------------------------------------------------------
 // Tiled J/K kernel: one workgroup per task = (bra shell pair, runs of the ket
 // class list (each of one primitive-pair count), primitive-pair ranges of
-// bra and ket).  Each lane loops over its ket primitive pairs
-// (vector loads, once each) and, inside, over the bra primitive pairs
-// (wave-uniform, scalar loads).  Splitting heavy quartets into primitive
-// ranges evens out the work per lane; the digestion is linear in the
-// integrals, so partial quartets are digested independently.  The bra pair is
-// wave-uniform (its data comes through scalar loads); each lane takes ket
-// pairs of the range.  D, J', K' are (n+1) x (n+1) with a zero row/column
-// n that absent Cartesian components point to, so the digestion has no
-// branches.  Per quartet:
-//   J'_ab += 2f sum_cd V D_cd      lane registers, one wave reduction per task
-//   K'_a. , K'_b.                  rows of the two bra shells staged in LDS
-//                                  (ds_add_f64), flushed once per task
-//   J'_cd += 2f sum_ab V D_ab      one global FP64 atomic per ket function pair,
-//                                  by ket position (coalesced), scattered into J' later
+// bra and ket).  Each lane loops over its ket primitive pairs (vector loads,
+// once each) and, inside, over the bra primitive pairs (staged in LDS,
+// wave-uniform).  Splitting heavy quartets into primitive ranges evens out
+// the work per lane; the digestion is linear in the integrals, so partial
+// quartets are digested independently.  D is (n+1) x (n+1) with a zero
+// row/column n that absent Cartesian components point to, so the digestion
+// has no branches.  Per quartet:
+//   J'_ab += 2f sum_cd V D_cd   lane registers; at the end one wave
+//                               reduction, the waves summed in order, one
+//                               store to the task's slot
+//   J'_cd  = 2f sum_ab V D_ab   one store per ket to its own slot entry
+//   K'_a. , K'_b.               rows of the two bra shells staged in LDS
+//                               (compact columns: the functions of the
+//                               launch's ket shells), stored to the task's
+//                               slot at the end
+// No global atomics: the slots are summed in a fixed order afterwards, so a
+// Fock build is bitwise reproducible.  In a workgroup of several waves the
+// waves add into the LDS rows in turn (wave 0, then 1, ...; a barrier
+// between turns); within one wave the hardware orders the lanes' adds, so
+// the LDS sums do not depend on scheduling either.
 // ---------------------------------------------------------------------
-// Workgroups are 64 lanes, or 256 lanes (4 waves sharing one set of staged K
-// rows) when the rows are long enough to limit occupancy through LDS.
 template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
 __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NKET = NC * ND, NBRA = NA * NB;
-    extern __shared__ double krow[];   // [(NA+NB)][ld] when lds_rows, then the staged bra
+    constexpr int NR = NA + NB;
+    extern __shared__ double smem[];
+    const long long tix = a.task0 + blockIdx.x;    // slot of this task
     const int4 task = a.tasks[blockIdx.x];
     const int bp = task.x;
     // primitive-pair ranges: bra [b0,b1), ket [k0,k1), 8 bits each
     const int bprim0 = (task.w >> 24) & 0xff, bprim1 = (task.w >> 16) & 0xff;
     const int kprim0 = (task.w >> 8) & 0xff, kprim1 = task.w & 0xff;
     const int ld = a.n + 1;
-    const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & (WAVE - 1);
-    const bool lds = a.lds_rows != 0;
-    if (lds)
-        for (int t = tid; t < (NA + NB) * ld; t += nthr) krow[t] = 0.0;
+    const int ldk = a.ncols + 1;                   // LDS row: compact columns + one discarded
+    const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    const int nwaves = nthr / WAVE;
+    double* krow = smem;                           // [NR][ldk]
+    double* sred = krow + NR * ldk;                // [4][NBRA] cross-wave J'_ab
+    double* sbra = sred + 4 * NBRA;                // bra primitive-pair records, then their screen values
+    for (int t = tid; t < NR * ldk; t += nthr) krow[t] = 0.0;
     // the task's bra primitive-pair records and their per-component screen
     // values, read by every lane in every ket iteration: staged once in LDS
     const int nbs = bprim1 - bprim0;
-    double* sbra = krow + (lds ? (NA + NB) * ld : 0);
     double* sbq = sbra + nbs * PP_STRIDE;
     {
         const double* g = a.pp + (size_t)(a.pair_pp0[bp] + bprim0) * PP_STRIDE;
@@ -181,164 +189,208 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
     double jab[NBRA];
 #pragma unroll
     for (int k = 0; k < NBRA; ++k) jab[k] = 0.0;
+    double* __restrict__ Jcd = a.jcd;
+    const int* __restrict__ colmap = a.colmap;
 
-    // ket-side J'_cd: summed by position in the ket class list, so the
-    // lanes' atomics fall on consecutive addresses; tasks resident at the same
-    // time share ket runs, so workgroups spread them over a few copies
-    double* __restrict__ Jk = a.Jk + (size_t)(blockIdx.x % (unsigned)a.jk_copies) * a.nket * NKET;
-    // (lanes take the kets in run order: a wave meets one primitive-pair
-    // count except at run boundaries)
+    // lanes take the kets in run order (a wave meets one primitive-pair
+    // count except at run boundaries); every workgroup runs the same number
+    // of rounds, so the turn barriers are uniform
     const int len = task_ket_count(a, task);
-    for (int it = tid; it < len; it += nthr) {
-        const int pos = task_ket_pos(a, task, it);
-        const int kp = a.ket_idx[pos];
-        if (!quartet_selected(a, bp, kp)) continue;
-        T Vt[NBRA * NKET];
-        eri_class<LA, LB, LC, LD, T, BOYS, MODE>(sbra, sbq, nbs, a.pp + (size_t)(a.pair_pp0[kp] + kprim0) * PP_STRIDE,
-                                                 task_kprim_end(a, kprim1, kp) - kprim0, a.qcomp, a.bt, Vt, a.qtau);
-        const int C = a.pair_A[kp], Dd = a.pair_B[kp];
-        int ic[NC], id[ND];
-        double sc[NC], sd[ND];
+    KetWalk walk;
+    walk.begin(a, task);
+    for (int base = 0; base < len; base += nthr) {
+        const int it = base + tid;
+        const bool in = it < len;
+        int pos = 0;
+        long long entry = 0;
+        if (in) walk.seek(a, it, pos, entry);
+        const int kp = in ? a.ket_idx[pos] : 0;
+        const bool act = in && quartet_selected(a, bp, kp);
+        // K' contributions: rows a (x < NA) and b, at columns c and d
+        double kac[NA * NC], kad[NA * ND], kbc[NB * NC], kbd[NB * ND];
+        int cc[NC], cd[ND];
+        double jv[NKET];
 #pragma unroll
-        for (int x = 0; x < NC; ++x) { ic[x] = a.sh_bfp[C * HFK_MAXCOMP + x]; sc[x] = a.sh_scale[C * HFK_MAXCOMP + x]; }
+        for (int k = 0; k < NKET; ++k) jv[k] = 0.0;
+        if (act) {
+            T Vt[NBRA * NKET];
+            eri_class<LA, LB, LC, LD, T, BOYS, MODE>(sbra, sbq, nbs,
+                                                     a.pp + (size_t)(a.pair_pp0[kp] + kprim0) * PP_STRIDE,
+                                                     task_kprim_end(a, kprim1, kp) - kprim0, a.qcomp, a.bt, Vt, a.qtau);
+            const int C = a.pair_A[kp], Dd = a.pair_B[kp];
+            int ic[NC], id[ND];
+            double sc[NC], sd[ND];
 #pragma unroll
-        for (int x = 0; x < ND; ++x) { id[x] = a.sh_bfp[Dd * HFK_MAXCOMP + x]; sd[x] = a.sh_scale[Dd * HFK_MAXCOMP + x]; }
-        double f = 1.0;
-        if (A == B) f *= 0.5;
-        if (C == Dd) f *= 0.5;
-        if (bp == kp) f *= 0.5;
-        double V[NBRA * NKET];
+            for (int x = 0; x < NC; ++x) {
+                ic[x] = a.sh_bfp[C * HFK_MAXCOMP + x];
+                sc[x] = a.sh_scale[C * HFK_MAXCOMP + x];
+                cc[x] = colmap[ic[x]];
+            }
 #pragma unroll
-        for (int x = 0; x < NA; ++x)
+            for (int x = 0; x < ND; ++x) {
+                id[x] = a.sh_bfp[Dd * HFK_MAXCOMP + x];
+                sd[x] = a.sh_scale[Dd * HFK_MAXCOMP + x];
+                cd[x] = colmap[id[x]];
+            }
+            double f = 1.0;
+            if (A == B) f *= 0.5;
+            if (C == Dd) f *= 0.5;
+            if (bp == kp) f *= 0.5;
+            double V[NBRA * NKET];
 #pragma unroll
-            for (int y = 0; y < NB; ++y)
-#pragma unroll
-                for (int z = 0; z < NC; ++z)
-#pragma unroll
-                    for (int w = 0; w < ND; ++w) {
-                        const int k = ((x * NB + y) * NC + z) * ND + w;
-                        V[k] = (double)Vt[k] * (f * sa[x] * sb[y] * sc[z] * sd[w]);
-                    }
-        double Dcd[NKET];
-#pragma unroll
-        for (int z = 0; z < NC; ++z)
-#pragma unroll
-            for (int w = 0; w < ND; ++w) Dcd[z * ND + w] = D[(size_t)ic[z] * ld + id[w]];
-#pragma unroll
-        for (int b = 0; b < NBRA; ++b) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < NKET; ++k) s += V[b * NKET + k] * Dcd[k];
-            jab[b] += s;
-        }
-#pragma unroll
-        for (int k = 0; k < NKET; ++k) {
-            double s = 0.0;
-#pragma unroll
-            for (int b = 0; b < NBRA; ++b) s += V[b * NKET + k] * Dab[b];
-            atomic_add_f64(&Jk[(size_t)pos * NKET + k], 2.0 * s);
-        }
-        // density blocks D_bd, D_bc, D_ad, D_ac
-        double Dbd[NB * ND], Dbc[NB * NC], Dad[NA * ND], Dac[NA * NC];
-#pragma unroll
-        for (int y = 0; y < NB; ++y) {
-#pragma unroll
-            for (int w = 0; w < ND; ++w) Dbd[y * ND + w] = D[(size_t)ib[y] * ld + id[w]];
-#pragma unroll
-            for (int z = 0; z < NC; ++z) Dbc[y * NC + z] = D[(size_t)ib[y] * ld + ic[z]];
-        }
-#pragma unroll
-        for (int x = 0; x < NA; ++x) {
-#pragma unroll
-            for (int w = 0; w < ND; ++w) Dad[x * ND + w] = D[(size_t)ia[x] * ld + id[w]];
-#pragma unroll
-            for (int z = 0; z < NC; ++z) Dac[x * NC + z] = D[(size_t)ia[x] * ld + ic[z]];
-        }
-        double* __restrict__ Kg = a.K;
-#pragma unroll
-        for (int x = 0; x < NA; ++x) {
-#pragma unroll
-            for (int z = 0; z < NC; ++z) {
-                double s = 0.0;
+            for (int x = 0; x < NA; ++x)
 #pragma unroll
                 for (int y = 0; y < NB; ++y)
 #pragma unroll
-                    for (int w = 0; w < ND; ++w) s += V[((x * NB + y) * NC + z) * ND + w] * Dbd[y * ND + w];
-                if (lds) atomic_add_f64(&krow[x * ld + ic[z]], s);
-                else atomic_add_f64(&Kg[(size_t)ia[x] * ld + ic[z]], s);
-            }
+                    for (int z = 0; z < NC; ++z)
 #pragma unroll
-            for (int w = 0; w < ND; ++w) {
+                        for (int w = 0; w < ND; ++w) {
+                            const int k = ((x * NB + y) * NC + z) * ND + w;
+                            V[k] = (double)Vt[k] * (f * sa[x] * sb[y] * sc[z] * sd[w]);
+                        }
+            double Dcd[NKET];
+#pragma unroll
+            for (int z = 0; z < NC; ++z)
+#pragma unroll
+                for (int w = 0; w < ND; ++w) Dcd[z * ND + w] = D[(size_t)ic[z] * ld + id[w]];
+#pragma unroll
+            for (int b = 0; b < NBRA; ++b) {
                 double s = 0.0;
 #pragma unroll
-                for (int y = 0; y < NB; ++y)
+                for (int k = 0; k < NKET; ++k) s += V[b * NKET + k] * Dcd[k];
+                jab[b] += s;
+            }
 #pragma unroll
-                    for (int z = 0; z < NC; ++z) s += V[((x * NB + y) * NC + z) * ND + w] * Dbc[y * NC + z];
-                if (lds) atomic_add_f64(&krow[x * ld + id[w]], s);
-                else atomic_add_f64(&Kg[(size_t)ia[x] * ld + id[w]], s);
+            for (int k = 0; k < NKET; ++k) {
+                double s = 0.0;
+#pragma unroll
+                for (int b = 0; b < NBRA; ++b) s += V[b * NKET + k] * Dab[b];
+                jv[k] = 2.0 * s;
+            }
+            // density blocks D_bd, D_bc, D_ad, D_ac
+            double Dbd[NB * ND], Dbc[NB * NC], Dad[NA * ND], Dac[NA * NC];
+#pragma unroll
+            for (int y = 0; y < NB; ++y) {
+#pragma unroll
+                for (int w = 0; w < ND; ++w) Dbd[y * ND + w] = D[(size_t)ib[y] * ld + id[w]];
+#pragma unroll
+                for (int z = 0; z < NC; ++z) Dbc[y * NC + z] = D[(size_t)ib[y] * ld + ic[z]];
+            }
+#pragma unroll
+            for (int x = 0; x < NA; ++x) {
+#pragma unroll
+                for (int w = 0; w < ND; ++w) Dad[x * ND + w] = D[(size_t)ia[x] * ld + id[w]];
+#pragma unroll
+                for (int z = 0; z < NC; ++z) Dac[x * NC + z] = D[(size_t)ia[x] * ld + ic[z]];
+            }
+#pragma unroll
+            for (int x = 0; x < NA; ++x) {
+#pragma unroll
+                for (int z = 0; z < NC; ++z) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int y = 0; y < NB; ++y)
+#pragma unroll
+                        for (int w = 0; w < ND; ++w) s += V[((x * NB + y) * NC + z) * ND + w] * Dbd[y * ND + w];
+                    kac[x * NC + z] = s;
+                }
+#pragma unroll
+                for (int w = 0; w < ND; ++w) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int y = 0; y < NB; ++y)
+#pragma unroll
+                        for (int z = 0; z < NC; ++z) s += V[((x * NB + y) * NC + z) * ND + w] * Dbc[y * NC + z];
+                    kad[x * ND + w] = s;
+                }
+            }
+#pragma unroll
+            for (int y = 0; y < NB; ++y) {
+#pragma unroll
+                for (int z = 0; z < NC; ++z) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int x = 0; x < NA; ++x)
+#pragma unroll
+                        for (int w = 0; w < ND; ++w) s += V[((x * NB + y) * NC + z) * ND + w] * Dad[x * ND + w];
+                    kbc[y * NC + z] = s;
+                }
+#pragma unroll
+                for (int w = 0; w < ND; ++w) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int x = 0; x < NA; ++x)
+#pragma unroll
+                        for (int z = 0; z < NC; ++z) s += V[((x * NB + y) * NC + z) * ND + w] * Dac[x * NC + z];
+                    kbd[y * ND + w] = s;
+                }
             }
         }
+        // this ket's J'_cd entry (zero when the quartet is skipped: every
+        // entry is rewritten by every build)
+        if (in) {
 #pragma unroll
-        for (int y = 0; y < NB; ++y) {
+            for (int k = 0; k < NKET; ++k) Jcd[entry * NKET + k] = jv[k];
+        }
+        // K' rows in LDS, the waves in turn
+        for (int w = 0; w < nwaves; ++w) {
+            if (w == wave && act) {
 #pragma unroll
-            for (int z = 0; z < NC; ++z) {
-                double s = 0.0;
+                for (int x = 0; x < NA; ++x) {
 #pragma unroll
-                for (int x = 0; x < NA; ++x)
+                    for (int z = 0; z < NC; ++z) atomic_add_lds(&krow[x * ldk + cc[z]], kac[x * NC + z]);
 #pragma unroll
-                    for (int w = 0; w < ND; ++w) s += V[((x * NB + y) * NC + z) * ND + w] * Dad[x * ND + w];
-                if (lds) atomic_add_f64(&krow[(NA + y) * ld + ic[z]], s);
-                else atomic_add_f64(&Kg[(size_t)ib[y] * ld + ic[z]], s);
+                    for (int v = 0; v < ND; ++v) atomic_add_lds(&krow[x * ldk + cd[v]], kad[x * ND + v]);
+                }
+#pragma unroll
+                for (int y = 0; y < NB; ++y) {
+#pragma unroll
+                    for (int z = 0; z < NC; ++z) atomic_add_lds(&krow[(NA + y) * ldk + cc[z]], kbc[y * NC + z]);
+#pragma unroll
+                    for (int v = 0; v < ND; ++v) atomic_add_lds(&krow[(NA + y) * ldk + cd[v]], kbd[y * ND + v]);
+                }
             }
-#pragma unroll
-            for (int w = 0; w < ND; ++w) {
-                double s = 0.0;
-#pragma unroll
-                for (int x = 0; x < NA; ++x)
-#pragma unroll
-                    for (int z = 0; z < NC; ++z) s += V[((x * NB + y) * NC + z) * ND + w] * Dac[x * NC + z];
-                if (lds) atomic_add_f64(&krow[(NA + y) * ld + id[w]], s);
-                else atomic_add_f64(&Kg[(size_t)ib[y] * ld + id[w]], s);
-            }
+            if (nwaves > 1) __syncthreads();
         }
     }
-    // J'_ab: wave reduction, one atomic per bra function pair per wave
+    // J'_ab: wave reduction (a fixed butterfly), then the waves in order
 #pragma unroll
     for (int b = 0; b < NBRA; ++b) {
         double v = jab[b];
 #pragma unroll
         for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-        if (lane == 0 && v != 0.0) atomic_add_f64(&a.J[(size_t)ia[b / NB] * ld + ib[b % NB]], 2.0 * v);
+        if (lane == 0) sred[wave * NBRA + b] = v;
     }
-    // flush the staged K rows (row by row: no index division)
-    if (lds) {
-        __syncthreads();
+    __syncthreads();
+    if (tid < NBRA) {
+        double v = 0.0;
+        for (int w = 0; w < nwaves; ++w) v += sred[w * NBRA + tid];
+        a.jab[(size_t)a.jab_slot[tix] * NBRA + tid] = 2.0 * v;
+    }
+    // the staged K' rows to the task's slot (row by row: no index division)
+    double* __restrict__ ks = a.kslot + (size_t)tix * NR * a.ncols;
 #pragma unroll
-        for (int r = 0; r < NA + NB; ++r) {
-            double* __restrict__ Kr = a.K + (size_t)(r < NA ? ia[r] : ib[r - NA]) * ld;
-            const double* sr = krow + r * ld;
-            for (int col = tid; col < ld; col += nthr) {
-                const double v = sr[col];
-                if (v != 0.0) atomic_add_f64(&Kr[col], v);
-            }
-        }
-    }
+    for (int r = 0; r < NR; ++r)
+        for (int col = tid; col < a.ncols; col += nthr) ks[(size_t)r * a.ncols + col] = krow[r * ldk + col];
 }
 
+// 0, or -3 when the workgroup's LDS exceeds 64 KB
 template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
-void launch_tiles(const QuartetArgs& a, hipStream_t s)
+int launch_tiles(const QuartetArgs& a, hipStream_t s)
 {
-    if (a.ntask <= 0) return;
-    constexpr int NR = ncart(LA) + ncart(LB);
-    const size_t shm = (a.lds_rows ? sizeof(double) * NR * (size_t)(a.n + 1) : 0) + hfk_bra_stage_bytes(LA, LB, a.bra_stage);
+    if (a.ntask <= 0) return 0;
+    const size_t shm = hfk_jk_lds_bytes(LA, LB, a.ncols, a.bra_stage);
+    if (shm > 64 * 1024) return -3;
     long long done = 0;
     while (done < a.ntask) {   // grid.x limit
         const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);
         QuartetArgs b = a;
         b.tasks = a.tasks + done;
+        b.task0 = a.task0 + done;
         hipLaunchKernelGGL((k_jk_tiles<LA, LB, LC, LD, T, BOYS, MODE>), dim3((unsigned)cnt), dim3(a.block), shm, s, b);
         done += cnt;
     }
+    return 0;
 }
 
 // op: HFK_OP_JK (mode 0 main, 1 correction), HFK_OP_FLAT, HFK_OP_DIAG.
@@ -348,15 +400,14 @@ int dispatch(int op, int prec, int boys, int mode, const QuartetArgs& a, hipStre
     if (op == HFK_OP_JK) {
 #define HFK_T(T)                                                                        \
         if (boys == 0) {                                                                  \
-            if (mode == 0) launch_tiles<LA, LB, LC, LD, T, 0, 0>(a, s);                   \
-            else launch_tiles<LA, LB, LC, LD, T, 0, 1>(a, s);                             \
-        } else {                                                                          \
-            if (mode == 0) launch_tiles<LA, LB, LC, LD, T, 1, 0>(a, s);                   \
-            else launch_tiles<LA, LB, LC, LD, T, 1, 1>(a, s);                             \
-        }
-        if (prec == 0) { HFK_T(double) } else { HFK_T(float) }
+            if (mode == 0) return launch_tiles<LA, LB, LC, LD, T, 0, 0>(a, s);            \
+            return launch_tiles<LA, LB, LC, LD, T, 0, 1>(a, s);                           \
+        }                                                                                 \
+        if (mode == 0) return launch_tiles<LA, LB, LC, LD, T, 1, 0>(a, s);                \
+        return launch_tiles<LA, LB, LC, LD, T, 1, 1>(a, s);
+        if (prec == 0) { HFK_T(double) }
+        HFK_T(float)
 #undef HFK_T
-        return 0;
     }
     if (prec != 0) return -1;
     if (op == HFK_OP_FLAT) {

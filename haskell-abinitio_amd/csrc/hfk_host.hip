@@ -326,8 +326,6 @@ int hfk_ctx_init_device(hfk_ctx* c)
     const size_t nf = (size_t)c->n * (c->n + 1) / 2;
     const size_t np1 = (size_t)(c->n + 1) * (c->n + 1);   // padded: zero row/column n
     HFK_TRY(dev_alloc(&c->d_D, np1));
-    HFK_TRY(dev_alloc(&c->d_J, np1));
-    HFK_TRY(dev_alloc(&c->d_K, np1));
     HFK_TRY(dev_alloc(&c->d_flatA, nf));
     HFK_TRY(dev_alloc(&c->d_flatB, nf));
     HFK_TRY(dev_alloc(&c->d_flatC, nf));
@@ -517,16 +515,41 @@ void hfk_pair_class_l(int k, int* la, int* lb)
     *lb = k - a * (a + 1) / 2;
 }
 
+// Device allocations of one plan entry (freed when the plan is rebuilt).
+template <typename T>
+static int entry_upload(hfk_ctx* c, PlanEntry& e, T** p, const std::vector<T>& v)
+{
+    *p = nullptr;
+    int rc = dev_upload(c, p, v);
+    if (rc == HFK_OK) e.owned.push_back((void*)*p);
+    return rc;
+}
+template <typename T>
+static int entry_alloc(hfk_ctx* c, PlanEntry& e, T** p, size_t n)
+{
+    *p = nullptr;
+    int rc = dev_alloc(p, n);
+    if (rc == HFK_OK) e.owned.push_back((void*)*p);
+    return rc;
+}
+
+static void free_plan(hfk_ctx* c)
+{
+    for (PlanEntry& e : c->plan)
+        for (void* p : e.owned) dev_free(c, p);
+    c->plan.clear();
+    if (c->d_kpart) dev_free(c, c->d_kpart);
+    if (c->d_jpart) dev_free(c, c->d_jpart);
+    c->d_kpart = c->d_jpart = nullptr;
+    c->slot_bytes = 0;
+}
+
 // The launch plan: every class pair (bra class >= ket class), split into an
 // FP64 and an FP32 launch when the mixed-precision threshold is set.
 int hfk_ctx_plan(hfk_ctx* c)
 {
     if (const char* v = getenv("HFK_KETS_PER_LANE")) c->kets_per_lane = std::max(1, atoi(v));
-    for (PlanEntry& e : c->plan) {
-        if (e.d_tasks) dev_free(c, e.d_tasks);
-        if (e.d_runs) dev_free(c, e.d_runs);
-    }
-    c->plan.clear();
+    free_plan(c);
     const bool ref = c->opt.screen_mode == HFK_SCREEN_REFERENCE && c->opt.screen_tau > 0.0;
     for (int kb = 0; kb < HFK_NPC; ++kb) {
         if (c->class_count[kb] == 0) continue;
@@ -555,31 +578,48 @@ int hfk_ctx_plan(hfk_ctx* c)
             }
         }
     }
+    const int nshell = (int)c->shells.size();
     // Task lists of the J/K (and gradient) kernels.  Class lists are ordered
     // by primitive-pair count, then by descending screen value, so for each
     // bra pair the kets surviving the pair screen are a prefix of each
     // segment of equal primitive-pair count.  A task is a bra pair, a list of
     // runs (such prefixes, as position ranges in the ket class list) and
     // primitive ranges of bra and ket.  Lanes walk the runs in order, so a
-    // wave mostly meets one primitive-pair count.  Every task flushes the K
-    // rows it staged once, so when the launch has enough bra pairs to fill
-    // the GPU a bra pair keeps up to 1024 kets in one task.
+    // wave mostly meets one primitive-pair count.
+    //
+    // Slots.  Each task stores its K' rows (the bra shells' rows over the
+    // launch's compact columns) and its J'_ab; each (bra, ket, primitive
+    // piece) stores its J'_cd in an entry laid out by bra-list order: bra i
+    // and ket segment s own the entries jbase[i][s] + piece * len + (p -
+    // first) for its kets p in [first, jend[i][s]).  That layout lets the
+    // J'_cd reduction walk, for a ket position, the bras that cover it
+    // without any per-quartet index.
     for (PlanEntry& e : c->plan) {
         const bool coop = hfk_class_coop(e.la, e.lb, e.lc, e.ld);
         const int block = hfk_plan_block(c, e);
-        // cooperative kernel: 64 / (na nb) kets at a time, four rounds per task
-        const int chunk = coop ? 4 * (64 / (ncart(e.la) * ncart(e.lb))) : block * c->kets_per_lane;
+        const int NA = ncart(e.la), NB = ncart(e.lb), NC = ncart(e.lc), ND = ncart(e.ld);
+        const int nrow = NA + NB, nbra_comp = NA * NB, nket_comp = NC * ND;
+        // cooperative kernel: S = 64 / (na nb) kets at a time
+        const int coop_s = std::max(1, 64 / nbra_comp);
+        // kets per task of the primitive-split (heavy) segments: every task
+        // stores its K' rows, so large systems take longer ones
+        const int chunk = coop ? 16 * coop_s : block * c->kets_per_lane * (c->n >= 200 ? 16 : 1);
         const std::vector<int>& bl = c->h_class[e.kb];
         const std::vector<int>& kl = c->h_class[e.kk];
         std::vector<double> kq(kl.size());
         for (size_t j = 0; j < kl.size(); ++j) kq[j] = c->h_pair_qhi[kl[j]];
-        std::vector<std::pair<long long, long long>> segs;   // segments of equal primitive-pair count
-        for (size_t j = 0; j < kl.size();) {
-            size_t k = j + 1;
-            while (k < kl.size() && c->h_pair_npp[kl[k]] == c->h_pair_npp[kl[j]]) ++k;
-            segs.push_back({(long long)j, (long long)k});
-            j = k;
-        }
+        auto segments = [&](const std::vector<int>& lst) {
+            std::vector<std::pair<long long, long long>> out;   // runs of equal primitive-pair count
+            for (size_t j = 0; j < lst.size();) {
+                size_t k = j + 1;
+                while (k < lst.size() && c->h_pair_npp[lst[k]] == c->h_pair_npp[lst[j]]) ++k;
+                out.push_back({(long long)j, (long long)k});
+                j = k;
+            }
+            return out;
+        };
+        const auto segs = segments(kl), bsegs = segments(bl);
+        const int nkseg = (int)segs.size(), nbseg = (int)bsegs.size();
         // surviving kets of bra i in segment sg: [sg.first, hi) by binary search
         auto staircase = [&](size_t i, const std::pair<long long, long long>& sg) -> long long {
             const double qi = c->h_pair_qhi[bl[i]];
@@ -596,36 +636,85 @@ int hfk_ctx_plan(hfk_ctx* c)
             }
             return lo;
         };
-        // primitive quartets of the surviving shell quartets (before the
-        // primitive-level screen): sizes the primitive-range split below
+        // staircase ends, bra-major [i][kseg]
+        std::vector<int> jend(bl.size() * (size_t)nkseg);
         long long est_prim = 0;
         for (size_t i = 0; i < bl.size(); ++i)
-            for (const auto& sg : segs)
-                est_prim += (long long)c->h_pair_npp[bl[i]] * c->h_pair_npp[kl[sg.first]] * (staircase(i, sg) - sg.first);
+            for (int s = 0; s < nkseg; ++s) {
+                const long long end = staircase(i, segs[s]);
+                jend[i * nkseg + s] = (int)end;
+                est_prim += (long long)c->h_pair_npp[bl[i]] * c->h_pair_npp[kl[segs[s].first]] * (end - segs[s].first);
+            }
         // Primitive ranges: a lane's share of a task is about per_lane primitive
         // quartets (at most 128), sized from the launch's total so that small
-        // launches still fill the GPU (~1.3e5 lanes of work; with the
-        // costliest-first task order this beat 2.6e5: fewer, longer pieces and
-        // fewer partial digestions) and large ones keep quartets whole.  Bra
-        // primitives are wave-uniform (LDS broadcast), so a lane keeps as many
-        // of them as fit and the ket range is split.  Segments that need no
-        // split join the long multi-run tasks.
+        // launches still fill the GPU (~1.3e5 lanes of work) and large ones
+        // keep quartets whole.  Bra primitives are wave-uniform (LDS
+        // broadcast), so a lane keeps as many of them as fit and the ket range
+        // is split.  Segments that need no split join the long multi-run tasks.
         const int L = e.la + e.lb + e.lc + e.ld;
         const long long floor_l = std::max(4, 32 >> L);   // cheap classes: amortise the digestion
         const int per_lane = (int)std::max(floor_l, std::min(128LL, est_prim / 131072));
         const bool single = coop || hfk_class_grad_coop(e.la, e.lb, e.lc, e.ld);   // cooperative kernels read these
         auto whole = [&](int nbp, int nkp) { return !single && nbp <= HFK_BRA_STAGE && nbp * nkp <= per_lane; };
+        auto steps = [&](int nbp, int nkp, int& bstep, int& kstep) {
+            bstep = nbp;
+            kstep = nkp;
+            if (!coop && nbp * nkp > per_lane) {
+                bstep = std::max(1, std::min(nbp, per_lane));
+                kstep = std::max(1, std::min(nkp, per_lane / bstep));
+            }
+            if (!coop) bstep = std::min(bstep, HFK_BRA_STAGE);   // staged in LDS by the kernel
+        };
+        // pieces per (bra segment, ket segment) and the J'_cd entry layout
+        std::vector<int> npiece((size_t)nbseg * nkseg, 1);
+        for (int b = 0; b < nbseg; ++b)
+            for (int s = 0; s < nkseg; ++s) {
+                const int nbp = c->h_pair_npp[bl[bsegs[b].first]], nkp = c->h_pair_npp[kl[segs[s].first]];
+                if (whole(nbp, nkp)) continue;
+                int bstep, kstep;
+                steps(nbp, nkp, bstep, kstep);
+                npiece[(size_t)b * nkseg + s] = ((nbp + bstep - 1) / bstep) * ((nkp + kstep - 1) / kstep);
+            }
+        std::vector<int> bseg_of(bl.size());
+        for (int b = 0; b < nbseg; ++b)
+            for (long long i = bsegs[b].first; i < bsegs[b].second; ++i) bseg_of[i] = b;
+        std::vector<long long> jbase(bl.size() * (size_t)nkseg);
+        long long njcd = 0;
+        for (size_t i = 0; i < bl.size(); ++i)
+            for (int s = 0; s < nkseg; ++s) {
+                jbase[i * nkseg + s] = njcd;
+                njcd += (long long)npiece[(size_t)bseg_of[i] * nkseg + s] * (jend[i * nkseg + s] - segs[s].first);
+            }
+        e.njcd = njcd;
+        // compact K columns: the functions of the ket class's shells
+        std::vector<int> colmap(c->n + 1, -1), collist;
+        for (int S = 0; S < nshell; ++S) {
+            const HostShell& h = c->shells[S];
+            if (h.L != e.lc && h.L != e.ld) continue;
+            for (int x = 0; x < HFK_MAXCOMP; ++x)
+                if (h.bf[x] >= 0) colmap[h.bf[x]] = 1;
+        }
+        for (int f = 0; f < c->n; ++f)
+            if (colmap[f] > 0) {
+                colmap[f] = (int)collist.size();
+                collist.push_back(f);
+            }
+        e.ncols = (int)collist.size();
+        for (int f = 0; f <= c->n; ++f)
+            if (colmap[f] < 0) colmap[f] = e.ncols;   // never a ket column (or the padding): discarded
         long long merged_kets = 0;
         for (size_t i = 0; i < bl.size(); ++i)
-            for (const auto& sg : segs)
-                if (whole(c->h_pair_npp[bl[i]], c->h_pair_npp[kl[sg.first]])) merged_kets += staircase(i, sg) - sg.first;
-        // long tasks: enough of them for ~40960 waves in flight, and at most
-        // 1024 kets (longer ones leave a few workgroups running alone at the
-        // end of the launch: measured slower from 4096 kets up)
+            for (int s = 0; s < nkseg; ++s)
+                if (whole(c->h_pair_npp[bl[i]], c->h_pair_npp[kl[segs[s].first]]))
+                    merged_kets += jend[i * nkseg + s] - segs[s].first;
+        // long tasks: about 40960 waves of work per launch, at least 256 and at
+        // most 8192 kets each (each task stores its K' rows once, so longer
+        // tasks store less; the costliest-first order keeps the tail short)
         const long long target = std::max(1, 40960 * 64 / block);
-        const long long cmerge = std::max<long long>(chunk, std::min<long long>(1024, (merged_kets + target - 1) / target));
+        const long long cmerge = std::max<long long>(std::max(chunk, 128),
+                                                     std::min<long long>(8192, (merged_kets + target - 1) / target));
         std::vector<int4> tasks;
-        std::vector<int2> runs;
+        std::vector<KetRun> runs;
         std::vector<long long> cost;   // estimated primitive quartets per task
         long long cost_acc = 0;
         auto push = [&](int bp, long long r0, int w, int len) {
@@ -636,19 +725,27 @@ int hfk_ctx_plan(hfk_ctx* c)
             e.task_len_max = std::max(e.task_len_max, len);
             return true;
         };
+        auto run_of = [](long long x, long long y, long long jcd) {
+            KetRun r;
+            r.x = (int)x;
+            r.y = (int)y;
+            r.jcd = jcd;
+            return r;
+        };
         for (size_t i = 0; i < bl.size(); ++i) {
             const int nbp = c->h_pair_npp[bl[i]];
             // kets of the unsplit segments, as one list of runs cut every cmerge kets
             long long r0 = (long long)runs.size();
             int len = 0;
-            for (const auto& sg : segs) {
+            for (int s = 0; s < nkseg; ++s) {
+                const auto& sg = segs[s];
                 const int nkp = c->h_pair_npp[kl[sg.first]];
                 if (!whole(nbp, nkp)) continue;
                 long long x = sg.first;
-                const long long x1 = staircase(i, sg);
+                const long long x1 = jend[i * nkseg + s];
                 while (x < x1) {
                     const long long take = std::min<long long>(x1 - x, cmerge - len);
-                    runs.push_back(make_int2((int)x, (int)(x + take)));
+                    runs.push_back(run_of(x, x + take, jbase[i * nkseg + s] + (x - sg.first)));
                     cost_acc += take * nbp * nkp;
                     x += take;
                     len += (int)take;
@@ -662,26 +759,24 @@ int hfk_ctx_plan(hfk_ctx* c)
             if (len > 0 && !push(bl[i], r0, (nbp << 16) | 0xff, len)) return hfk_set_error(HFK_EUNSUP, "task list too long");
             if (nbp > 0) e.bra_stage = std::max(e.bra_stage, coop ? nbp : std::min(nbp, HFK_BRA_STAGE));
             // segments split into primitive ranges: one run per task
-            for (const auto& sg : segs) {
+            for (int s = 0; s < nkseg; ++s) {
+                const auto& sg = segs[s];
                 const int nkp = c->h_pair_npp[kl[sg.first]];
                 if (whole(nbp, nkp)) continue;
-                const long long k0 = sg.first, k1 = staircase(i, sg);
+                const long long k0 = sg.first, k1 = jend[i * nkseg + s];
                 if (k1 <= k0) continue;
-                int bstep = nbp, kstep = nkp;
-                if (!coop && nbp * nkp > per_lane) {
-                    bstep = std::max(1, std::min(nbp, per_lane));
-                    kstep = std::max(1, std::min(nkp, per_lane / bstep));
-                }
-                if (!coop) bstep = std::min(bstep, HFK_BRA_STAGE);   // staged in LDS by the kernel
+                int bstep, kstep;
+                steps(nbp, nkp, bstep, kstep);
+                int piece = 0;   // pieces in (bra range, ket range) order, as npiece counts them
                 for (int b0 = 0; b0 < nbp; b0 += bstep) {
                     const int b1 = std::min(nbp, b0 + bstep);
-                    for (int q0 = 0; q0 < nkp; q0 += kstep) {
+                    for (int q0 = 0; q0 < nkp; q0 += kstep, ++piece) {
                         const int q1 = std::min(nkp, q0 + kstep);
                         const int w = (b0 << 24) | (b1 << 16) | (q0 << 8) | q1;
                         for (long long j0 = k0; j0 < k1; j0 += chunk) {
                             const long long j1 = std::min<long long>(k1, j0 + chunk);
                             const long long rr = (long long)runs.size();
-                            runs.push_back(make_int2((int)j0, (int)j1));
+                            runs.push_back(run_of(j0, j1, jbase[i * nkseg + s] + (long long)piece * (k1 - k0) + (j0 - k0)));
                             cost_acc = (j1 - j0) * (long long)(b1 - b0) * (q1 - q0);
                             if (!push(bl[i], rr, w, (int)(j1 - j0))) return hfk_set_error(HFK_EUNSUP, "task list too long");
                         }
@@ -711,49 +806,143 @@ int hfk_ctx_plan(hfk_ctx* c)
             tasks.swap(sorted);
         }
         e.ntask = (long long)tasks.size();
-        e.d_tasks = nullptr;
-        e.d_runs = nullptr;
+        // LDS of one workgroup: the K' rows must fit
+        {
+            const size_t lds = coop ? hfk_coop_jk_lds_bytes(e.la, e.lb, e.ncols) + 16 * 1024
+                                    : hfk_jk_lds_bytes(e.la, e.lb, e.ncols, e.bra_stage);
+            if (lds > 64 * 1024 || e.ncols > 4096)
+                return hfk_set_error(HFK_EUNSUP,
+                                     "(%d%d|%d%d): K rows of %d columns need %zu bytes of LDS per workgroup (at most 64 KB)",
+                                     e.la, e.lb, e.lc, e.ld, e.ncols, lds);
+        }
+        // reduction lists: the tasks of each bra shell (K' rows) and of each bra pair (J'_ab), in task order
+        std::vector<int> pos_of(c->npair, -1);
+        for (size_t i = 0; i < bl.size(); ++i) pos_of[bl[i]] = (int)i;
+        std::vector<int> koff(nshell + 1, 0), boff(bl.size() + 1, 0);
+        for (const int4& t : tasks) {
+            ++koff[c->h_pair_A[t.x] + 1];
+            ++koff[c->h_pair_B[t.x] + 1];
+            ++boff[pos_of[t.x] + 1];
+        }
+        for (int S = 0; S < nshell; ++S) koff[S + 1] += koff[S];
+        for (size_t i = 0; i < bl.size(); ++i) boff[i + 1] += boff[i];
+        std::vector<int2> kcsr(koff[nshell]);
+        std::vector<int> jab_slot(tasks.size());   // the tasks of one bra pair: consecutive J'_ab slots
+        {
+            std::vector<int> kfill(koff.begin(), koff.end() - 1), bfill(boff.begin(), boff.end() - 1);
+            for (size_t t = 0; t < tasks.size(); ++t) {
+                const int bp = tasks[t].x;
+                kcsr[kfill[c->h_pair_A[bp]]++] = make_int2((int)t, 0);
+                kcsr[kfill[c->h_pair_B[bp]]++] = make_int2((int)t, NA);
+                jab_slot[t] = bfill[pos_of[bp]]++;
+            }
+        }
+        if (e.ntask > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "too many tasks (%lld)", e.ntask);
         e.d_kets = c->d_class[e.kk];   // runs index the ket class list itself (not owned)
         if (!tasks.empty()) {
-            HFK_TRY(dev_upload(c, &e.d_tasks, tasks));
-            HFK_TRY(dev_upload(c, &e.d_runs, runs));
+            HFK_TRY(entry_upload(c, e, &e.d_tasks, tasks));
+            HFK_TRY(entry_upload(c, e, &e.d_runs, runs));
+        }
+        HFK_TRY(entry_upload(c, e, &e.d_colmap, colmap));
+        HFK_TRY(entry_upload(c, e, &e.d_collist, collist));
+        HFK_TRY(entry_upload(c, e, &e.d_kcsr_off, koff));
+        HFK_TRY(entry_upload(c, e, &e.d_kcsr, kcsr));
+        HFK_TRY(entry_upload(c, e, &e.d_bcsr_off, boff));
+        HFK_TRY(entry_upload(c, e, &e.d_jab_slot, jab_slot));
+        // J'_cd layout; the first reduction pass works on chunks of
+        // consecutive bras of one bra segment (enough chunks to fill the GPU
+        // for small launches, at most ~128 plus one per segment for large ones)
+        {
+            std::vector<int> pos_kseg(kl.size()), kfirst(nkseg + 1);
+            for (int s = 0; s < nkseg; ++s) {
+                kfirst[s] = (int)segs[s].first;
+                for (long long j = segs[s].first; j < segs[s].second; ++j) pos_kseg[j] = s;
+            }
+            kfirst[nkseg] = (int)kl.size();
+            const long long ch = std::max<long long>(16, ((long long)bl.size() + 127) / 128);
+            std::vector<int4> chunks;
+            for (int b = 0; b < nbseg; ++b)
+                for (long long i0 = bsegs[b].first; i0 < bsegs[b].second; i0 += ch)
+                    chunks.push_back(make_int4(b, (int)i0, (int)std::min<long long>(bsegs[b].second, i0 + ch), 0));
+            // tiles of ptile consecutive positions within one ket segment
+            const int ptile = std::max(1, 256 / nket_comp);
+            std::vector<int2> tiles;
+            for (int s = 0; s < nkseg; ++s)
+                for (long long t0 = segs[s].first; t0 < segs[s].second; t0 += ptile) tiles.push_back(make_int2(s, (int)t0));
+            JcdLayout& J = e.jl;
+            memset(&J, 0, sizeof J);
+            int2* d_tl = nullptr;
+            HFK_TRY(entry_upload(c, e, &d_tl, tiles));
+            J.tiles = d_tl;
+            J.ntile = (int)tiles.size();
+            J.ptile = ptile;
+            int *d_pk = nullptr, *d_kf = nullptr, *d_je = nullptr, *d_np = nullptr;
+            int4* d_ch = nullptr;
+            long long* d_jb = nullptr;
+            HFK_TRY(entry_upload(c, e, &d_pk, pos_kseg));
+            HFK_TRY(entry_upload(c, e, &d_kf, kfirst));
+            HFK_TRY(entry_upload(c, e, &d_ch, chunks));
+            HFK_TRY(entry_upload(c, e, &d_je, jend));
+            HFK_TRY(entry_upload(c, e, &d_jb, jbase));
+            HFK_TRY(entry_upload(c, e, &d_np, npiece));
+            J.pos_kseg = d_pk;
+            J.kseg_first = d_kf;
+            J.chunks = d_ch;
+            J.jend = d_je;
+            J.jbase = d_jb;
+            J.npiece = d_np;
+            J.nket = (int)kl.size();
+            J.nkseg = nkseg;
+            J.nchunk = (int)chunks.size();
+            J.nket_comp = nket_comp;
+            J.same = e.kb == e.kk;
+            HFK_TRY(entry_alloc(c, e, &J.part, std::max<size_t>(1, chunks.size() * kl.size() * (size_t)nket_comp)));
+        }
+        // the slots
+        HFK_TRY(entry_alloc(c, e, &e.d_kslot, std::max<size_t>(1, (size_t)e.ntask * nrow * e.ncols)));
+        HFK_TRY(entry_alloc(c, e, &e.d_jab, std::max<size_t>(1, (size_t)e.ntask * nbra_comp)));
+        HFK_TRY(entry_alloc(c, e, &e.d_jcd, std::max<size_t>(1, (size_t)njcd * nket_comp)));
+        HFK_TRY(entry_alloc(c, e, &e.d_jk, std::max<size_t>(1, kl.size() * (size_t)nket_comp)));
+        e.jl.jcd = e.d_jcd;
+        e.jl.out = e.d_jk;
+        e.slot_bytes = sizeof(double) * ((size_t)e.ntask * (nrow * e.ncols + nbra_comp) + ((size_t)njcd + kl.size()) * nket_comp);
+        c->slot_bytes += e.slot_bytes;
+    }
+    // the launches' partial J', K'
+    {
+        const size_t np1 = (size_t)(c->n + 1) * (c->n + 1), nl = std::max<size_t>(1, c->plan.size());
+        HFK_TRY(dev_alloc(&c->d_kpart, nl * np1));
+        HFK_TRY(dev_alloc(&c->d_jpart, nl * np1));
+        for (size_t k = 0; k < c->plan.size(); ++k) {
+            PlanEntry& e = c->plan[k];
+            JKPart& P = e.jkp;
+            memset(&P, 0, sizeof P);
+            P.kslot = e.d_kslot;
+            P.csr_off = e.d_kcsr_off;
+            P.csr = e.d_kcsr;
+            P.collist = e.d_collist;
+            P.ncols = e.ncols;
+            P.nrow = ncart(e.la) + ncart(e.lb);
+            P.nshell = nshell;
+            P.jab = e.d_jab;
+            P.bcsr_off = e.d_bcsr_off;
+            P.jk = e.d_jk;
+            P.bra_list = c->d_class[e.kb];
+            P.ket_list = c->d_class[e.kk];
+            P.nbra = (int)c->class_count[e.kb];
+            P.nket = (int)c->class_count[e.kk];
+            P.same = e.kb == e.kk;
+            P.na = ncart(e.la);
+            P.nb = ncart(e.lb);
+            P.nc = ncart(e.lc);
+            P.nd = ncart(e.ld);
+            P.Kpart = c->d_kpart + k * np1;
+            P.Jpart = c->d_jpart + k * np1;
         }
     }
     c->counted = false;   // exact counts on first request (hfk_ctx_count)
     c->first_launch = -1;
     c->serial_runs = 0;
-    // ket-side J'_cd sums: one region per launch, [copies][nket][NKET]
-    {
-        size_t per = 0;
-        for (const PlanEntry& e : c->plan) per += (size_t)c->class_count[e.kk] * ncart(e.lc) * ncart(e.ld);
-        c->jk_copies = (int)std::max<size_t>(1, std::min<size_t>(16, per ? ((size_t)64 << 20) / (per * sizeof(double)) : 1));
-        size_t off = 0;
-        for (PlanEntry& e : c->plan) {
-            e.jk_off = off;
-            off += (size_t)c->jk_copies * c->class_count[e.kk] * ncart(e.lc) * ncart(e.ld);
-        }
-        if (c->d_Jk) dev_free(c, c->d_Jk);
-        c->d_Jk = nullptr;
-        c->jk_len = off;
-        HFK_TRY(dev_alloc(&c->d_Jk, std::max<size_t>(off, 1)));
-        std::vector<JkScatter> ent;
-        c->jk_max_count = 0;
-        for (const PlanEntry& e : c->plan) {
-            if (c->class_count[e.kk] == 0) continue;
-            JkScatter j;
-            j.ket_list = c->d_class[e.kk];
-            j.nket = c->class_count[e.kk];
-            j.off = (long long)e.jk_off;
-            j.nc = ncart(e.lc);
-            j.nd = ncart(e.ld);
-            ent.push_back(j);
-            c->jk_max_count = std::max(c->jk_max_count, j.nket * j.nc * j.nd);
-        }
-        if (c->d_jk_ent) dev_free(c, c->d_jk_ent);
-        c->d_jk_ent = nullptr;
-        c->jk_nent = (int)ent.size();
-        if (!ent.empty()) HFK_TRY(dev_upload(c, &c->d_jk_ent, ent));
-    }
     const size_t np = c->plan.size();
     // event pool for per-launch timing
     while (c->events.size() < 2 * np + 2) {
@@ -770,16 +959,16 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
     a.runs = e.d_runs;
     a.ket_idx = e.d_kets;
     a.ntask = e.ntask;
+    a.task0 = 0;
     a.bra_stage = e.bra_stage;
     a.task_len_max = e.task_len_max;
     a.block = hfk_plan_block(c, e);
-    {
-        const int nr = (e.la + 1) * (e.la + 2) / 2 + (e.lb + 1) * (e.lb + 2) / 2;
-        // the cooperative kernel also holds ~12 KB of static LDS
-        const size_t cap = hfk_class_coop(e.la, e.lb, e.lc, e.ld) ? 48 * 1024
-                                                                  : 64 * 1024 - hfk_bra_stage_bytes(e.la, e.lb, e.bra_stage);
-        a.lds_rows = ((size_t)nr * (c->n + 1) * sizeof(double) <= cap) ? 1 : 0;
-    }
+    a.kslot = e.d_kslot;
+    a.jab = e.d_jab;
+    a.jab_slot = e.d_jab_slot;
+    a.jcd = e.d_jcd;
+    a.colmap = e.d_colmap;
+    a.ncols = e.ncols;
     a.bra_list = c->d_class[e.kb];
     a.ket_list = c->d_class[e.kk];
     a.nbra = c->class_count[e.kb];
@@ -793,26 +982,25 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
 // Run the plan for one operation (J/K or flat export); per-launch HIP events.
 int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
 {
-    if (op == HFK_OP_JK && (!c->d_Jk || !a.J || !a.K || !a.D))
-        return hfk_set_error(HFK_EINVAL, "J/K launch without its accumulators");
+    if (op == HFK_OP_JK && !a.D) return hfk_set_error(HFK_EINVAL, "J/K launch without a density");
     long long total = 0;
     const size_t np = c->plan.size();
-    // J/K launches only accumulate (atomically) into J', K' and their own
-    // ket-side sums, so they may run concurrently.  The longest launch (as
-    // measured in the context's second J/K run; the first two are serial and
-    // the first pays one-time costs) runs first,
+    // J/K launches write only their own slots, so they may run concurrently.
+    // The longest launch (as measured in the context's second J/K run; the
+    // first two are serial and the first pays one-time costs) runs first,
     // alone on the context stream: it fills the GPU by itself and keeps a
     // clean time.  The others then run round-robin on the auxiliary streams,
     // forked from and joined back into the context stream, and fill each
-    // other's tails.
+    // other's tails.  Each J/K launch is followed on its stream by the
+    // fixed-order reduction of its J'_cd entries.
     const bool conc = op == HFK_OP_JK && c->concurrent && np > 2 && c->first_launch >= 0;
     const size_t first = conc ? (size_t)c->first_launch : 0;
     std::vector<size_t> order;
     if (conc) order.push_back(first);
     for (size_t i = 0; i < np; ++i)
         if (!conc || i != first) order.push_back(i);
-    int slot = 0, missing = -1;
-    for (size_t oi = 0; oi < order.size() && missing < 0; ++oi) {
+    int slot = 0, missing = -1, lds_fail = -1;
+    for (size_t oi = 0; oi < order.size() && missing < 0 && lds_fail < 0; ++oi) {
         const size_t i = order[oi];
         const PlanEntry& e = c->plan[i];
         if (conc && oi == 1) {
@@ -822,23 +1010,30 @@ int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
         if (op != HFK_OP_JK && e.prec == 1) continue;   // exports are FP64
         hfk_plan_args(c, e, a);
         if (op != HFK_OP_JK) a.prec_sel = 0;
-        if (op == HFK_OP_JK) {
-            a.Jk = c->d_Jk + e.jk_off;
-            a.jk_copies = c->jk_copies;
-        }
         total += e.nquart;
         hipStream_t s = (conc && oi > 0) ? c->aux[slot++ % hfk_ctx::NAUX] : c->stream;
         HFK_HIP(hipEventRecord(c->events[2 * i], s));
-        if (hfk_launch_quartets(e.la, e.lb, e.lc, e.ld, op, e.prec, c->opt.boys_mode, op == HFK_OP_JK ? e.mode : 0, a, s))
-            missing = (int)i;
+        const int rc = hfk_launch_quartets(e.la, e.lb, e.lc, e.ld, op, e.prec, c->opt.boys_mode,
+                                           op == HFK_OP_JK ? e.mode : 0, a, s);
+        if (rc == -3) lds_fail = (int)i;
+        else if (rc) missing = (int)i;
         c->plan[i].solo = (!conc || oi == 0) ? 1 : 0;
         HFK_HIP(hipEventRecord(c->events[2 * i + 1], s));
+        if (op == HFK_OP_JK && rc == 0) {
+            hfk_launch_reduce_jcd(e.jl, s);
+            hfk_launch_jk_part(e.jkp, c->n, c->d_sh_bf, c->d_sh_bfp, c->d_pair_A, c->d_pair_B, s);
+        }
     }
     if (conc)
         for (int k = 0; k < hfk_ctx::NAUX; ++k) {
             HFK_HIP(hipEventRecord(c->ev_join[k], c->aux[k]));
             HFK_HIP(hipStreamWaitEvent(c->stream, c->ev_join[k], 0));
         }
+    if (lds_fail >= 0) {
+        const PlanEntry& e = c->plan[lds_fail];
+        return hfk_set_error(HFK_EUNSUP, "(%d%d|%d%d): the K rows of %d columns do not fit in LDS", e.la, e.lb, e.lc,
+                             e.ld, e.ncols);
+    }
     if (missing >= 0) {
         const PlanEntry& e = c->plan[missing];
         return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", e.la, e.lb, e.lc, e.ld);
@@ -932,20 +1127,17 @@ extern "C" int hfk_get_class_stats(hfk_ctx* c, hfk_class_stat* out, int32_t cap,
 // Device Fock build: G (or H+G) flat from a flat density, all on device.
 int hfk_ctx_fock_device(hfk_ctx* c, const double* H_dev, const double* D_dev_flat, double* F_dev_flat, double extra_cut)
 {
-    // timed from here: the set-up launch (D padded, J', K' and the ket-side
-    // sums cleared), the class launches and the scatter of the ket-side sums
+    // timed from here: the set-up launch (padded D), the class launches, each
+    // followed on its stream by the fixed-order reductions of its slots into
+    // its partial J', K', and the sum of the partials into G
     HFK_HIP(hipEventRecord(c->ev0, c->stream));
-    hfk_launch_fock_prepare(c->n, D_dev_flat, c->d_D, c->d_J, c->d_K, c->d_Jk, (long long)c->jk_len, c->stream);
+    hfk_launch_fock_prepare(c->n, D_dev_flat, c->d_D, c->stream);
     QuartetArgs a = hfk_ctx_args(c);
     a.sch_cut = std::max(c->opt.schwarz_cut, extra_cut);
     a.D = c->d_D;
-    a.J = c->d_J;
-    a.K = c->d_K;
     HFK_TRY(hfk_ctx_run_quartets(c, a, HFK_OP_JK));
-    hfk_launch_scatter_jk(c->d_J, c->d_Jk, c->jk_copies, c->d_jk_ent, c->jk_nent, c->jk_max_count, c->d_pair_A,
-                          c->d_pair_B, c->d_sh_bfp, c->n + 1, c->stream);
+    hfk_launch_symmetrize_parts(c->n, c->d_jpart, c->d_kpart, (int)c->plan.size(), H_dev, F_dev_flat, c->stream);
     HFK_HIP(hipEventRecord(c->ev1, c->stream));
-    hfk_launch_symmetrize_g(c->n, c->d_J, c->d_K, H_dev, F_dev_flat, nullptr, c->stream);
     HFK_HIP(hipGetLastError());
     HFK_HIP(hipStreamSynchronize(c->stream));
     float ms = 0.f;

@@ -45,13 +45,29 @@ struct PlanEntry {
     double ms;                  // last timed duration
     int mode;                   // 0 main pass, 1 screen-correction pass (shares the main tasks)
     int4* d_tasks;              // tasks {bra pair, first run, end run, primitive ranges}
-    int2* d_runs;               // runs [begin, end) of positions in d_kets
+    KetRun* d_runs;             // runs [begin, end) of positions in d_kets, with their J'_cd entries
     int* d_kets;                // ket pair indices the tasks refer to (the ket class list)
     long long ntask;
     int bra_stage;              // longest bra primitive-pair range of the tasks
-    int task_len_max;           // longest ket run of the tasks
-    size_t jk_off;              // its ket-side J'_cd sums in hfk_ctx::d_Jk
+    int task_len_max;           // longest ket sequence of the tasks
     int solo;                   // its last timed run had the GPU to itself
+    // J/K slots (written by the launch with plain stores) and their layout
+    int ncols;                  // compact K columns: functions of the ket class's shells
+    int* d_colmap;              // [n+1] function -> column (n -> ncols)
+    int* d_collist;             // [ncols]
+    double* d_kslot;            // [ntask][NA+NB][ncols]
+    double* d_jab;              // [ntask][NA*NB]
+    double* d_jcd;              // [njcd][NC*ND]
+    double* d_jk;               // [nket][NC*ND]: reduced ket side
+    long long njcd;
+    JcdLayout jl;               // device pointers of the J'_cd layout
+    JKPart jkp;                 // its partial J', K' (reduced on its stream after the launch)
+    int* d_kcsr_off;            // [nshell+1]
+    int2* d_kcsr;               // {task, row offset}
+    int* d_bcsr_off;            // [nbra+1]: J'_ab slots of each bra position
+    int* d_jab_slot;            // [ntask]
+    size_t slot_bytes;          // device memory of the slots
+    std::vector<void*> owned;   // device allocations of this entry
 };
 
 struct hfk_ctx {
@@ -102,13 +118,11 @@ struct hfk_ctx {
     int *d_sh_L = nullptr, *d_sh_prim0 = nullptr, *d_sh_nprim = nullptr, *d_sh_bf = nullptr, *d_sh_bfp = nullptr;
     int *d_pair_A = nullptr, *d_pair_B = nullptr, *d_pair_pp0 = nullptr, *d_pair_npp = nullptr;
     double *d_pp = nullptr, *d_qcomp = nullptr, *d_pair_qhi = nullptr, *d_pair_sch = nullptr, *d_pair_diag = nullptr;
-    double *d_D = nullptr, *d_J = nullptr, *d_K = nullptr;
-    double* d_Jk = nullptr;     // ket-side J'_cd sums of all J/K launches, by ket position
-    size_t jk_len = 0;          // doubles in d_Jk
-    int jk_copies = 1;
-    JkScatter* d_jk_ent = nullptr;   // per J/K launch: where its sums sit in d_Jk
-    int jk_nent = 0;
-    long long jk_max_count = 0;
+    double* d_D = nullptr;      // padded density of the Fock build
+    // per-launch partial J', K' of the J/K slots: [plan entries][n+1][n+1] each
+    double* d_kpart = nullptr;
+    double* d_jpart = nullptr;
+    size_t slot_bytes = 0;          // device memory of all J/K slots
     double *d_flatA = nullptr, *d_flatB = nullptr, *d_flatC = nullptr;
     int *d_sh_atom = nullptr, *d_atom_of_bf = nullptr;   // gradient (uploaded on first use)
     int* d_vpair = nullptr;                 // function pairs grouped by l_i + l_j (one-electron V)

@@ -8,6 +8,14 @@
 #define HFK_MAXCOMP 6        // Cartesian components of a d shell
 #define HFK_LMAX 2           // highest shell angular momentum with generated kernels
 
+// Ket run of a task: positions [x, y) of the ket class list, walked in
+// order; jcd is the J'_cd slot entry of position x (position p writes entry
+// jcd + p - x, NKET doubles each).
+struct KetRun {
+    int x, y;
+    long long jcd;
+};
+
 // Everything a quartet kernel needs.  Pair p: shells pair_A[p] (the one
 // with the larger angular momentum) and pair_B[p]; its primitive pairs are
 // pp[(pair_pp0[p] + i) * PP_STRIDE], i < pair_npp[p].
@@ -26,7 +34,7 @@ struct QuartetArgs {
     int same;                   // bra class == ket class: only j <= i
     long long nquart;
     const int* sh_bf;           // [nshell][HFK_MAXCOMP] basis-function index or -1
-    const int* sh_bfp;          // same with -1 replaced by n (the zero row of padded D/J/K)
+    const int* sh_bfp;          // same with -1 replaced by n (the zero row of padded D)
     const double* sh_scale;     // [nshell][HFK_MAXCOMP] component coefficient scale
     BoysTables bt;
     double qtau;                // reference primitive screen threshold (0 = off)
@@ -35,23 +43,27 @@ struct QuartetArgs {
     int prec_sel;               // 0: all quartets; 1: bound >= fp32_thr; 2: bound < fp32_thr
     int n;                      // number of basis functions
     const double* D;            // [n+1][n+1] density (symmetric), row/column n zero
-    double* J;                  // [n+1][n+1] accumulator J' (symmetrised on output)
-    double* Jk;                 // ket-side sums J'_cd by ket position: [jk_copies][nket][NKET]
-    int jk_copies;              // (workgroup w adds into copy w % jk_copies; scattered into J)
-    double* K;                  // [n+1][n+1] accumulator K'
+    // J/K digestion: every task writes its partial results to its own slots
+    // with plain stores (no atomics); fixed-order reductions sum them later
+    double* kslot;              // [task][(NA+NB) rows][ncols]: K' rows of the bra shells
+    double* jab;                // [slot][NBRA]: J'_ab of the bra pair; task t writes slot jab_slot[t]
+    const int* jab_slot;        // [task]: the tasks of one bra pair have consecutive slots
+    double* jcd;                // J'_cd entries (KetRun.jcd), [entry][NKET]
+    const int* colmap;          // [n+1] basis function -> compact K column; n -> ncols (discarded)
+    int ncols;                  // K columns of the launch (functions of its ket shells)
+    long long task0;            // index of tasks[0] in the launch's task list (slot numbering)
     double* flat;               // flat ERI output (calcIntegrals order)
     const long long* flat_base; // [n][n] rank of (i,j,i,j)
     double* diag;               // per-pair max |(ab|ab)| output (Schwarz setup)
     // task list of the tiled kernels: {bra pair, first run, end run, primitive
     // ranges (b0 << 24 | b1 << 16 | k0 << 8 | k1; k1 = 255: all of the ket's)}
     const int4* tasks;
-    const int2* runs;           // [begin, end) positions in ket_idx, walked in order
+    const KetRun* runs;         // runs of the tasks
     const int* ket_idx;         // ket pair indices (the ket class list)
     long long ntask;
-    int lds_rows;               // K rows staged in LDS (0: global atomics)
     int block;                  // workgroup size of the tiled kernel (64 or 256)
     int bra_stage;              // longest bra primitive-pair range of the tasks (LDS staging)
-    int task_len_max;           // longest ket run of the tasks
+    int task_len_max;           // longest ket sequence of the tasks
     // nuclear gradient
     const int* sh_atom;         // [nshell] atom of each shell
     double* grad;               // [natom][3] accumulator
@@ -89,6 +101,20 @@ inline size_t hfk_bra_stage_bytes(int la, int lb, int nprim)
     return sizeof(double) * nprim * (16 + (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2));
 }
 
+// LDS bytes of a per-lane J/K workgroup: K' rows of the bra shells over the
+// launch's ncols compact columns (+ one discarded), the cross-wave J'_ab
+// sums, the staged bra primitive pairs.
+inline size_t hfk_jk_lds_bytes(int la, int lb, int ncols, int bra_stage)
+{
+    const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
+    return sizeof(double) * ((size_t)(na + nb) * (ncols + 1) + 4 * (size_t)na * nb) + hfk_bra_stage_bytes(la, lb, bra_stage);
+}
+// Dynamic LDS of a cooperative J/K workgroup (its static tables come on top).
+inline size_t hfk_coop_jk_lds_bytes(int la, int lb, int ncols)
+{
+    return sizeof(double) * ((size_t)((la + 1) * (la + 2) / 2 + (lb + 1) * (lb + 2) / 2) * (ncols + 1) + 64);
+}
+
 // Launch one class kernel.  classes are (la,lb,lc,ld) with la>=lb, lc>=ld,
 // (la,lb) >= (lc,ld).  prec: 0 = FP64, 1 = FP32.  boys: 0 reference, 1 accurate.
 // mode (J/K only): 0 main pass, 1 screen-correction pass.
@@ -122,26 +148,62 @@ void hfk_launch_fill(double* p, double v, long long n, hipStream_t s);
 // primitive range.
 void hfk_launch_count_tasks(const QuartetArgs& a, int na, int nb, int nc, int nd, unsigned long long* out,
                             hipStream_t s);
-// J, K padded (leading dimension n+1)
-void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const double* H, double* G_flat,
-                             double* F_full, hipStream_t s);
-// One J/K launch's ket-side sums in Jk: [copies][nket][nc * nd] from offset off,
-// for the ket pairs ket_list[0..nket)
-struct JkScatter {
-    const int* ket_list;
-    long long nket;
-    long long off;
-    int nc, nd;
+// Fixed-order reductions of the J/K slots (hfk_setup.hip).
+// J'_cd of one launch, in two passes: for each chunk of consecutive bras of
+// one bra segment and each ket position, the sum over the chunk's bras that
+// cover the position (bra-list order) and over the primitive pieces, into
+// part[chunk][pos][k]; then the sum over the chunks in order into
+// out[pos][nket_comp] (plain stores).
+struct JcdLayout {
+    const double* jcd;          // the launch's entries
+    double* part;               // [nchunk][nket][nket_comp]
+    double* out;                // [nket][nket_comp]
+    const int* pos_kseg;        // [nket] ket segment of each position
+    const int* kseg_first;      // [nkseg + 1]
+    const int4* chunks;         // [nchunk] {bra segment, first bra, end bra, -}
+    const int2* tiles;          // [ntile] {ket segment, first position}: ptile positions of one segment
+    int ntile, ptile;
+    const int* jend;            // [nbra][nkseg] exclusive end position of bra i's kets in the segment
+    const long long* jbase;     // [nbra][nkseg] entry of the first piece
+    const int* npiece;          // [nbseg][nkseg] primitive pieces per (bra, ket) quartet
+    int nket, nkseg, nchunk, nket_comp, same;
 };
-// J'[c][d] += sum over copies of Jk[off + (copy * nket + pos) * nc * nd + z] for
-// every launch e of ent[0..nent) (one kernel; ld = n + 1)
-void hfk_launch_scatter_jk(double* J, const double* Jk, int copies, const JkScatter* ent, int nent, long long max_count,
-                           const int* pair_A, const int* pair_B, const int* sh_bfp, int ld, hipStream_t s);
+void hfk_launch_reduce_jcd(const JcdLayout& L, hipStream_t s);
+// The per-launch partial J' and K' ((n+1) x (n+1) each, zero-filled on the
+// launch's stream first):
+//   K: for each basis function r of a shell S, the sum over the tasks whose
+//      bra holds S (task order) of their slot rows, scattered to the launch's
+//      columns;
+//   J: the bra side (J'_ab slots of each bra pair, in order) and the ket
+//      side (the reduced J'_cd) at the pairs' function elements; for a class
+//      paired with itself both sides of a pair go to one element in one sum.
+// The Fock build then sums the launches' partials in plan order.
+struct JKPart {
+    const double* kslot;        // [task][nrow][ncols]
+    const int* csr_off;         // [nshell + 1] into csr
+    const int2* csr;            // {task, row offset (0: shell A of the bra, NA: shell B)}
+    const int* collist;         // [ncols] basis function of each compact column
+    int ncols, nrow, nshell;
+    const double* jab;          // [slot][NBRA] (bra side)
+    const int* bcsr_off;        // [nbra + 1]: slots of each bra position
+    const double* jk;           // [nket][NKET] reduced ket side
+    const int* bra_list;        // bra class list
+    const int* ket_list;        // ket class list
+    int nbra, nket, same;
+    int na, nb, nc, nd;         // Cartesian components of the four shells
+    double* Kpart;              // [n+1][n+1]
+    double* Jpart;              // [n+1][n+1]
+};
+void hfk_launch_jk_part(const JKPart& P, int n, const int* sh_bf, const int* sh_bfp, const int* pair_A,
+                        const int* pair_B, hipStream_t s);
+// G = (J' + J'^T) - 1/2 (K' + K'^T) with J' = sum_e Jpart[e], K' = sum_e Kpart[e]
+// (launches in plan order); G_flat = H_flat + G when H is given.
+void hfk_launch_symmetrize_parts(int n, const double* Jpart, const double* Kpart, int nlaunch, const double* H,
+                                 double* G_flat, hipStream_t s);
 // flat -> padded (n+1)x(n+1) full, row/column n zero
 void hfk_launch_flat_to_padded(int n, const double* flat, double* full, hipStream_t s);
-// padded D from the flat density; J', K' ((n+1)^2) and Jk (jk_len) zeroed; one launch
-void hfk_launch_fock_prepare(int n, const double* D_flat, double* D, double* J, double* K, double* Jk, long long jk_len,
-                             hipStream_t s);
+// padded D ((n+1)^2, row/column n zero) from the flat density
+void hfk_launch_fock_prepare(int n, const double* D_flat, double* D, hipStream_t s);
 void hfk_launch_flat_to_full(int n, const double* flat, double* full, hipStream_t s);
 void hfk_launch_full_to_flat(int n, const double* full, double* flat, hipStream_t s);
 

@@ -276,7 +276,9 @@ __device__ __forceinline__ void oe_R(const double* F, double X, double Y, double
     for (int e = 0; e < oe_nherm(LR); ++e) R[e] = cur[e];
 }
 
-// !DERIV: H[idx] += sum_C -Z_C V^C_ij.  DERIV: sum D_ij dV_ij/dR into grad.
+// !DERIV: H[idx] += sum_C -Z_C V^C_ij, one thread per function pair summing
+// the nuclei in order (a fixed-order sum: hcore is bitwise reproducible).
+// DERIV: one thread per (function pair, nucleus); sum D_ij dV_ij/dR into grad.
 template <int LS, int BOYS, bool DERIV>
 __global__ void __launch_bounds__(128) k_onee_v(OneEArgs a, const int* plist, int np, int ng, const int* atom_of_bf,
                                                 const double* Dflat, double* H, double* grad)
@@ -291,8 +293,10 @@ __global__ void __launch_bounds__(128) k_onee_v(OneEArgs a, const int* plist, in
     }
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long p = tid / ng;
-    const int c = (int)(tid - p * ng);   // the nucleus of this thread
-    if (p < np) {
+    const int c0 = DERIV ? (int)(tid - p * ng) : 0;   // the nucleus of this thread (DERIV), else all
+    const int c1 = DERIV ? c0 + 1 : a.natom;
+    double hsum = 0.0;
+    if (p < np) for (int c = c0; c < c1; ++c) {
         const long long idx = plist[p];
         int i, j;
         flat_ij(idx, a.n, i, j);
@@ -355,7 +359,8 @@ __global__ void __launch_bounds__(128) k_onee_v(OneEArgs a, const int* plist, in
             }
         const double zc = -a.Z[c];
         if constexpr (!DERIV) {
-            atomic_add_f64(&H[idx], zc * hv);
+            hsum += zc * hv;
+            if (c == c1 - 1) H[idx] += hsum;
         } else {
             const double s = zc * ((i == j) ? 1.0 : 2.0) * Dflat[idx];
             const int ai = atom_of_bf[i], aj = atom_of_bf[j];
@@ -430,7 +435,7 @@ template <bool DERIV>
 static void launch_onee_v(const OneEArgs& a, const int* atom_of_bf, const double* D_flat, double* H, double* grad,
                           hipStream_t s)
 {
-    const int ng = a.natom;   // one thread per (function pair, nucleus)
+    const int ng = DERIV ? a.natom : 1;   // one thread per (function pair, nucleus) / per function pair
     const size_t shm = DERIV ? sizeof(double) * 3 * (size_t)a.natom : 0;
     for (int ls = 0; ls <= 4; ++ls) {
         const int np = a.vpair_off[ls + 1] - a.vpair_off[ls];

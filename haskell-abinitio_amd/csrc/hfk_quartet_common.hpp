@@ -43,13 +43,50 @@ __device__ __forceinline__ int task_ket_count(const QuartetArgs& a, const int4& 
 __device__ __forceinline__ int task_ket_pos(const QuartetArgs& a, const int4& task, int v)
 {
     for (int r = task.y; r < task.z; ++r) {
-        const int2 q = a.runs[r];
+        const KetRun q = a.runs[r];
         const int len = q.y - q.x;
         if (v < len) return q.x + v;
         v -= len;
     }
     return -1;
 }
+
+// A lane's walk over the task's kets v = v0, v0 + step, ... (increasing):
+// the run holding v is found by advancing from the previous one, so a lane
+// reads each run record once.
+struct KetWalk {
+    int r;          // run holding the current ket
+    int start;      // task-sequence index of that run's first ket
+    int x, len;     // the run's first position and length
+    long long jcd;  // the run's first J'_cd entry
+    __device__ __forceinline__ void begin(const QuartetArgs& a, const int4& task)
+    {
+        r = task.y;
+        start = 0;
+        x = len = 0;
+        jcd = 0;
+        if (task.y < task.z) {
+            const KetRun q = a.runs[r];
+            x = q.x;
+            len = q.y - q.x;
+            jcd = q.jcd;
+        }
+    }
+    // position and J'_cd entry of the task's v-th ket (v below the task's count)
+    __device__ __forceinline__ void seek(const QuartetArgs& a, int v, int& pos, long long& entry)
+    {
+        while (v >= start + len) {
+            start += len;
+            ++r;
+            const KetRun q = a.runs[r];
+            x = q.x;
+            len = q.y - q.x;
+            jcd = q.jcd;
+        }
+        pos = x + (v - start);
+        entry = jcd + (v - start);
+    }
+};
 
 // End of the task's ket primitive range for ket pair kp (k1 = 255: all).
 __device__ __forceinline__ int task_kprim_end(const QuartetArgs& a, int k1, int kp)

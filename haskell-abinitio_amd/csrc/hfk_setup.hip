@@ -210,34 +210,6 @@ __global__ void k_fill(double* p, double v, long long n)
         p[i] = v;
 }
 
-// G = (J' + J'^T) - 1/2 (K' + K'^T); G_flat (upper triangle, row-major) =
-// H_flat + G if H is given; F_full (optional) receives the symmetric matrix.
-__global__ void k_symmetrize_g(int n, const double* J, const double* K, const double* H, double* G_flat,
-                               double* F_full)
-{
-    const long long m = (long long)n * (n + 1) / 2;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < m;
-         idx += (long long)gridDim.x * blockDim.x) {
-        // row i: starts at i*n - i(i-1)/2
-        const double nn = (double)n + 0.5;
-        long long i = (long long)(nn - sqrt(nn * nn - 2.0 * (double)idx));
-        if (i < 0) i = 0;
-        while (i > 0 && i * n - i * (i - 1) / 2 > idx) --i;
-        while ((i + 1) * n - (i + 1) * i / 2 <= idx) ++i;
-        const long long j = i + (idx - (i * n - i * (i - 1) / 2));
-        const long long ld = n + 1;
-        const double jv = J[i * ld + j] + J[j * ld + i];
-        const double kv = K[i * ld + j] + K[j * ld + i];
-        double g = jv - 0.5 * kv;
-        if (H) g += H[idx];
-        G_flat[idx] = g;
-        if (F_full) {
-            F_full[i * n + j] = g;
-            F_full[j * n + i] = g;
-        }
-    }
-}
-
 __global__ void k_flat_to_full(int n, const double* flat, double* full)
 {
     const long long m = (long long)n * (n + 1) / 2;
@@ -283,21 +255,13 @@ __global__ void k_flat_to_padded(int n, const double* flat, double* full)
         flat_to_padded_elem(n, flat, full, idx);
 }
 
-// Start of a Fock build in one launch: padded D from the flat density, and
-// J', K' ((n+1)^2 each) and the ket-side sums (jk_len) cleared.
-__global__ void k_fock_prepare(int n, const double* D_flat, double* D, double* J, double* K, double* Jk, long long jk_len)
+// Start of a Fock build: padded D ((n+1)^2, row/column n zero) from the flat density.
+__global__ void k_fock_prepare(int n, const double* D_flat, double* D)
 {
-    const long long m = (long long)n * (n + 1) / 2, np1 = (long long)(n + 1) * (n + 1);
-    const long long total = max(max(m + n + 1, np1), jk_len);
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        if (idx < m + n + 1) flat_to_padded_elem(n, D_flat, D, idx);
-        if (idx < np1) {
-            J[idx] = 0.0;
-            K[idx] = 0.0;
-        }
-        if (idx < jk_len) Jk[idx] = 0.0;
-    }
+    const long long m = (long long)n * (n + 1) / 2;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < m + n + 1;
+         idx += (long long)gridDim.x * blockDim.x)
+        flat_to_padded_elem(n, D_flat, D, idx);
 }
 
 __global__ void k_full_to_flat(int n, const double* full, double* flat)
@@ -394,43 +358,229 @@ void hfk_launch_fill(double* p, double v, long long n, hipStream_t s)
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
 }
 
-void hfk_launch_symmetrize_g(int n, const double* J, const double* K, const double* H, double* G_flat,
-                             double* F_full, hipStream_t s)
+
+// ----------------------------------------------------------------------
+// Fixed-order reductions of the J/K slots.  Every sum below runs over its
+// terms in an order fixed by the plan (bra-list order, piece order, task
+// order, plan order), and every output element is written by one thread
+// with a plain store, so a Fock build is bitwise reproducible.
+// ----------------------------------------------------------------------
+
+// J'_cd, pass 1: workgroup = (tile of ptile consecutive positions of one
+// ket segment, chunk of consecutive bras of one bra segment); thread =
+// (position, component).  The chunk's ket ends and entry offsets for the
+// segment are loaded into LDS first; the workgroup then walks the chunk's
+// bras in order and a thread adds the bras that cover its position.  The
+// bras' ket ends do not increase along a bra segment (bras sorted by
+// descending screen value), so the walk stops at the first bra that covers
+// none of the tile; for a class paired with itself only bras at or after a
+// position hold it.
+#define HFK_JCD_STAGE 256
+__global__ void __launch_bounds__(256) k_reduce_jcd_part(JcdLayout L)
 {
-    const long long m = (long long)n * (n + 1) / 2;
-    hipLaunchKernelGGL(k_symmetrize_g, dim3(grid_for(m, 256)), dim3(256), 0, s, n, J, K, H, G_flat, F_full);
+    __shared__ int s_end[HFK_JCD_STAGE];
+    __shared__ long long s_base[HFK_JCD_STAGE];
+    const int2 tile = L.tiles[blockIdx.x];
+    const int4 ch = L.chunks[blockIdx.y];
+    const int ks = tile.x, t0 = tile.y;
+    const int first = L.kseg_first[ks], kend = L.kseg_first[ks + 1];
+    const int nkc = L.nket_comp;
+    const int pl = threadIdx.x / nkc, k = threadIdx.x - pl * nkc;
+    const int pos = t0 + pl;
+    const bool valid = pl < L.ptile && pos < kend;
+    const int off = pos - first;
+    const int np = L.npiece[ch.x * L.nkseg + ks];
+    // the chunk's bras [ch.y, ch.z), staged HFK_JCD_STAGE at a time
+    int i0 = ch.y;
+    if (L.same) i0 = max(i0, t0);
+    double sum = 0.0;
+    bool done = false;   // uniform
+    for (int base = i0; base < ch.z && !done; base += HFK_JCD_STAGE) {
+        const int cnt = min(HFK_JCD_STAGE, ch.z - base);
+        __syncthreads();
+        for (int b = threadIdx.x; b < cnt; b += blockDim.x) {
+            const long long qi = (long long)(base + b) * L.nkseg + ks;
+            s_end[b] = L.jend[qi];
+            s_base[b] = L.jbase[qi];
+        }
+        __syncthreads();
+        for (int b = 0; b < cnt; ++b) {
+            const int end = s_end[b];
+            if (end <= t0) {   // uniform: no later bra covers the tile
+                done = true;
+                break;
+            }
+            if (valid && pos < end && (!L.same || pos <= base + b)) {
+                const long long len = end - first;
+                const long long e0 = s_base[b] + off;
+                for (int p = 0; p < np; ++p) sum += L.jcd[(e0 + p * len) * nkc + k];
+            }
+        }
+    }
+    if (valid) L.part[((long long)blockIdx.y * L.nket + pos) * nkc + k] = sum;
 }
 
-// Ket-side J'_cd sums, kept by position in the ket class list during the
-// J/K launch, added into J' (atomic: absent components of different pairs
-// share the zero row/column n).
-// grid.y runs over the launches
-__global__ void k_scatter_jk(double* J, const double* Jk, int copies, const JkScatter* ent, const int* pair_A,
-                             const int* pair_B, const int* sh_bfp, int ld)
+// J'_cd, pass 2: thread = (ket position, component); the chunks in order.
+__global__ void k_reduce_jcd_final(JcdLayout L)
 {
-    const JkScatter e = ent[blockIdx.y];
-    const int nz = e.nc * e.nd;
-    const long long count = e.nket * nz;
-    const double* src = Jk + e.off;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-         i += (long long)gridDim.x * blockDim.x) {
-        double v = 0.0;
-        for (int c = 0; c < copies; ++c) v += src[(size_t)c * count + i];
-        if (v == 0.0) continue;
-        const long long pos = i / nz;
-        const int z = (int)(i - pos * nz);
-        const int kp = e.ket_list[pos];
-        const int ic = sh_bfp[pair_A[kp] * HFK_MAXCOMP + z / e.nd], id = sh_bfp[pair_B[kp] * HFK_MAXCOMP + z % e.nd];
-        atomic_add_f64(&J[(size_t)ic * ld + id], v);
+    const long long per = (long long)L.nket * L.nket_comp;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < per;
+         t += (long long)gridDim.x * blockDim.x) {
+        double sum = 0.0;
+        for (int q = 0; q < L.nchunk; ++q) sum += L.part[(long long)q * per + t];
+        L.out[t] = sum;
     }
 }
 
-void hfk_launch_scatter_jk(double* J, const double* Jk, int copies, const JkScatter* ent, int nent, long long max_count,
-                           const int* pair_A, const int* pair_B, const int* sh_bfp, int ld, hipStream_t s)
+void hfk_launch_reduce_jcd(const JcdLayout& L, hipStream_t s)
 {
-    if (nent <= 0 || max_count <= 0) return;
-    hipLaunchKernelGGL(k_scatter_jk, dim3(grid_for(max_count, 256), (unsigned)nent), dim3(256), 0, s, J, Jk, copies, ent,
-                       pair_A, pair_B, sh_bfp, ld);
+    const long long per = (long long)L.nket * L.nket_comp;
+    if (per <= 0) return;
+    if (L.nchunk > 0 && L.ntile > 0)
+        hipLaunchKernelGGL(k_reduce_jcd_part, dim3((unsigned)L.ntile, (unsigned)L.nchunk), dim3(256), 0, s, L);
+    hipLaunchKernelGGL(k_reduce_jcd_final, dim3(grid_for(per, 256)), dim3(256), 0, s, L);
+}
+
+// K' partial: one workgroup per shell component row.  Its entries (task,
+// row offset) are staged in LDS 256 at a time; the threads form G = 256 /
+// (64 ceil(ncols / 64)) groups (G = 4 for narrow launches), group g summing
+// the entries g, g + G, ... in task order, a thread owning compact columns
+// in registers; the groups' rows are added in group order.
+__global__ void __launch_bounds__(256) k_k_part(JKPart P, int n, const int* sh_bf)
+{
+    constexpr int NCOL = 16;   // columns per thread: ncols <= 4096 (the planner checks)
+    __shared__ double red[3][256];
+    __shared__ int2 s_csr[256];
+    const int S = blockIdx.x / HFK_MAXCOMP, x = blockIdx.x - S * HFK_MAXCOMP;
+    const int r = sh_bf[S * HFK_MAXCOMP + x];
+    if (r < 0) return;
+    const int tid = threadIdx.x;
+    const int width = 64 * ((P.ncols + 63) / 64);
+    const int G = width >= 256 ? 1 : 256 / width;
+    const int gw = G == 1 ? 256 : width;
+    const int g = tid / gw, lane = tid - g * gw;
+    double acc[NCOL];
+#pragma unroll
+    for (int c = 0; c < NCOL; ++c) acc[c] = 0.0;
+    const int q0 = P.csr_off[S], q1 = P.csr_off[S + 1];
+    for (int base = q0; base < q1; base += 256) {
+        const int cnt = min(256, q1 - base);
+        __syncthreads();
+        if (tid < cnt) s_csr[tid] = P.csr[base + tid];
+        __syncthreads();
+        if (g < G) {
+#pragma unroll 4
+            for (int q = g; q < cnt; q += G) {
+                const int2 en = s_csr[q];
+                const double* src = P.kslot + ((size_t)en.x * P.nrow + en.y + x) * P.ncols;
+#pragma unroll
+                for (int c = 0; c < NCOL; ++c) {
+                    const int j = lane + gw * c;
+                    if (j < P.ncols) acc[c] += src[j];
+                }
+            }
+        }
+    }
+    double* row = P.Kpart + (size_t)r * (n + 1);   // zero-filled beforehand
+    if (G == 1) {
+#pragma unroll
+        for (int c = 0; c < NCOL; ++c) {
+            const int j = lane + gw * c;
+            if (j < P.ncols) row[P.collist[j]] = acc[c];
+        }
+        return;
+    }
+    // G > 1: ncols <= 128, one column per thread (c = 0)
+    __syncthreads();
+    if (g > 0 && g < G) red[g - 1][lane] = acc[0];
+    __syncthreads();
+    if (g == 0 && lane < P.ncols) {
+        double v = acc[0];
+        for (int h = 1; h < G; ++h) v += red[h - 1][lane];
+        row[P.collist[lane]] = v;
+    }
+}
+
+// J' partial: thread = (position in the bra class list, bra component)
+// (bra side, plus the ket side of the same pair when the classes are one),
+// then for two classes (position in the ket class list, ket component).
+__global__ void k_j_part(JKPart P, int n, const int* sh_bfp, const int* pair_A, const int* pair_B)
+{
+    const int nbc = P.na * P.nb, nkc = P.nc * P.nd;
+    const long long tb = (long long)P.nbra * nbc;
+    const long long total = tb + (P.same ? 0 : (long long)P.nket * nkc);
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        int p, x, y;
+        double v = 0.0;
+        if (t < tb) {
+            const int pos = (int)(t / nbc), c = (int)(t - (long long)pos * nbc);
+            p = P.bra_list[pos];
+            x = c / P.nb;
+            y = c - x * P.nb;
+            for (int q = P.bcsr_off[pos]; q < P.bcsr_off[pos + 1]; ++q) v += P.jab[(size_t)q * nbc + c];
+            if (P.same) v += P.jk[(size_t)pos * nkc + c];
+        } else {
+            const long long u = t - tb;
+            const int pos = (int)(u / nkc), c = (int)(u - (long long)pos * nkc);
+            p = P.ket_list[pos];
+            x = c / P.nd;
+            y = c - x * P.nd;
+            v = P.jk[(size_t)pos * nkc + c];
+        }
+        const int ia = sh_bfp[pair_A[p] * HFK_MAXCOMP + x], ib = sh_bfp[pair_B[p] * HFK_MAXCOMP + y];
+        if (ia < n && ib < n) P.Jpart[(size_t)ia * (n + 1) + ib] = v;
+    }
+}
+
+void hfk_launch_jk_part(const JKPart& P, int n, const int* sh_bf, const int* sh_bfp, const int* pair_A,
+                        const int* pair_B, hipStream_t s)
+{
+    const long long np1 = (long long)(n + 1) * (n + 1);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(np1, 256)), dim3(256), 0, s, P.Kpart, 0.0, np1);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(np1, 256)), dim3(256), 0, s, P.Jpart, 0.0, np1);
+    if (P.nshell > 0)
+        hipLaunchKernelGGL(k_k_part, dim3((unsigned)P.nshell * HFK_MAXCOMP), dim3(256), 0, s, P, n, sh_bf);
+    const long long total = (long long)P.nbra * P.na * P.nb + (P.same ? 0 : (long long)P.nket * P.nc * P.nd);
+    if (total > 0) hipLaunchKernelGGL(k_j_part, dim3(grid_for(total, 256)), dim3(256), 0, s, P, n, sh_bfp, pair_A, pair_B);
+}
+
+// G_ij = (J'_ij + J'_ji) - 1/2 (K'_ij + K'_ji), J' and K' the sums over the
+// launches' partials in plan order.
+__global__ void k_symmetrize_parts(int n, const double* Jpart, const double* Kpart, int nlaunch, const double* H,
+                                   double* G_flat)
+{
+    const long long m = (long long)n * (n + 1) / 2;
+    const long long np1 = (long long)(n + 1) * (n + 1);
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < m;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const double nn = (double)n + 0.5;
+        long long i = (long long)(nn - sqrt(nn * nn - 2.0 * (double)idx));
+        if (i < 0) i = 0;
+        while (i > 0 && i * n - i * (i - 1) / 2 > idx) --i;
+        while ((i + 1) * n - (i + 1) * i / 2 <= idx) ++i;
+        const long long j = i + (idx - (i * n - i * (i - 1) / 2));
+        const long long ld = n + 1;
+        double jij = 0.0, jji = 0.0, kij = 0.0, kji = 0.0;
+        for (int e = 0; e < nlaunch; ++e) {
+            const double* Je = Jpart + (size_t)e * np1;
+            const double* Ke = Kpart + (size_t)e * np1;
+            jij += Je[i * ld + j];
+            jji += Je[j * ld + i];
+            kij += Ke[i * ld + j];
+            kji += Ke[j * ld + i];
+        }
+        double g = (jij + jji) - 0.5 * (kij + kji);
+        if (H) g += H[idx];
+        G_flat[idx] = g;
+    }
+}
+
+void hfk_launch_symmetrize_parts(int n, const double* Jpart, const double* Kpart, int nlaunch, const double* H,
+                                 double* G_flat, hipStream_t s)
+{
+    const long long m = (long long)n * (n + 1) / 2;
+    hipLaunchKernelGGL(k_symmetrize_parts, dim3(grid_for(m, 256)), dim3(256), 0, s, n, Jpart, Kpart, nlaunch, H, G_flat);
 }
 
 void hfk_launch_flat_to_full(int n, const double* flat, double* full, hipStream_t s)
@@ -445,12 +595,10 @@ void hfk_launch_flat_to_padded(int n, const double* flat, double* full, hipStrea
     hipLaunchKernelGGL(k_flat_to_padded, dim3(grid_for(m, 256)), dim3(256), 0, s, n, flat, full);
 }
 
-void hfk_launch_fock_prepare(int n, const double* D_flat, double* D, double* J, double* K, double* Jk, long long jk_len,
-                             hipStream_t s)
+void hfk_launch_fock_prepare(int n, const double* D_flat, double* D, hipStream_t s)
 {
-    const long long m = (long long)n * (n + 1) / 2, np1 = (long long)(n + 1) * (n + 1);
-    const long long total = std::max(std::max(m + n + 1, np1), jk_len);
-    hipLaunchKernelGGL(k_fock_prepare, dim3(grid_for(total, 256)), dim3(256), 0, s, n, D_flat, D, J, K, Jk, jk_len);
+    const long long m = (long long)n * (n + 1) / 2 + n + 1;
+    hipLaunchKernelGGL(k_fock_prepare, dim3(grid_for(m, 256)), dim3(256), 0, s, n, D_flat, D);
 }
 
 void hfk_launch_full_to_flat(int n, const double* full, double* flat, hipStream_t s)
