@@ -138,8 +138,18 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 // between turns); within one wave the hardware orders the lanes' adds, so
 // the LDS sums do not depend on scheduling either.
 // ---------------------------------------------------------------------
+// Waves per SIMD the register allocation must leave room for: the L = 3
+// classes need a few registers over 256 (VGPRs + AGPRs), which alone would
+// leave one wave per SIMD.
+template <int LA, int LB, int LC, int LD>
+constexpr int jk_min_waves()
+{
+    return (LA + LB + LC + LD == 3) ? 2 : 1;
+}
+
 template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
-__global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_waves<LA, LB, LC, LD>())))
+k_jk_tiles(QuartetArgs a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NKET = NC * ND, NBRA = NA * NB;
@@ -202,8 +212,7 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
         const int it = base + tid;
         const bool in = it < len;
         int pos = 0;
-        long long entry = 0;
-        if (in) walk.seek(a, it, pos, entry);
+        if (in) walk.seek(a, it, pos);
         const int kp = in ? a.ket_idx[pos] : 0;
         const bool act = in && quartet_selected(a, bp, kp);
         // K' contributions: rows a (x < NA) and b, at columns c and d
@@ -327,16 +336,24 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
             }
         }
         // this ket's J'_cd entry (zero when the quartet is skipped: every
-        // entry is rewritten by every build)
+        // entry is rewritten by every build); the run record is read again
+        // here rather than kept in registers across the integrals
+        asm volatile("" ::: "memory");
         if (in) {
+            const long long entry = walk.entry(a, it);
 #pragma unroll
             for (int k = 0; k < NKET; ++k) Jcd[entry * NKET + k] = jv[k];
         }
-        // K' rows in LDS, the waves in turn
-        for (int w = 0; w < nwaves; ++w) {
-            if (w == wave && act) {
+        // K' rows in LDS.  With several waves, phase ph lets wave w add to the
+        // rows r with r mod nwaves == (w + ph) mod nwaves: in every phase each
+        // row takes adds from one wave only, and over the phases a row takes
+        // them in a fixed wave order.
+        for (int ph = 0; ph < nwaves; ++ph) {
+            const int grp = (wave + ph) % nwaves;
+            if (act) {
 #pragma unroll
                 for (int x = 0; x < NA; ++x) {
+                    if (x % nwaves != grp) continue;
 #pragma unroll
                     for (int z = 0; z < NC; ++z) atomic_add_lds(&krow[x * ldk + cc[z]], kac[x * NC + z]);
 #pragma unroll
@@ -344,6 +361,7 @@ __global__ void __launch_bounds__(256) k_jk_tiles(QuartetArgs a)
                 }
 #pragma unroll
                 for (int y = 0; y < NB; ++y) {
+                    if ((NA + y) % nwaves != grp) continue;
 #pragma unroll
                     for (int z = 0; z < NC; ++z) atomic_add_lds(&krow[(NA + y) * ldk + cc[z]], kbc[y * NC + z]);
 #pragma unroll

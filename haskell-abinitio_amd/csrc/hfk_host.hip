@@ -902,10 +902,9 @@ int hfk_ctx_plan(hfk_ctx* c)
         HFK_TRY(entry_alloc(c, e, &e.d_kslot, std::max<size_t>(1, (size_t)e.ntask * nrow * e.ncols)));
         HFK_TRY(entry_alloc(c, e, &e.d_jab, std::max<size_t>(1, (size_t)e.ntask * nbra_comp)));
         HFK_TRY(entry_alloc(c, e, &e.d_jcd, std::max<size_t>(1, (size_t)njcd * nket_comp)));
-        HFK_TRY(entry_alloc(c, e, &e.d_jk, std::max<size_t>(1, kl.size() * (size_t)nket_comp)));
         e.jl.jcd = e.d_jcd;
-        e.jl.out = e.d_jk;
-        e.slot_bytes = sizeof(double) * ((size_t)e.ntask * (nrow * e.ncols + nbra_comp) + ((size_t)njcd + kl.size()) * nket_comp);
+        e.slot_bytes = sizeof(double) * ((size_t)e.ntask * (nrow * e.ncols + nbra_comp) +
+                                         ((size_t)njcd + (size_t)e.jl.nchunk * kl.size()) * nket_comp);
         c->slot_bytes += e.slot_bytes;
     }
     // the launches' partial J', K'
@@ -913,6 +912,10 @@ int hfk_ctx_plan(hfk_ctx* c)
         const size_t np1 = (size_t)(c->n + 1) * (c->n + 1), nl = std::max<size_t>(1, c->plan.size());
         HFK_TRY(dev_alloc(&c->d_kpart, nl * np1));
         HFK_TRY(dev_alloc(&c->d_jpart, nl * np1));
+        // every build writes the same elements of the partials; the rest stay zero
+        hfk_launch_fill(c->d_kpart, 0.0, (long long)(nl * np1), c->stream);
+        hfk_launch_fill(c->d_jpart, 0.0, (long long)(nl * np1), c->stream);
+        HFK_HIP(hipStreamSynchronize(c->stream));
         for (size_t k = 0; k < c->plan.size(); ++k) {
             PlanEntry& e = c->plan[k];
             JKPart& P = e.jkp;
@@ -926,7 +929,8 @@ int hfk_ctx_plan(hfk_ctx* c)
             P.nshell = nshell;
             P.jab = e.d_jab;
             P.bcsr_off = e.d_bcsr_off;
-            P.jk = e.d_jk;
+            P.jpart = e.jl.part;
+            P.nchunk = e.jl.nchunk;
             P.bra_list = c->d_class[e.kb];
             P.ket_list = c->d_class[e.kk];
             P.nbra = (int)c->class_count[e.kb];

@@ -58,7 +58,6 @@ struct PlanEntry {
     double* d_kslot;            // [ntask][NA+NB][ncols]
     double* d_jab;              // [ntask][NA*NB]
     double* d_jcd;              // [njcd][NC*ND]
-    double* d_jk;               // [nket][NC*ND]: reduced ket side
     long long njcd;
     JcdLayout jl;               // device pointers of the J'_cd layout
     JKPart jkp;                 // its partial J', K' (reduced on its stream after the launch)

@@ -149,15 +149,14 @@ void hfk_launch_fill(double* p, double v, long long n, hipStream_t s);
 void hfk_launch_count_tasks(const QuartetArgs& a, int na, int nb, int nc, int nd, unsigned long long* out,
                             hipStream_t s);
 // Fixed-order reductions of the J/K slots (hfk_setup.hip).
-// J'_cd of one launch, in two passes: for each chunk of consecutive bras of
-// one bra segment and each ket position, the sum over the chunk's bras that
-// cover the position (bra-list order) and over the primitive pieces, into
-// part[chunk][pos][k]; then the sum over the chunks in order into
-// out[pos][nket_comp] (plain stores).
+// J'_cd of one launch: for each chunk of consecutive bras of one bra
+// segment and each ket position, the sum over the chunk's bras that cover
+// the position (bra-list order) and over the primitive pieces, into
+// part[chunk][pos][k] (plain stores); the J' partial then sums the chunks in
+// order.
 struct JcdLayout {
     const double* jcd;          // the launch's entries
     double* part;               // [nchunk][nket][nket_comp]
-    double* out;                // [nket][nket_comp]
     const int* pos_kseg;        // [nket] ket segment of each position
     const int* kseg_first;      // [nkseg + 1]
     const int4* chunks;         // [nchunk] {bra segment, first bra, end bra, -}
@@ -169,13 +168,14 @@ struct JcdLayout {
     int nket, nkseg, nchunk, nket_comp, same;
 };
 void hfk_launch_reduce_jcd(const JcdLayout& L, hipStream_t s);
-// The per-launch partial J' and K' ((n+1) x (n+1) each, zero-filled on the
-// launch's stream first):
+// The per-launch partial J' and K' ((n+1) x (n+1) each; every build writes
+// the same elements, the others stay zero from planning):
 //   K: for each basis function r of a shell S, the sum over the tasks whose
 //      bra holds S (task order) of their slot rows, scattered to the launch's
 //      columns;
 //   J: the bra side (J'_ab slots of each bra pair, in order) and the ket
-//      side (the reduced J'_cd) at the pairs' function elements; for a class
+//      side (the J'_cd chunk sums, in chunk order) at the pairs' function
+//      elements; for a class
 //      paired with itself both sides of a pair go to one element in one sum.
 // The Fock build then sums the launches' partials in plan order.
 struct JKPart {
@@ -186,7 +186,8 @@ struct JKPart {
     int ncols, nrow, nshell;
     const double* jab;          // [slot][NBRA] (bra side)
     const int* bcsr_off;        // [nbra + 1]: slots of each bra position
-    const double* jk;           // [nket][NKET] reduced ket side
+    const double* jpart;        // [nchunk][nket][NKET] ket-side chunk sums
+    int nchunk;
     const int* bra_list;        // bra class list
     const int* ket_list;        // ket class list
     int nbra, nket, same;

@@ -420,38 +420,31 @@ __global__ void __launch_bounds__(256) k_reduce_jcd_part(JcdLayout L)
     if (valid) L.part[((long long)blockIdx.y * L.nket + pos) * nkc + k] = sum;
 }
 
-// J'_cd, pass 2: thread = (ket position, component); the chunks in order.
-__global__ void k_reduce_jcd_final(JcdLayout L)
-{
-    const long long per = (long long)L.nket * L.nket_comp;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < per;
-         t += (long long)gridDim.x * blockDim.x) {
-        double sum = 0.0;
-        for (int q = 0; q < L.nchunk; ++q) sum += L.part[(long long)q * per + t];
-        L.out[t] = sum;
-    }
-}
-
 void hfk_launch_reduce_jcd(const JcdLayout& L, hipStream_t s)
 {
     const long long per = (long long)L.nket * L.nket_comp;
     if (per <= 0) return;
     if (L.nchunk > 0 && L.ntile > 0)
         hipLaunchKernelGGL(k_reduce_jcd_part, dim3((unsigned)L.ntile, (unsigned)L.nchunk), dim3(256), 0, s, L);
-    hipLaunchKernelGGL(k_reduce_jcd_final, dim3(grid_for(per, 256)), dim3(256), 0, s, L);
 }
 
-// K' partial: one workgroup per shell component row.  Its entries (task,
-// row offset) are staged in LDS 256 at a time; the threads form G = 256 /
-// (64 ceil(ncols / 64)) groups (G = 4 for narrow launches), group g summing
-// the entries g, g + G, ... in task order, a thread owning compact columns
-// in registers; the groups' rows are added in group order.
-__global__ void __launch_bounds__(256) k_k_part(JKPart P, int n, const int* sh_bf)
+// The launch's partial J', K' in one kernel.
+// Blocks [0, nshell * HFK_MAXCOMP): K' partial, one workgroup per shell
+// component row.  Its entries (task, row offset) are staged in LDS 256 at a
+// time; the threads form G = 256 / (64 ceil(ncols / 64)) groups (G = 4 for
+// narrow launches), group g summing the entries g, g + G, ... in task order,
+// a thread owning compact columns in registers; the groups' rows are added
+// in group order.
+// Blocks above: J' partial, thread = (position in the bra class list, bra
+// component) (bra side, plus the ket side of the same pair when the classes
+// are one), then for two classes (position in the ket class list, ket
+// component); the ket side sums the J'_cd chunk sums in chunk order.
+__device__ __forceinline__ void k_part_rows(const JKPart& P, int n, const int* sh_bf, int blk)
 {
     constexpr int NCOL = 16;   // columns per thread: ncols <= 4096 (the planner checks)
     __shared__ double red[3][256];
     __shared__ int2 s_csr[256];
-    const int S = blockIdx.x / HFK_MAXCOMP, x = blockIdx.x - S * HFK_MAXCOMP;
+    const int S = blk / HFK_MAXCOMP, x = blk - S * HFK_MAXCOMP;
     const int r = sh_bf[S * HFK_MAXCOMP + x];
     if (r < 0) return;
     const int tid = threadIdx.x;
@@ -481,7 +474,7 @@ __global__ void __launch_bounds__(256) k_k_part(JKPart P, int n, const int* sh_b
             }
         }
     }
-    double* row = P.Kpart + (size_t)r * (n + 1);   // zero-filled beforehand
+    double* row = P.Kpart + (size_t)r * (n + 1);
     if (G == 1) {
 #pragma unroll
         for (int c = 0; c < NCOL; ++c) {
@@ -501,48 +494,56 @@ __global__ void __launch_bounds__(256) k_k_part(JKPart P, int n, const int* sh_b
     }
 }
 
-// J' partial: thread = (position in the bra class list, bra component)
-// (bra side, plus the ket side of the same pair when the classes are one),
-// then for two classes (position in the ket class list, ket component).
-__global__ void k_j_part(JKPart P, int n, const int* sh_bfp, const int* pair_A, const int* pair_B)
+__device__ __forceinline__ double k_part_ket_sum(const JKPart& P, int pos, int c)
 {
+    const long long per = (long long)P.nket * P.nc * P.nd;
+    const long long t = (long long)pos * P.nc * P.nd + c;
+    double v = 0.0;
+    for (int q = 0; q < P.nchunk; ++q) v += P.jpart[(long long)q * per + t];
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_jk_part(JKPart P, int n, const int* sh_bf, const int* sh_bfp,
+                                                 const int* pair_A, const int* pair_B)
+{
+    const int nkrow = P.nshell * HFK_MAXCOMP;
+    if ((int)blockIdx.x < nkrow) {
+        k_part_rows(P, n, sh_bf, blockIdx.x);
+        return;
+    }
     const int nbc = P.na * P.nb, nkc = P.nc * P.nd;
     const long long tb = (long long)P.nbra * nbc;
     const long long total = tb + (P.same ? 0 : (long long)P.nket * nkc);
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
-        int p, x, y;
-        double v = 0.0;
-        if (t < tb) {
-            const int pos = (int)(t / nbc), c = (int)(t - (long long)pos * nbc);
-            p = P.bra_list[pos];
-            x = c / P.nb;
-            y = c - x * P.nb;
-            for (int q = P.bcsr_off[pos]; q < P.bcsr_off[pos + 1]; ++q) v += P.jab[(size_t)q * nbc + c];
-            if (P.same) v += P.jk[(size_t)pos * nkc + c];
-        } else {
-            const long long u = t - tb;
-            const int pos = (int)(u / nkc), c = (int)(u - (long long)pos * nkc);
-            p = P.ket_list[pos];
-            x = c / P.nd;
-            y = c - x * P.nd;
-            v = P.jk[(size_t)pos * nkc + c];
-        }
-        const int ia = sh_bfp[pair_A[p] * HFK_MAXCOMP + x], ib = sh_bfp[pair_B[p] * HFK_MAXCOMP + y];
-        if (ia < n && ib < n) P.Jpart[(size_t)ia * (n + 1) + ib] = v;
+    const long long t = (long long)(blockIdx.x - nkrow) * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    int p, x, y;
+    double v = 0.0;
+    if (t < tb) {
+        const int pos = (int)(t / nbc), c = (int)(t - (long long)pos * nbc);
+        p = P.bra_list[pos];
+        x = c / P.nb;
+        y = c - x * P.nb;
+        for (int q = P.bcsr_off[pos]; q < P.bcsr_off[pos + 1]; ++q) v += P.jab[(size_t)q * nbc + c];
+        if (P.same) v += k_part_ket_sum(P, pos, c);
+    } else {
+        const long long u = t - tb;
+        const int pos = (int)(u / nkc), c = (int)(u - (long long)pos * nkc);
+        p = P.ket_list[pos];
+        x = c / P.nd;
+        y = c - x * P.nd;
+        v = k_part_ket_sum(P, pos, c);
     }
+    const int ia = sh_bfp[pair_A[p] * HFK_MAXCOMP + x], ib = sh_bfp[pair_B[p] * HFK_MAXCOMP + y];
+    if (ia < n && ib < n) P.Jpart[(size_t)ia * (n + 1) + ib] = v;
 }
 
 void hfk_launch_jk_part(const JKPart& P, int n, const int* sh_bf, const int* sh_bfp, const int* pair_A,
                         const int* pair_B, hipStream_t s)
 {
-    const long long np1 = (long long)(n + 1) * (n + 1);
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(np1, 256)), dim3(256), 0, s, P.Kpart, 0.0, np1);
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(np1, 256)), dim3(256), 0, s, P.Jpart, 0.0, np1);
-    if (P.nshell > 0)
-        hipLaunchKernelGGL(k_k_part, dim3((unsigned)P.nshell * HFK_MAXCOMP), dim3(256), 0, s, P, n, sh_bf);
     const long long total = (long long)P.nbra * P.na * P.nb + (P.same ? 0 : (long long)P.nket * P.nc * P.nd);
-    if (total > 0) hipLaunchKernelGGL(k_j_part, dim3(grid_for(total, 256)), dim3(256), 0, s, P, n, sh_bfp, pair_A, pair_B);
+    const long long blocks = (long long)P.nshell * HFK_MAXCOMP + (total + 255) / 256;
+    if (blocks > 0)
+        hipLaunchKernelGGL(k_jk_part, dim3((unsigned)blocks), dim3(256), 0, s, P, n, sh_bf, sh_bfp, pair_A, pair_B);
 }
 
 // G_ij = (J'_ij + J'_ji) - 1/2 (K'_ij + K'_ji), J' and K' the sums over the
