@@ -139,12 +139,13 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 // the LDS sums do not depend on scheduling either.
 // ---------------------------------------------------------------------
 // Waves per SIMD the register allocation must leave room for: the L = 3
-// classes need a few registers over 256 (VGPRs + AGPRs), which alone would
-// leave one wave per SIMD.
+// classes need a few registers over 256 (VGPRs + AGPRs), (ps|ps) one over
+// 128, which alone would leave one and three waves per SIMD (a few bytes of
+// spill per ket instead).
 template <int LA, int LB, int LC, int LD>
 constexpr int jk_min_waves()
 {
-    return (LA + LB + LC + LD == 3) ? 2 : 1;
+    return (LA + LB + LC + LD == 3) ? 2 : ((LA == 1 && LB == 0 && LC == 1 && LD == 0) ? 4 : 1);
 }
 
 template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
@@ -167,8 +168,23 @@ k_jk_tiles(QuartetArgs a)
     const int nwaves = nthr / WAVE;
     double* krow = smem;                           // [NR][ldk]
     double* sred = krow + NR * ldk;                // [4][NBRA] cross-wave J'_ab
-    double* sbra = sred + 4 * NBRA;                // bra primitive-pair records, then their screen values
+    long long* rjcd = (long long*)(sred + 4 * NBRA);   // [HFK_TASK_RUNS + 1] the task's runs: J'_cd entry,
+    int* rx = (int*)(rjcd + HFK_TASK_RUNS + 1);        //   first position,
+    int* rstart = rx + HFK_TASK_RUNS + 1;              //   index of its first ket in the task's sequence
+    double* sbra = (double*)(rstart + HFK_TASK_RUNS + 1);   // bra primitive-pair records, then their screen values
     for (int t = tid; t < NR * ldk; t += nthr) krow[t] = 0.0;
+    const int nrun = task.z - task.y;              // <= HFK_TASK_RUNS (the planner's tasks)
+    if (tid == 0) {
+        int st = 0;
+        for (int r = 0; r < nrun; ++r) {
+            const KetRun q = a.runs[task.y + r];
+            rjcd[r] = q.jcd;
+            rx[r] = q.x;
+            rstart[r] = st;
+            st += q.y - q.x;
+        }
+        rstart[nrun] = st;
+    }
     // the task's bra primitive-pair records and their per-component screen
     // values, read by every lane in every ket iteration: staged once in LDS
     const int nbs = bprim1 - bprim0;
@@ -205,14 +221,25 @@ k_jk_tiles(QuartetArgs a)
     // lanes take the kets in run order (a wave meets one primitive-pair
     // count except at run boundaries); every workgroup runs the same number
     // of rounds, so the turn barriers are uniform
-    const int len = task_ket_count(a, task);
-    KetWalk walk;
-    walk.begin(a, task);
+    const int len = rstart[nrun];
+    // run holding the task's v-th ket: binary search over the staged starts
+    auto run_of = [&](int v) {
+        int lo = 0, hi = nrun - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (rstart[mid] <= v) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
     for (int base = 0; base < len; base += nthr) {
         const int it = base + tid;
         const bool in = it < len;
         int pos = 0;
-        if (in) walk.seek(a, it, pos);
+        if (in) {
+            const int r = run_of(it);
+            pos = rx[r] + (it - rstart[r]);
+        }
         const int kp = in ? a.ket_idx[pos] : 0;
         const bool act = in && quartet_selected(a, bp, kp);
         // K' contributions: rows a (x < NA) and b, at columns c and d
@@ -336,11 +363,10 @@ k_jk_tiles(QuartetArgs a)
             }
         }
         // this ket's J'_cd entry (zero when the quartet is skipped: every
-        // entry is rewritten by every build); the run record is read again
-        // here rather than kept in registers across the integrals
-        asm volatile("" ::: "memory");
+        // entry is rewritten by every build)
         if (in) {
-            const long long entry = walk.entry(a, it);
+            const int r = run_of(it);
+            const long long entry = rjcd[r] + (it - rstart[r]);
 #pragma unroll
             for (int k = 0; k < NKET; ++k) Jcd[entry * NKET + k] = jv[k];
         }

@@ -749,7 +749,7 @@ int hfk_ctx_plan(hfk_ctx* c)
                     cost_acc += take * nbp * nkp;
                     x += take;
                     len += (int)take;
-                    if (len == cmerge) {
+                    if (len == cmerge || (long long)runs.size() - r0 == HFK_TASK_RUNS) {
                         if (!push(bl[i], r0, (nbp << 16) | 0xff, len)) return hfk_set_error(HFK_EUNSUP, "task list too long");
                         r0 = (long long)runs.size();
                         len = 0;

@@ -101,13 +101,17 @@ inline size_t hfk_bra_stage_bytes(int la, int lb, int nprim)
     return sizeof(double) * nprim * (16 + (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2));
 }
 
+// Runs of one J/K task at most (the per-lane kernel stages them in LDS).
+#define HFK_TASK_RUNS 64
 // LDS bytes of a per-lane J/K workgroup: K' rows of the bra shells over the
 // launch's ncols compact columns (+ one discarded), the cross-wave J'_ab
-// sums, the staged bra primitive pairs.
+// sums, the task's runs (first ket, entry, start), the staged bra
+// primitive pairs.
 inline size_t hfk_jk_lds_bytes(int la, int lb, int ncols, int bra_stage)
 {
     const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
-    return sizeof(double) * ((size_t)(na + nb) * (ncols + 1) + 4 * (size_t)na * nb) + hfk_bra_stage_bytes(la, lb, bra_stage);
+    return sizeof(double) * ((size_t)(na + nb) * (ncols + 1) + 4 * (size_t)na * nb) +
+           (sizeof(long long) + 2 * sizeof(int)) * (HFK_TASK_RUNS + 1) + hfk_bra_stage_bytes(la, lb, bra_stage);
 }
 // Dynamic LDS of a cooperative J/K workgroup (its static tables come on top).
 inline size_t hfk_coop_jk_lds_bytes(int la, int lb, int ncols)

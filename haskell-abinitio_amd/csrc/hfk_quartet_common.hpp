@@ -51,33 +51,6 @@ __device__ __forceinline__ int task_ket_pos(const QuartetArgs& a, const int4& ta
     return -1;
 }
 
-// A lane's walk over the task's kets v = v0, v0 + step, ... (increasing):
-// the run holding v is found by advancing from the previous one.  Only the
-// run index and the sequence index of its first ket are kept; the run
-// record itself is read when needed.
-struct KetWalk {
-    int r;          // run holding the current ket
-    int start;      // task-sequence index of that run's first ket
-    __device__ __forceinline__ void begin(const QuartetArgs& a, const int4& task)
-    {
-        r = task.y;
-        start = 0;
-    }
-    // position of the task's v-th ket (v below the task's count)
-    __device__ __forceinline__ void seek(const QuartetArgs& a, int v, int& pos)
-    {
-        KetRun q = a.runs[r];
-        while (v >= start + (q.y - q.x)) {
-            start += q.y - q.x;
-            ++r;
-            q = a.runs[r];
-        }
-        pos = q.x + (v - start);
-    }
-    // J'_cd entry of the v-th ket (after seek(v))
-    __device__ __forceinline__ long long entry(const QuartetArgs& a, int v) const { return a.runs[r].jcd + (v - start); }
-};
-
 // End of the task's ket primitive range for ket pair kp (k1 = 255: all).
 __device__ __forceinline__ int task_kprim_end(const QuartetArgs& a, int k1, int kp)
 {
