@@ -234,6 +234,8 @@ extern "C" int hfk_set_options(hfk_ctx* c, const hfk_options* opt)
 
 int hfk_ctx_init_device(hfk_ctx* c)
 {
+    HFK_HIP(hipGetDevice(&c->device));
+    HFK_TRY(hfk_noalloc_setup(c->device));
     HFK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (int k = 0; k < hfk_ctx::NAUX; ++k) {
         HFK_HIP(hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking));
@@ -999,13 +1001,10 @@ int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
     // fixed-order reduction of its J'_cd entries.
     const bool conc = op == HFK_OP_JK && c->concurrent && np > 2 && c->first_launch >= 0;
     const size_t first = conc ? (size_t)c->first_launch : 0;
-    std::vector<size_t> order;
-    if (conc) order.push_back(first);
-    for (size_t i = 0; i < np; ++i)
-        if (!conc || i != first) order.push_back(i);
     int slot = 0, missing = -1, lds_fail = -1;
-    for (size_t oi = 0; oi < order.size() && missing < 0 && lds_fail < 0; ++oi) {
-        const size_t i = order[oi];
+    for (size_t oi = 0; oi < np && missing < 0 && lds_fail < 0; ++oi) {
+        // launch order: the first launch, then the others in plan order
+        const size_t i = !conc ? oi : oi == 0 ? first : oi <= first ? oi - 1 : oi;
         const PlanEntry& e = c->plan[i];
         if (conc && oi == 1) {
             HFK_HIP(hipEventRecord(c->ev_fork, c->stream));

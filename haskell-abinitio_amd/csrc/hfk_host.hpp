@@ -71,6 +71,7 @@ struct PlanEntry {
 
 struct hfk_ctx {
     std::mutex mu;
+    int device = -1;            // HIP device current at hfk_create
     int n = 0, natom = 0;
     hfk_options opt;
     // basis as given
@@ -190,6 +191,8 @@ void dev_free(hfk_ctx* c, T* p)
 #define dev_alloc(p, n) dev_alloc_tracked(c, p, n)
 
 int hfk_ctx_init_device(hfk_ctx* c);
+// hfk_noalloc.hip: this device's copy of the fixed scratch array (once per device)
+int hfk_noalloc_setup(int device);
 void hfk_ctx_free_device(hfk_ctx* c);
 int hfk_ctx_classify_pairs(hfk_ctx* c);
 void hfk_pair_class_l(int k, int* la, int* lb);

@@ -18,7 +18,7 @@ EXPORTS = (
     "hfk_default_options", "hfk_create", "hfk_destroy", "hfk_set_options", "hfk_eri_flat",
     "hfk_fock_G", "hfk_fock", "hfk_fock_dev", "hfk_density", "hfk_symm_ortho", "hfk_unitary",
     "hfk_overlap", "hfk_hcore", "hfk_default_scf_options", "hfk_scf", "hfk_gradient",
-    "hfk_get_stats", "hfk_get_class_stats", "hfk_last_error",
+    "hfk_get_stats", "hfk_get_class_stats", "hfk_last_error", "hfk_eri_flat_noalloc",
 )
 
 
@@ -93,6 +93,7 @@ def lib():
     L.hfk_destroy.argtypes = [vp]
     L.hfk_set_options.argtypes = [vp, C.POINTER(HfkOptions)]
     L.hfk_eri_flat.argtypes = [vp, dp, C.c_size_t]
+    L.hfk_eri_flat_noalloc.argtypes = [vp, vp, C.c_size_t]
     L.hfk_fock_G.argtypes = [vp, dp, dp]
     L.hfk_fock.argtypes = [vp, dp, dp, dp]
     L.hfk_fock_dev.argtypes = [vp, vp, vp, vp]

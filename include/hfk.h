@@ -96,6 +96,19 @@ int hfk_set_options(hfk_ctx *ctx, const hfk_options *opt);
  * doubles in calcIntegrals order.  Meant for drop-in parity at small nbf. */
 int hfk_eri_flat(hfk_ctx *ctx, double *out, size_t out_len);
 
+/* No counterpart in the reference: hfk_eri_flat for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same calcIntegrals array,
+ * written to out_dev (device memory of the context's device, out_len =
+ * M(M+1)/2 doubles, M = nbf(nbf+1)/2).  The call allocates nothing: its one
+ * scratch table lives in a fixed device array of the library, so nbf is at
+ * most HFK_NOALLOC_MAX_NBF (HFK_EINVAL above).  All calls on one GPU share
+ * that array: calls that overlap on one GPU race.  Runs on the context's
+ * stream (work already queued for out_dev elsewhere must have completed) and
+ * returns when the array is written. */
+#define HFK_NOALLOC_MAX_NBF 256
+int hfk_eri_flat_noalloc(hfk_ctx *ctx, double *out_dev, size_t out_len);
+
 /* calcGmatrix (HartreeFock.hs:225-241): G_ab = sum_kl D_lk [(ab|kl) - 1/2 (al|kb)],
  * integral-direct.  D_flat, G_flat: flat, nbf(nbf+1)/2. */
 int hfk_fock_G(hfk_ctx *ctx, const double *D_flat, double *G_flat);
