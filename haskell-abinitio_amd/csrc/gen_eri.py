@@ -99,31 +99,30 @@ def gen_class(la, lb, lc, ld):
     out.append("// (%s%s|%s%s): %d integrals, L=%d" % ("spdf"[la], "spdf"[lb], "spdf"[lc], "spdf"[ld], nbra * nket, L))
     out.append("// bra: nbp primitive-pair records (LDS or global); bq: their per-component")
     out.append("// screen values, bq[ib * %d + ab] (MODE 1)." % nbra)
-    out.append("template <typename T, int BOYS, int MODE>")
+    out.append("// kr: the ket's primitive-pair records (KetAoS or KetSoA).")
+    out.append("template <typename T, int BOYS, int MODE, typename KR>")
     out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, const double* __restrict__ bq, int nbp," % name)
-    out.append("    const double* __restrict__ ket, int nkp, const double* __restrict__ qcomp, const BoysTables& bt,")
-    out.append("    T* __restrict__ V, double qtau)")
+    out.append("    const KR& kr, int nkp, const BoysTables& bt, T* __restrict__ V, double qtau)")
     out.append("{")
     out.append("#pragma unroll")
     out.append("  for (int z = 0; z < %d; ++z) V[z] = T(0);" % (nbra * nket))
     # ket primitive pairs outer (per-lane data, loaded once each); bra
     # primitive pairs inner (wave-uniform: scalar loads)
     out.append("  for (int ik = 0; ik < nkp; ++ik) {")
-    out.append("    const double* pk = ket + (size_t)ik * PP_STRIDE;")
+    out.append("    const double* pk = kr.prim(ik);")
     em = Emitter()
-    em.raw("    const double kqlo = pk[PP_QLO], kqhi = pk[PP_QHI];")
-    em.raw("    const T kq = T(pk[PP_P]);")
-    em.raw("    const T kQx = T(pk[PP_PX]), kQy = T(pk[PP_PY]), kQz = T(pk[PP_PZ]);")
-    em.raw("    const T kCx = T(pk[PP_AX]), kCy = T(pk[PP_AY]), kCz = T(pk[PP_AZ]);")
-    em.raw("    const T kDx = T(pk[PP_BX]), kDy = T(pk[PP_BY]), kDz = T(pk[PP_BZ]);")
-    em.raw("    const T kK = T(pk[PP_K]);")
-    em.raw("    const T koo2q = T(pk[PP_OO2P]);")
+    em.raw("    const double kqlo = kr.at(pk, PP_QLO), kqhi = kr.at(pk, PP_QHI);")
+    em.raw("    const T kq = T(kr.at(pk, PP_P));")
+    em.raw("    const T kQx = T(kr.at(pk, PP_PX)), kQy = T(kr.at(pk, PP_PY)), kQz = T(kr.at(pk, PP_PZ));")
+    em.raw("    const T kCx = T(kr.at(pk, PP_AX)), kCy = T(kr.at(pk, PP_AY)), kCz = T(kr.at(pk, PP_AZ));")
+    em.raw("    const T kDx = T(kr.at(pk, PP_BX)), kDy = T(kr.at(pk, PP_BY)), kDz = T(kr.at(pk, PP_BZ));")
+    em.raw("    const T kK = T(kr.at(pk, PP_K));")
+    em.raw("    const T koo2q = T(kr.at(pk, PP_OO2P));")
     if nket > 1 or nbra > 1:
         em.raw("    double kqv[%d];" % nket)
         em.raw("    if constexpr (MODE == 1) {")
-        em.raw("      const int kqoff = (int)pk[PP_QOFF];")
         em.raw("#pragma unroll")
-        em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = qcomp[kqoff + z];" % nket)
+        em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = kr.q(pk, z);" % nket)
         em.raw("    }")
     Ek = [hermite_E(em, lc, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
     out.extend(em.lines)
@@ -273,29 +272,28 @@ def gen_grad_class(la, lb, lc, ld):
     nbra, nket = na * nb, nc * nd
     name = "grad_%d%d%d%d" % (la, lb, lc, ld)
     out = []
-    out.append("template <typename T, int BOYS, int MODE>")
-    out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp," % name)
+    out.append("template <typename T, int BOYS, int MODE, typename KR>")
+    out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, int nbp, const KR& kr, int nkp," % name)
     out.append("    const double* __restrict__ qcomp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
     out.append("{")
     out.append("#pragma unroll")
     out.append("  for (int z = 0; z < 9; ++z) g[z] = T(0);")
     out.append("  for (int ik = 0; ik < nkp; ++ik) {")
-    out.append("    const double* pk = ket + (size_t)ik * PP_STRIDE;")
+    out.append("    const double* pk = kr.prim(ik);")
     em = Emitter()
-    em.raw("    const double kqlo = pk[PP_QLO], kqhi = pk[PP_QHI];")
-    em.raw("    const T kq = T(pk[PP_P]);")
-    em.raw("    const T kQx = T(pk[PP_PX]), kQy = T(pk[PP_PY]), kQz = T(pk[PP_PZ]);")
-    em.raw("    const T kCx = T(pk[PP_AX]), kCy = T(pk[PP_AY]), kCz = T(pk[PP_AZ]);")
-    em.raw("    const T kDx = T(pk[PP_BX]), kDy = T(pk[PP_BY]), kDz = T(pk[PP_BZ]);")
-    em.raw("    const T kK = T(pk[PP_K]);")
-    em.raw("    const T koo2q = T(pk[PP_OO2P]);")
-    em.raw("    const T kgam2 = T(2) * T(pk[PP_EA]);")
+    em.raw("    const double kqlo = kr.at(pk, PP_QLO), kqhi = kr.at(pk, PP_QHI);")
+    em.raw("    const T kq = T(kr.at(pk, PP_P));")
+    em.raw("    const T kQx = T(kr.at(pk, PP_PX)), kQy = T(kr.at(pk, PP_PY)), kQz = T(kr.at(pk, PP_PZ));")
+    em.raw("    const T kCx = T(kr.at(pk, PP_AX)), kCy = T(kr.at(pk, PP_AY)), kCz = T(kr.at(pk, PP_AZ));")
+    em.raw("    const T kDx = T(kr.at(pk, PP_BX)), kDy = T(kr.at(pk, PP_BY)), kDz = T(kr.at(pk, PP_BZ));")
+    em.raw("    const T kK = T(kr.at(pk, PP_K));")
+    em.raw("    const T koo2q = T(kr.at(pk, PP_OO2P));")
+    em.raw("    const T kgam2 = T(2) * T(kr.at(pk, PP_EA));")
     if nket > 1 or nbra > 1:
         em.raw("    double kqv[%d];" % nket)
         em.raw("    if constexpr (MODE == 1) {")
-        em.raw("      const int kqoff = (int)pk[PP_QOFF];")
         em.raw("#pragma unroll")
-        em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = qcomp[kqoff + z];" % nket)
+        em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = kr.q(pk, z);" % nket)
         em.raw("    }")
     Ek = [hermite_E(em, lc + 1, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
     out.extend(em.lines)
@@ -530,17 +528,16 @@ def main():
         body.append('#include "hfk_internal.hpp"')
         body.append("namespace {")
         body.append(src)
-        body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>")
+        body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, typename KR>")
         body.append("__device__ __forceinline__ void eri_class(const double* __restrict__ bra, const double* __restrict__ bq, int nbp,")
-        body.append("    const double* __restrict__ ket, int nkp, const double* __restrict__ qcomp, const BoysTables& bt,")
-        body.append("    T* __restrict__ V, double qtau)")
-        body.append("{ eri_%s<T, BOYS, MODE>(bra, bq, nbp, ket, nkp, qcomp, bt, V, qtau); }" % tag)
+        body.append("    const KR& kr, int nkp, const BoysTables& bt, T* __restrict__ V, double qtau)")
+        body.append("{ eri_%s<T, BOYS, MODE>(bra, bq, nbp, kr, nkp, bt, V, qtau); }" % tag)
         if not grad_coop:
             body.append(gen_grad_class(*q))
-            body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>")
-            body.append("__device__ __forceinline__ void grad_class(const double* __restrict__ bra, int nbp, const double* __restrict__ ket, int nkp,")
+            body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, typename KR>")
+            body.append("__device__ __forceinline__ void grad_class(const double* __restrict__ bra, int nbp, const KR& kr, int nkp,")
             body.append("    const double* __restrict__ qcomp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
-            body.append("{ grad_%s<T, BOYS, MODE>(bra, nbp, ket, nkp, qcomp, bt, G, qtau, g); }" % tag)
+            body.append("{ grad_%s<T, BOYS, MODE>(bra, nbp, kr, nkp, qcomp, bt, G, qtau, g); }" % tag)
         body.append("}  // namespace")
         body.append('#include "hfk_eri_impl.hpp"')
         if grad_coop:

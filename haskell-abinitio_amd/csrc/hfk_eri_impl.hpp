@@ -30,7 +30,8 @@ __device__ __forceinline__ void eval_quartet(const QuartetArgs& a, int bp, int k
     const int nbp = (b1 < 0 ? a.pair_npp[bp] : b1) - b0;
     const int nkp = (k1 < 0 ? a.pair_npp[kp] : k1) - k0;
     const double* bq = a.qcomp + (MODE == 1 ? (long long)bra[PP_QOFF] : 0LL);
-    eri_class<LA, LB, LC, LD, T, BOYS, MODE>(bra, bq, nbp, ket, nkp, a.qcomp, a.bt, V, a.qtau);
+    const KetAoS kr{ket, a.qcomp};
+    eri_class<LA, LB, LC, LD, T, BOYS, MODE>(bra, bq, nbp, kr, nkp, a.bt, V, a.qtau);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -117,13 +118,20 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 // ---------------------------------------------------------------------
 // Tiled J/K kernel: one workgroup per task = (bra shell pair, runs of the ket
 // class list (each of one primitive-pair count), primitive-pair ranges of
-// bra and ket).  Each lane loops over its ket primitive pairs (vector loads,
-// once each) and, inside, over the bra primitive pairs (staged in LDS,
-// wave-uniform).  Splitting heavy quartets into primitive ranges evens out
-// the work per lane; the digestion is linear in the integrals, so partial
-// quartets are digested independently.  D is (n+1) x (n+1) with a zero
-// row/column n that absent Cartesian components point to, so the digestion
-// has no branches.  Per quartet:
+// bra and ket).  Each lane loops over its ket primitive pairs and, inside,
+// over the bra primitive pairs (staged in LDS, wave-uniform).  Splitting
+// heavy quartets into primitive ranges evens out the work per lane; the
+// digestion is linear in the integrals, so partial quartets are digested
+// independently.  D is (n+1) x (n+1) with a zero row/column n that absent
+// Cartesian components point to, so the digestion has no branches.
+//
+// Lanes take consecutive positions of a run, and everything a lane reads per
+// ket is laid out by ket position: the ket's primitive-pair records
+// (position-major copy of the class), its descriptor (function indices,
+// compact columns, flags) and screen values, so those loads are coalesced.
+// The density elements D_ac, D_ad, D_bc, D_bd come from the bra shells' rows
+// staged in LDS over the compact columns (drows); only D_cd is gathered.
+// Per quartet:
 //   J'_ab += 2f sum_cd V D_cd   lane registers; at the end one wave
 //                               reduction, the waves summed in order, one
 //                               store to the task's slot
@@ -134,9 +142,9 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 //                               slot at the end
 // No global atomics: the slots are summed in a fixed order afterwards, so a
 // Fock build is bitwise reproducible.  In a workgroup of several waves the
-// waves add into the LDS rows in turn (wave 0, then 1, ...; a barrier
-// between turns); within one wave the hardware orders the lanes' adds, so
-// the LDS sums do not depend on scheduling either.
+// waves add into the LDS rows in turn (a barrier between turns); within one
+// wave the hardware orders the lanes' adds, so the LDS sums do not depend on
+// scheduling either.
 // ---------------------------------------------------------------------
 // Waves per SIMD the register allocation must leave room for: the L = 3
 // classes need a few registers over 256 (VGPRs + AGPRs), (ps|ps) one over
@@ -155,6 +163,7 @@ k_jk_tiles(QuartetArgs a)
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NKET = NC * ND, NBRA = NA * NB;
     constexpr int NR = NA + NB;
+    constexpr int NF = hfk_ket_fields(LC, LD);
     extern __shared__ double smem[];
     const long long tix = a.task0 + blockIdx.x;    // slot of this task
     const int4 task = a.tasks[blockIdx.x];
@@ -163,15 +172,20 @@ k_jk_tiles(QuartetArgs a)
     const int bprim0 = (task.w >> 24) & 0xff, bprim1 = (task.w >> 16) & 0xff;
     const int kprim0 = (task.w >> 8) & 0xff, kprim1 = task.w & 0xff;
     const int ld = a.n + 1;
-    const int ldk = a.ncols + 1;                   // LDS row: compact columns + one discarded
+    const int ncols = a.ncols, ldk = ncols + 1;    // LDS row: compact columns + one discarded
     const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
     const int nwaves = nthr / WAVE;
-    double* krow = smem;                           // [NR][ldk]
-    double* sred = krow + NR * ldk;                // [4][NBRA] cross-wave J'_ab
+    const bool drows = a.drows != 0;
+    double* krow = smem;                           // [NR][ldk] K' rows
+    double* drow = krow + NR * ldk;                // [NR][ldk] D rows of the bra shells (drows)
+    double* sred = drow + (drows ? NR * ldk : 0);  // [4][NBRA] cross-wave J'_ab
     long long* rjcd = (long long*)(sred + 4 * NBRA);   // [HFK_TASK_RUNS + 1] the task's runs: J'_cd entry,
-    int* rx = (int*)(rjcd + HFK_TASK_RUNS + 1);        //   first position,
-    int* rstart = rx + HFK_TASK_RUNS + 1;              //   index of its first ket in the task's sequence
-    double* sbra = (double*)(rstart + HFK_TASK_RUNS + 1);   // bra primitive-pair records, then their screen values
+    long long* rsoa = rjcd + HFK_TASK_RUNS + 1;        //   record base,
+    int* rx = (int*)(rsoa + HFK_TASK_RUNS + 1);        //   first position,
+    int* rstart = rx + HFK_TASK_RUNS + 1;              //   index of its first ket in the task's sequence,
+    int* rlen = rstart + HFK_TASK_RUNS + 1;            //   segment length,
+    int* rnpp = rlen + HFK_TASK_RUNS + 1;              //   primitive pairs per ket
+    double* sbra = (double*)(rnpp + HFK_TASK_RUNS + 1);   // bra primitive-pair records, then their screen values
     for (int t = tid; t < NR * ldk; t += nthr) krow[t] = 0.0;
     const int nrun = task.z - task.y;              // <= HFK_TASK_RUNS (the planner's tasks)
     if (tid == 0) {
@@ -179,11 +193,31 @@ k_jk_tiles(QuartetArgs a)
         for (int r = 0; r < nrun; ++r) {
             const KetRun q = a.runs[task.y + r];
             rjcd[r] = q.jcd;
+            rsoa[r] = q.soa;
             rx[r] = q.x;
             rstart[r] = st;
+            rlen[r] = q.len;
+            rnpp[r] = q.npp;
             st += q.y - q.x;
         }
         rstart[nrun] = st;
+    }
+    const int A = a.pair_A[bp], B = a.pair_B[bp];
+    int ia[NA], ib[NB];
+    double sa[NA], sb[NB];
+#pragma unroll
+    for (int x = 0; x < NA; ++x) { ia[x] = a.sh_bfp[A * HFK_MAXCOMP + x]; sa[x] = a.sh_scale[A * HFK_MAXCOMP + x]; }
+#pragma unroll
+    for (int x = 0; x < NB; ++x) { ib[x] = a.sh_bfp[B * HFK_MAXCOMP + x]; sb[x] = a.sh_scale[B * HFK_MAXCOMP + x]; }
+    const double* __restrict__ D = a.D;
+    // the bra shells' density rows over the compact columns (the discarded
+    // column reads the zero column n)
+    if (drows) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double* Drow = D + (size_t)(r < NA ? ia[r] : ib[r - NA]) * ld;
+            for (int col = tid; col < ldk; col += nthr) drow[r * ldk + col] = col < ncols ? Drow[a.collist[col]] : 0.0;
+        }
     }
     // the task's bra primitive-pair records and their per-component screen
     // values, read by every lane in every ket iteration: staged once in LDS
@@ -199,14 +233,6 @@ k_jk_tiles(QuartetArgs a)
     }
     __syncthreads();
 
-    const int A = a.pair_A[bp], B = a.pair_B[bp];
-    int ia[NA], ib[NB];
-    double sa[NA], sb[NB];
-#pragma unroll
-    for (int x = 0; x < NA; ++x) { ia[x] = a.sh_bfp[A * HFK_MAXCOMP + x]; sa[x] = a.sh_scale[A * HFK_MAXCOMP + x]; }
-#pragma unroll
-    for (int x = 0; x < NB; ++x) { ib[x] = a.sh_bfp[B * HFK_MAXCOMP + x]; sb[x] = a.sh_scale[B * HFK_MAXCOMP + x]; }
-    const double* __restrict__ D = a.D;
     double Dab[NBRA];
 #pragma unroll
     for (int x = 0; x < NA; ++x)
@@ -216,7 +242,10 @@ k_jk_tiles(QuartetArgs a)
 #pragma unroll
     for (int k = 0; k < NBRA; ++k) jab[k] = 0.0;
     double* __restrict__ Jcd = a.jcd;
-    const int* __restrict__ colmap = a.colmap;
+    const int* __restrict__ kd = a.kdesc;
+    const int nket = (int)a.nket;
+    const bool kscl = a.kscale != nullptr;
+    const double bqhi = a.pair_qhi[bp], bsch = a.pair_sch[bp];
 
     // lanes take the kets in run order (a wave meets one primitive-pair
     // count except at run boundaries); every workgroup runs the same number
@@ -235,13 +264,12 @@ k_jk_tiles(QuartetArgs a)
     for (int base = 0; base < len; base += nthr) {
         const int it = base + tid;
         const bool in = it < len;
-        int pos = 0;
+        int r = 0, pos = 0;
         if (in) {
-            const int r = run_of(it);
+            r = run_of(it);
             pos = rx[r] + (it - rstart[r]);
         }
-        const int kp = in ? a.ket_idx[pos] : 0;
-        const bool act = in && quartet_selected(a, bp, kp);
+        const bool act = in && quartet_selected_pos(a, bqhi, bsch, pos);
         // K' contributions: rows a (x < NA) and b, at columns c and d
         double kac[NA * NC], kad[NA * ND], kbc[NB * NC], kbd[NB * ND];
         int cc[NC], cd[ND];
@@ -250,28 +278,31 @@ k_jk_tiles(QuartetArgs a)
         for (int k = 0; k < NKET; ++k) jv[k] = 0.0;
         if (act) {
             T Vt[NBRA * NKET];
-            eri_class<LA, LB, LC, LD, T, BOYS, MODE>(sbra, sbq, nbs,
-                                                     a.pp + (size_t)(a.pair_pp0[kp] + kprim0) * PP_STRIDE,
-                                                     task_kprim_end(a, kprim1, kp) - kprim0, a.qcomp, a.bt, Vt, a.qtau);
-            const int C = a.pair_A[kp], Dd = a.pair_B[kp];
+            {
+                const int slen = rlen[r];
+                const KetSoA<NF> kr{a.ksoa + rsoa[r] + pos + (size_t)kprim0 * NF * slen, slen};
+                eri_class<LA, LB, LC, LD, T, BOYS, MODE>(sbra, sbq, nbs, kr, min(kprim1, rnpp[r]) - kprim0, a.bt, Vt,
+                                                         a.qtau);
+            }
             int ic[NC], id[ND];
             double sc[NC], sd[ND];
 #pragma unroll
             for (int x = 0; x < NC; ++x) {
-                ic[x] = a.sh_bfp[C * HFK_MAXCOMP + x];
-                sc[x] = a.sh_scale[C * HFK_MAXCOMP + x];
-                cc[x] = colmap[ic[x]];
+                ic[x] = kd[x * nket + pos];
+                cc[x] = kd[(NC + ND + x) * nket + pos];
+                sc[x] = kscl ? a.kscale[x * nket + pos] : 1.0;
             }
 #pragma unroll
             for (int x = 0; x < ND; ++x) {
-                id[x] = a.sh_bfp[Dd * HFK_MAXCOMP + x];
-                sd[x] = a.sh_scale[Dd * HFK_MAXCOMP + x];
-                cd[x] = colmap[id[x]];
+                id[x] = kd[(NC + x) * nket + pos];
+                cd[x] = kd[(2 * NC + ND + x) * nket + pos];
+                sd[x] = kscl ? a.kscale[(NC + x) * nket + pos] : 1.0;
             }
+            const int flags = kd[2 * (NC + ND) * nket + pos];
             double f = 1.0;
             if (A == B) f *= 0.5;
-            if (C == Dd) f *= 0.5;
-            if (bp == kp) f *= 0.5;
+            if (flags & 1) f *= 0.5;
+            if (bp == a.ket_idx[pos]) f *= 0.5;
             double V[NBRA * NKET];
 #pragma unroll
             for (int x = 0; x < NA; ++x)
@@ -305,19 +336,36 @@ k_jk_tiles(QuartetArgs a)
             }
             // density blocks D_bd, D_bc, D_ad, D_ac
             double Dbd[NB * ND], Dbc[NB * NC], Dad[NA * ND], Dac[NA * NC];
+            if (drows) {
 #pragma unroll
-            for (int y = 0; y < NB; ++y) {
+                for (int y = 0; y < NB; ++y) {
 #pragma unroll
-                for (int w = 0; w < ND; ++w) Dbd[y * ND + w] = D[(size_t)ib[y] * ld + id[w]];
+                    for (int w = 0; w < ND; ++w) Dbd[y * ND + w] = drow[(NA + y) * ldk + cd[w]];
 #pragma unroll
-                for (int z = 0; z < NC; ++z) Dbc[y * NC + z] = D[(size_t)ib[y] * ld + ic[z]];
-            }
+                    for (int z = 0; z < NC; ++z) Dbc[y * NC + z] = drow[(NA + y) * ldk + cc[z]];
+                }
 #pragma unroll
-            for (int x = 0; x < NA; ++x) {
+                for (int x = 0; x < NA; ++x) {
 #pragma unroll
-                for (int w = 0; w < ND; ++w) Dad[x * ND + w] = D[(size_t)ia[x] * ld + id[w]];
+                    for (int w = 0; w < ND; ++w) Dad[x * ND + w] = drow[x * ldk + cd[w]];
 #pragma unroll
-                for (int z = 0; z < NC; ++z) Dac[x * NC + z] = D[(size_t)ia[x] * ld + ic[z]];
+                    for (int z = 0; z < NC; ++z) Dac[x * NC + z] = drow[x * ldk + cc[z]];
+                }
+            } else {
+#pragma unroll
+                for (int y = 0; y < NB; ++y) {
+#pragma unroll
+                    for (int w = 0; w < ND; ++w) Dbd[y * ND + w] = D[(size_t)ib[y] * ld + id[w]];
+#pragma unroll
+                    for (int z = 0; z < NC; ++z) Dbc[y * NC + z] = D[(size_t)ib[y] * ld + ic[z]];
+                }
+#pragma unroll
+                for (int x = 0; x < NA; ++x) {
+#pragma unroll
+                    for (int w = 0; w < ND; ++w) Dad[x * ND + w] = D[(size_t)ia[x] * ld + id[w]];
+#pragma unroll
+                    for (int z = 0; z < NC; ++z) Dac[x * NC + z] = D[(size_t)ia[x] * ld + ic[z]];
+                }
             }
 #pragma unroll
             for (int x = 0; x < NA; ++x) {
@@ -365,7 +413,6 @@ k_jk_tiles(QuartetArgs a)
         // this ket's J'_cd entry (zero when the quartet is skipped: every
         // entry is rewritten by every build)
         if (in) {
-            const int r = run_of(it);
             const long long entry = rjcd[r] + (it - rstart[r]);
 #pragma unroll
             for (int k = 0; k < NKET; ++k) Jcd[entry * NKET + k] = jv[k];
@@ -412,10 +459,10 @@ k_jk_tiles(QuartetArgs a)
         a.jab[(size_t)a.jab_slot[tix] * NBRA + tid] = 2.0 * v;
     }
     // the staged K' rows to the task's slot (row by row: no index division)
-    double* __restrict__ ks = a.kslot + (size_t)tix * NR * a.ncols;
+    double* __restrict__ ks = a.kslot + (size_t)tix * NR * ncols;
 #pragma unroll
     for (int r = 0; r < NR; ++r)
-        for (int col = tid; col < a.ncols; col += nthr) ks[(size_t)r * a.ncols + col] = krow[r * ldk + col];
+        for (int col = tid; col < ncols; col += nthr) ks[(size_t)r * ncols + col] = krow[r * ldk + col];
 }
 
 // 0, or -3 when the workgroup's LDS exceeds 64 KB
@@ -423,7 +470,7 @@ template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
 int launch_tiles(const QuartetArgs& a, hipStream_t s)
 {
     if (a.ntask <= 0) return 0;
-    const size_t shm = hfk_jk_lds_bytes(LA, LB, a.ncols, a.bra_stage);
+    const size_t shm = hfk_jk_lds_bytes(LA, LB, a.ncols, a.bra_stage, a.drows);
     if (shm > 64 * 1024) return -3;
     long long done = 0;
     while (done < a.ntask) {   // grid.x limit

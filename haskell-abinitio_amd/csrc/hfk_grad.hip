@@ -86,7 +86,7 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
     const int n = c->n, natom = (int)c->Z.size();
     if (natom == 0 || (int)c->atom_of_bf.size() != n)
         return hfk_set_error(HFK_EINVAL, "hfk_gradient: basis needs natom, Z, R and atom_of_bf");
-    if ((size_t)3 * natom * sizeof(double) > 64 * 1024)
+    if (hfk_grad_lds_bytes(0, 0, 0, natom, 0) > 64 * 1024)
         return hfk_set_error(HFK_EUNSUP, "hfk_gradient: %d atoms exceed the per-workgroup accumulator", natom);
     HFK_TRY(grad_upload_maps(c));
     const size_t nf = (size_t)n * (n + 1) / 2;
@@ -119,6 +119,9 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
     int slot = 0;
     for (const PlanEntry& e : c->plan) {
         hfk_plan_args(c, e, a);   // FP32 entries select their quartets; all run in FP64 here
+        // the bra shells' density rows in LDS while four waves per SIMD still fit
+        a.drows = !hfk_class_grad_coop(e.la, e.lb, e.lc, e.ld) &&
+                  hfk_grad_lds_bytes(e.la, e.lb, e.ncols, natom, 1) <= hfk_lds_budget(a.block);
         hipStream_t s = conc ? c->aux[slot++ % hfk_ctx::NAUX] : c->stream;
         if (hfk_launch_grad(e.la, e.lb, e.lc, e.ld, c->opt.boys_mode, e.mode, a, s)) {
             rc = hfk_set_error(HFK_EUNSUP, "no gradient kernel for class (%d%d|%d%d)", e.la, e.lb, e.lc, e.ld);

@@ -29,19 +29,43 @@ k_grad_tiles(QuartetArgs a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NKET = NC * ND, NBRA = NA * NB;
-    extern __shared__ double gacc[];   // [natom][3] when grad_lds
+    constexpr int NR = NA + NB;
+    constexpr int NF = hfk_ket_fields(LC, LD);
+    extern __shared__ double gsm[];
     const int4 task = a.tasks[blockIdx.x];
     const int bp = task.x;
     const int bprim0 = (task.w >> 24) & 0xff, bprim1 = (task.w >> 16) & 0xff;
     const int kprim0 = (task.w >> 8) & 0xff, kprim1 = task.w & 0xff;
     const int ld = a.n + 1;
+    const int ncols = a.ncols, ldk = ncols + 1;
     const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & (WAVE - 1);
-    const bool lds = a.grad_lds != 0;
+    const bool lds = a.grad_lds != 0, drows = a.drows != 0;
+    // LDS: [natom][3] accumulator (grad_lds), [NR][ldk] density rows of the
+    // bra shells (drows), the task's runs
+    double* gacc = gsm;
+    double* drow = gacc + (lds ? 3 * a.natom : 0);
+    long long* rsoa = (long long*)(drow + (drows ? NR * ldk : 0));
+    int* rx = (int*)(rsoa + HFK_TASK_RUNS + 1);
+    int* rstart = rx + HFK_TASK_RUNS + 1;
+    int* rlen = rstart + HFK_TASK_RUNS + 1;
+    int* rnpp = rlen + HFK_TASK_RUNS + 1;
     double* __restrict__ acc = lds ? gacc : a.grad;
     if (lds)
         for (int t = tid; t < 3 * a.natom; t += nthr) gacc[t] = 0.0;
-    __syncthreads();
-
+    const int nrun = task.z - task.y;
+    if (tid == 0) {
+        int st = 0;
+        for (int r = 0; r < nrun; ++r) {
+            const KetRun q = a.runs[task.y + r];
+            rsoa[r] = q.soa;
+            rx[r] = q.x;
+            rstart[r] = st;
+            rlen[r] = q.len;
+            rnpp[r] = q.npp;
+            st += q.y - q.x;
+        }
+        rstart[nrun] = st;
+    }
     const int A = a.pair_A[bp], B = a.pair_B[bp];
     const int atA = a.sh_atom[A], atB = a.sh_atom[B];
     int ia[NA], ib[NB];
@@ -51,6 +75,15 @@ k_grad_tiles(QuartetArgs a)
 #pragma unroll
     for (int x = 0; x < NB; ++x) { ib[x] = a.sh_bfp[B * HFK_MAXCOMP + x]; sb[x] = a.sh_scale[B * HFK_MAXCOMP + x]; }
     const double* __restrict__ D = a.D;
+    if (drows) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double* Drow = D + (size_t)(r < NA ? ia[r] : ib[r - NA]) * ld;
+            for (int col = tid; col < ldk; col += nthr) drow[r * ldk + col] = col < ncols ? Drow[a.collist[col]] : 0.0;
+        }
+    }
+    __syncthreads();
+
     double Dab[NBRA];
 #pragma unroll
     for (int x = 0; x < NA; ++x)
@@ -58,23 +91,41 @@ k_grad_tiles(QuartetArgs a)
         for (int y = 0; y < NB; ++y) Dab[x * NB + y] = D[(size_t)ia[x] * ld + ib[y]];
     const double* bra = a.pp + (size_t)(a.pair_pp0[bp] + bprim0) * PP_STRIDE;
     const int nbp = bprim1 - bprim0;   // ranges are explicit: 0 <= b0 < b1 <= npp
+    const int* __restrict__ kd = a.kdesc;
+    const int nket = (int)a.nket;
+    const bool kscl = a.kscale != nullptr;
+    const double bqhi = a.pair_qhi[bp], bsch = a.pair_sch[bp];
     double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
 
-    const int len = task_ket_count(a, task);
+    const int len = rstart[nrun];
     for (int it = tid; it < len; it += nthr) {
-        const int kp = a.ket_idx[task_ket_pos(a, task, it)];
-        if (!quartet_selected(a, bp, kp)) continue;
-        const int C = a.pair_A[kp], Dd = a.pair_B[kp];
-        int ic[NC], id[ND];
+        int lo = 0, hi = nrun - 1;   // run of the it-th ket
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (rstart[mid] <= it) lo = mid;
+            else hi = mid - 1;
+        }
+        const int r = lo, pos = rx[r] + (it - rstart[r]);
+        if (!quartet_selected_pos(a, bqhi, bsch, pos)) continue;
+        int ic[NC], id[ND], cc[NC], cd[ND];
         double sc[NC], sd[ND];
 #pragma unroll
-        for (int x = 0; x < NC; ++x) { ic[x] = a.sh_bfp[C * HFK_MAXCOMP + x]; sc[x] = a.sh_scale[C * HFK_MAXCOMP + x]; }
+        for (int x = 0; x < NC; ++x) {
+            ic[x] = kd[x * nket + pos];
+            cc[x] = kd[(NC + ND + x) * nket + pos];
+            sc[x] = kscl ? a.kscale[x * nket + pos] : 1.0;
+        }
 #pragma unroll
-        for (int x = 0; x < ND; ++x) { id[x] = a.sh_bfp[Dd * HFK_MAXCOMP + x]; sd[x] = a.sh_scale[Dd * HFK_MAXCOMP + x]; }
+        for (int x = 0; x < ND; ++x) {
+            id[x] = kd[(NC + x) * nket + pos];
+            cd[x] = kd[(2 * NC + ND + x) * nket + pos];
+            sd[x] = kscl ? a.kscale[(NC + x) * nket + pos] : 1.0;
+        }
+        const int flags = kd[2 * (NC + ND) * nket + pos];
         double f = 4.0;
         if (A == B) f *= 0.5;
-        if (C == Dd) f *= 0.5;
-        if (bp == kp) f *= 0.5;
+        if (flags & 1) f *= 0.5;
+        if (bp == a.ket_idx[pos]) f *= 0.5;
         double G[NBRA * NKET];
 #pragma unroll
         for (int x = 0; x < NA; ++x)
@@ -85,21 +136,32 @@ k_grad_tiles(QuartetArgs a)
 #pragma unroll
                     for (int w = 0; w < ND; ++w) {
                         const double dcd = D[(size_t)ic[z] * ld + id[w]];
-                        const double dac = D[(size_t)ia[x] * ld + ic[z]], dbd = D[(size_t)ib[y] * ld + id[w]];
-                        const double dad = D[(size_t)ia[x] * ld + id[w]], dbc = D[(size_t)ib[y] * ld + ic[z]];
+                        double dac, dbd, dad, dbc;
+                        if (drows) {
+                            dac = drow[x * ldk + cc[z]];
+                            dbd = drow[(NA + y) * ldk + cd[w]];
+                            dad = drow[x * ldk + cd[w]];
+                            dbc = drow[(NA + y) * ldk + cc[z]];
+                        } else {
+                            dac = D[(size_t)ia[x] * ld + ic[z]];
+                            dbd = D[(size_t)ib[y] * ld + id[w]];
+                            dad = D[(size_t)ia[x] * ld + id[w]];
+                            dbc = D[(size_t)ib[y] * ld + ic[z]];
+                        }
                         const double gam = Dab[x * NB + y] * dcd - 0.25 * (dac * dbd + dad * dbc);
                         G[((x * NB + y) * NC + z) * ND + w] = (f * sa[x] * sb[y] * sc[z] * sd[w]) * gam;
                     }
-        const double* ket = a.pp + (size_t)(a.pair_pp0[kp] + kprim0) * PP_STRIDE;
-        const int nkp = task_kprim_end(a, kprim1, kp) - kprim0;
+        const int slen = rlen[r];
+        const KetSoA<NF> kr{a.ksoa + rsoa[r] + pos + (size_t)kprim0 * NF * slen, slen};
         double g[9];
-        grad_class<LA, LB, LC, LD, double, BOYS, MODE>(bra, nbp, ket, nkp, a.qcomp, a.bt, G, a.qtau, g);
+        grad_class<LA, LB, LC, LD, double, BOYS, MODE>(bra, nbp, kr, min(kprim1, rnpp[r]) - kprim0, a.qcomp, a.bt, G,
+                                                       a.qtau, g);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             gA[k] += g[k];
             gB[k] += g[3 + k];
         }
-        const int atC = a.sh_atom[C], atD = a.sh_atom[Dd];
+        const int atC = kd[(2 * (NC + ND) + 1) * nket + pos], atD = kd[(2 * (NC + ND) + 2) * nket + pos];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             atomic_add_f64(&acc[3 * atC + k], g[6 + k]);
@@ -132,7 +194,7 @@ template <int LA, int LB, int LC, int LD, int BOYS, int MODE>
 void launch_grad_tiles(const QuartetArgs& a, hipStream_t s)
 {
     if (a.ntask <= 0) return;
-    const size_t shm = a.grad_lds ? sizeof(double) * 3 * (size_t)a.natom : 0;
+    const size_t shm = hfk_grad_lds_bytes(LA, LB, a.ncols, a.grad_lds ? a.natom : 0, a.drows);
     long long done = 0;
     while (done < a.ntask) {
         const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);

@@ -450,6 +450,7 @@ int hfk_ctx_classify_pairs(hfk_ctx* c)
         });
     for (int k = 0; k < HFK_NPC; ++k) {
         if (c->d_class[k]) { dev_free(c, c->d_class[k]); c->d_class[k] = nullptr; }
+        if (c->d_ksoa[k]) { dev_free(c, c->d_ksoa[k]); c->d_ksoa[k] = nullptr; }
         c->class_count[k] = (long long)lists[k].size();
         c->h_class[k] = lists[k];
         if (!lists[k].empty()) HFK_TRY(dev_upload(c, &c->d_class[k], lists[k]));
@@ -622,6 +623,14 @@ int hfk_ctx_plan(hfk_ctx* c)
         };
         const auto segs = segments(kl), bsegs = segments(bl);
         const int nkseg = (int)segs.size(), nbseg = (int)bsegs.size();
+        // position-major ket records of the class: segment s at soa_base[s],
+        // field f of primitive ik of position p at soa_base[s] + (p - first)
+        // + (ik * NF + f) * len
+        const int NF = hfk_ket_fields(e.lc, e.ld);
+        std::vector<long long> soa_base(nkseg + 1, 0);
+        for (int sg = 0; sg < nkseg; ++sg)
+            soa_base[sg + 1] = soa_base[sg] + (long long)NF * c->h_pair_npp[kl[segs[sg].first]] *
+                                                  (segs[sg].second - segs[sg].first);
         // surviving kets of bra i in segment sg: [sg.first, hi) by binary search
         auto staircase = [&](size_t i, const std::pair<long long, long long>& sg) -> long long {
             const double qi = c->h_pair_qhi[bl[i]];
@@ -727,11 +736,14 @@ int hfk_ctx_plan(hfk_ctx* c)
             e.task_len_max = std::max(e.task_len_max, len);
             return true;
         };
-        auto run_of = [](long long x, long long y, long long jcd) {
+        auto run_of = [&](long long x, long long y, long long jcd, int sg) {
             KetRun r;
             r.x = (int)x;
             r.y = (int)y;
             r.jcd = jcd;
+            r.soa = soa_base[sg] - segs[sg].first;
+            r.len = (int)(segs[sg].second - segs[sg].first);
+            r.npp = c->h_pair_npp[kl[segs[sg].first]];
             return r;
         };
         for (size_t i = 0; i < bl.size(); ++i) {
@@ -747,7 +759,7 @@ int hfk_ctx_plan(hfk_ctx* c)
                 const long long x1 = jend[i * nkseg + s];
                 while (x < x1) {
                     const long long take = std::min<long long>(x1 - x, cmerge - len);
-                    runs.push_back(run_of(x, x + take, jbase[i * nkseg + s] + (x - sg.first)));
+                    runs.push_back(run_of(x, x + take, jbase[i * nkseg + s] + (x - sg.first), s));
                     cost_acc += take * nbp * nkp;
                     x += take;
                     len += (int)take;
@@ -778,7 +790,7 @@ int hfk_ctx_plan(hfk_ctx* c)
                         for (long long j0 = k0; j0 < k1; j0 += chunk) {
                             const long long j1 = std::min<long long>(k1, j0 + chunk);
                             const long long rr = (long long)runs.size();
-                            runs.push_back(run_of(j0, j1, jbase[i * nkseg + s] + (long long)piece * (k1 - k0) + (j0 - k0)));
+                            runs.push_back(run_of(j0, j1, jbase[i * nkseg + s] + (long long)piece * (k1 - k0) + (j0 - k0), s));
                             cost_acc = (j1 - j0) * (long long)(b1 - b0) * (q1 - q0);
                             if (!push(bl[i], rr, w, (int)(j1 - j0))) return hfk_set_error(HFK_EUNSUP, "task list too long");
                         }
@@ -808,14 +820,18 @@ int hfk_ctx_plan(hfk_ctx* c)
             tasks.swap(sorted);
         }
         e.ntask = (long long)tasks.size();
-        // LDS of one workgroup: the K' rows must fit
+        // LDS of one workgroup: the K' rows must fit; the bra shells' density
+        // rows join them while the workgroups of four waves per SIMD still fit
+        // in the 160 KB of a CU
         {
             const size_t lds = coop ? hfk_coop_jk_lds_bytes(e.la, e.lb, e.ncols) + 16 * 1024
-                                    : hfk_jk_lds_bytes(e.la, e.lb, e.ncols, e.bra_stage);
+                                    : hfk_jk_lds_bytes(e.la, e.lb, e.ncols, e.bra_stage, 0);
             if (lds > 64 * 1024 || e.ncols > 4096)
                 return hfk_set_error(HFK_EUNSUP,
                                      "(%d%d|%d%d): K rows of %d columns need %zu bytes of LDS per workgroup (at most 64 KB)",
                                      e.la, e.lb, e.lc, e.ld, e.ncols, lds);
+            e.drows = !coop && hfk_jk_lds_bytes(e.la, e.lb, e.ncols, e.bra_stage, 1) <= hfk_lds_budget(block) &&
+                      !getenv("HFK_NO_DROWS");
         }
         // reduction lists: the tasks of each bra shell (K' rows) and of each bra pair (J'_ab), in task order
         std::vector<int> pos_of(c->npair, -1);
@@ -847,6 +863,63 @@ int hfk_ctx_plan(hfk_ctx* c)
         }
         HFK_TRY(entry_upload(c, e, &e.d_colmap, colmap));
         HFK_TRY(entry_upload(c, e, &e.d_collist, collist));
+        if (!coop) {
+            // the class's position-major records (once per class)
+            if (!c->d_ksoa[e.kk]) {
+                std::vector<long long> pos_soa(kl.size());
+                std::vector<int> pos_len(kl.size()), pos_npp(kl.size());
+                for (int sg = 0; sg < nkseg; ++sg)
+                    for (long long j = segs[sg].first; j < segs[sg].second; ++j) {
+                        pos_soa[j] = soa_base[sg] - segs[sg].first;
+                        pos_len[j] = (int)(segs[sg].second - segs[sg].first);
+                        pos_npp[j] = c->h_pair_npp[kl[j]];
+                    }
+                long long* d_ps = nullptr;
+                int *d_pl = nullptr, *d_pn = nullptr;
+                HFK_TRY(dev_upload(c, &d_ps, pos_soa));
+                HFK_TRY(dev_upload(c, &d_pl, pos_len));
+                HFK_TRY(dev_upload(c, &d_pn, pos_npp));
+                HFK_TRY(dev_alloc(&c->d_ksoa[e.kk], (size_t)std::max(1LL, soa_base[nkseg])));
+                hfk_launch_ket_soa(c->d_pp, c->d_qcomp, c->d_pair_pp0, c->d_class[e.kk], (int)kl.size(), d_ps, d_pl, d_pn,
+                                   NF, nket_comp, c->d_ksoa[e.kk], c->stream);
+                HFK_HIP(hipGetLastError());
+                dev_free(c, d_pn);
+                dev_free(c, d_pl);
+                dev_free(c, d_ps);
+            }
+            // per-position descriptors, screen values and component scales
+            const int W = hfk_kdesc_words(NC, ND);
+            const size_t nk = kl.size();
+            std::vector<int> kdesc((size_t)W * nk);
+            std::vector<double> kposq(2 * nk), kscale((size_t)(NC + ND) * nk);
+            bool unit = true;
+            for (size_t j = 0; j < nk; ++j) {
+                const int p = kl[j], C = c->h_pair_A[p], Dd = c->h_pair_B[p];
+                const HostShell &hc = c->shells[C], &hd = c->shells[Dd];
+                for (int x = 0; x < NC; ++x) {
+                    const int f = hc.bf[x] >= 0 ? hc.bf[x] : c->n;
+                    kdesc[(size_t)x * nk + j] = f;
+                    kdesc[(size_t)(NC + ND + x) * nk + j] = colmap[f];
+                    kscale[(size_t)x * nk + j] = hc.scale[x];
+                    unit = unit && hc.scale[x] == 1.0;
+                }
+                for (int x = 0; x < ND; ++x) {
+                    const int f = hd.bf[x] >= 0 ? hd.bf[x] : c->n;
+                    kdesc[(size_t)(NC + x) * nk + j] = f;
+                    kdesc[(size_t)(2 * NC + ND + x) * nk + j] = colmap[f];
+                    kscale[(size_t)(NC + x) * nk + j] = hd.scale[x];
+                    unit = unit && hd.scale[x] == 1.0;
+                }
+                kdesc[(size_t)(2 * (NC + ND)) * nk + j] = C == Dd ? 1 : 0;
+                kdesc[(size_t)(2 * (NC + ND) + 1) * nk + j] = hc.atom;
+                kdesc[(size_t)(2 * (NC + ND) + 2) * nk + j] = hd.atom;
+                kposq[j] = c->h_pair_qhi[p];
+                kposq[nk + j] = c->h_pair_sch[p];
+            }
+            HFK_TRY(entry_upload(c, e, &e.d_kdesc, kdesc));
+            HFK_TRY(entry_upload(c, e, &e.d_kpos_q, kposq));
+            if (!unit) HFK_TRY(entry_upload(c, e, &e.d_kscale, kscale));
+        }
         HFK_TRY(entry_upload(c, e, &e.d_kcsr_off, koff));
         HFK_TRY(entry_upload(c, e, &e.d_kcsr, kcsr));
         HFK_TRY(entry_upload(c, e, &e.d_bcsr_off, boff));
@@ -975,6 +1048,12 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
     a.jcd = e.d_jcd;
     a.colmap = e.d_colmap;
     a.ncols = e.ncols;
+    a.ksoa = c->d_ksoa[e.kk];
+    a.kdesc = e.d_kdesc;
+    a.kpos_q = e.d_kpos_q;
+    a.kscale = e.d_kscale;
+    a.collist = e.d_collist;
+    a.drows = e.drows;
     a.bra_list = c->d_class[e.kb];
     a.ket_list = c->d_class[e.kk];
     a.nbra = c->class_count[e.kb];

@@ -55,6 +55,11 @@ struct PlanEntry {
     int ncols;                  // compact K columns: functions of the ket class's shells
     int* d_colmap;              // [n+1] function -> column (n -> ncols)
     int* d_collist;             // [ncols]
+    // per-lane kernels: ket data by class-list position (QuartetArgs)
+    int* d_kdesc;               // [hfk_kdesc_words][nket]
+    double* d_kpos_q;           // [2][nket]
+    double* d_kscale;           // [NC + ND][nket], null when every scale is 1
+    int drows;                  // the J/K kernel stages the bra shells' density rows
     double* d_kslot;            // [ntask][NA+NB][ncols]
     double* d_jab;              // [ntask][NA*NB]
     double* d_jcd;              // [njcd][NC*ND]
@@ -87,6 +92,9 @@ struct hfk_ctx {
     std::vector<int> h_class[HFK_NPC];
     long long class_count[HFK_NPC] = {0};
     int* d_class[HFK_NPC] = {nullptr};
+    // position-major copies of the classes' primitive-pair records, for the
+    // classes that are kets of per-lane launches (KetRun.soa); built with the plan
+    double* d_ksoa[HFK_NPC] = {nullptr};
     // device data
     hipStream_t stream = nullptr;
     // J/K launches of one Fock build run concurrently on these (forked from
@@ -141,6 +149,13 @@ struct hfk_ctx {
 inline int hfk_plan_block(const hfk_ctx* c, const PlanEntry& e)
 {
     return (c->n >= 200 && !hfk_class_coop(e.la, e.lb, e.lc, e.ld)) ? 256 : 64;
+}
+// LDS a per-lane workgroup of `block` lanes may take so that four waves per
+// SIMD (16 per CU) fit in a CU's 160 KB; at most 64 KB.
+inline size_t hfk_lds_budget(int block)
+{
+    const size_t b = (size_t)160 * 1024 * block / (16 * 64);
+    return b < 64 * 1024 ? b : 64 * 1024;
 }
 
 template <typename T>

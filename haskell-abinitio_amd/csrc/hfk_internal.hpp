@@ -10,11 +10,25 @@
 
 // Ket run of a task: positions [x, y) of the ket class list, walked in
 // order; jcd is the J'_cd slot entry of position x (position p writes entry
-// jcd + p - x, NKET doubles each).
+// jcd + p - x, NKET doubles each).  The positions of a run lie in one ket
+// segment (equal primitive-pair count npp, len positions); field f of
+// primitive-pair ik of position p is ksoa[soa + p + (ik * NF + f) * len]
+// in the class's position-major copy of the records (per-lane kernels).
 struct KetRun {
     int x, y;
     long long jcd;
+    long long soa;
+    int len, npp;
 };
+
+// Fields of a position-major ket record: the PP_STRIDE fields of the
+// primitive-pair record, then its per-component screen values.
+constexpr int hfk_ket_fields(int lc, int ld) { return 16 + (lc + 1) * (lc + 2) / 2 * ((ld + 1) * (ld + 2) / 2); }
+// Per-position ket descriptor of a launch, kdesc[x * nket + p] for word x:
+// the padded basis-function indices of shells C (NC words) and D (ND), their
+// compact K columns (NC, ND), a flag word (bit 0: C == D), the atoms of C
+// and D.
+constexpr int hfk_kdesc_words(int nc, int nd) { return 2 * (nc + nd) + 3; }
 
 // Everything a quartet kernel needs.  Pair p: shells pair_A[p] (the one
 // with the larger angular momentum) and pair_B[p]; its primitive pairs are
@@ -60,6 +74,12 @@ struct QuartetArgs {
     const int4* tasks;
     const KetRun* runs;         // runs of the tasks
     const int* ket_idx;         // ket pair indices (the ket class list)
+    const double* ksoa;         // position-major ket records of the ket class (KetRun.soa)
+    const int* kdesc;           // [hfk_kdesc_words][nket] per-position ket descriptor
+    const double* kpos_q;       // [2][nket]: pair_qhi, pair_sch of each ket position
+    const double* kscale;       // [NC + ND][nket] component scales of C, D (null: all 1)
+    const int* collist;         // [ncols] basis function of each compact column
+    int drows;                  // stage the bra shells' density rows in LDS (compact columns)
     long long ntask;
     int block;                  // workgroup size of the tiled kernel (64 or 256)
     int bra_stage;              // longest bra primitive-pair range of the tasks (LDS staging)
@@ -101,17 +121,27 @@ inline size_t hfk_bra_stage_bytes(int la, int lb, int nprim)
     return sizeof(double) * nprim * (16 + (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2));
 }
 
-// Runs of one J/K task at most (the per-lane kernel stages them in LDS).
+// Runs of one J/K task at most (the per-lane kernel stages them in LDS:
+// J'_cd entry, record base, first position, start in the task's sequence,
+// segment length, primitive-pair count).
 #define HFK_TASK_RUNS 64
+inline size_t hfk_run_stage_bytes() { return (2 * sizeof(long long) + 4 * sizeof(int)) * (HFK_TASK_RUNS + 1); }
 // LDS bytes of a per-lane J/K workgroup: K' rows of the bra shells over the
 // launch's ncols compact columns (+ one discarded), the cross-wave J'_ab
 // sums, the task's runs (first ket, entry, start), the staged bra
 // primitive pairs.
-inline size_t hfk_jk_lds_bytes(int la, int lb, int ncols, int bra_stage)
+inline size_t hfk_jk_lds_bytes(int la, int lb, int ncols, int bra_stage, int drows)
 {
     const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
-    return sizeof(double) * ((size_t)(na + nb) * (ncols + 1) + 4 * (size_t)na * nb) +
-           (sizeof(long long) + 2 * sizeof(int)) * (HFK_TASK_RUNS + 1) + hfk_bra_stage_bytes(la, lb, bra_stage);
+    return sizeof(double) * ((size_t)(drows ? 2 : 1) * (na + nb) * (ncols + 1) + 4 * (size_t)na * nb) +
+           hfk_run_stage_bytes() + hfk_bra_stage_bytes(la, lb, bra_stage);
+}
+// LDS bytes of a per-lane gradient workgroup: the per-atom accumulator, the
+// bra shells' density rows (drows), the task's runs.
+inline size_t hfk_grad_lds_bytes(int la, int lb, int ncols, int natom, int drows)
+{
+    const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
+    return sizeof(double) * (3 * (size_t)natom + (drows ? (size_t)(na + nb) * (ncols + 1) : 0)) + hfk_run_stage_bytes();
 }
 // Dynamic LDS of a cooperative J/K workgroup (its static tables come on top).
 inline size_t hfk_coop_jk_lds_bytes(int la, int lb, int ncols)
@@ -143,6 +173,13 @@ struct ShellDev {
 void hfk_launch_pair_setup(const ShellDev& sh, int npair, const int* pair_A, const int* pair_B,
                            const int* pair_pp0, const int* pair_q0, double* pp, double* qcomp,
                            double* pair_qhi, hipStream_t s);
+// Position-major copy of a ket class's primitive-pair records: for position
+// j (pair klist[j], pos_npp[j] primitive pairs, segment length pos_len[j]),
+// field f < PP_STRIDE of primitive ik goes to out[pos_soa[j] + j + (ik * nf +
+// f) * pos_len[j]], its ncomp screen values to the fields after.
+void hfk_launch_ket_soa(const double* pp, const double* qcomp, const int* pair_pp0, const int* klist, int nket,
+                        const long long* pos_soa, const int* pos_len, const int* pos_npp, int nf, int ncomp,
+                        double* out, hipStream_t s);
 void hfk_launch_boys_tables(const double* glx, const double* glw, double* ref64, float* ref32,
                             double* acc64, float* acc32, hipStream_t s);
 void hfk_launch_fill(double* p, double v, long long n, hipStream_t s);

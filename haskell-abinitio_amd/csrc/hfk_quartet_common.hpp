@@ -31,6 +31,44 @@ __device__ __forceinline__ void decode_quartet(const QuartetArgs& a, long long q
     }
 }
 
+// Ket primitive-pair records as the generated ERI code reads them:
+// prim(ik) is primitive pair ik of the ket, at(pk, f) its field f and
+// q(pk, z) its screen value for Cartesian component z.
+//  KetAoS: the pair's own records (PP_STRIDE doubles each) and qcomp[].
+//  KetSoA: the position-major copy of the ket class (KetRun): field f of
+//          primitive ik at base[(ik * NF + f) * len], the screen values
+//          after the PP_STRIDE record fields, so that lanes holding
+//          consecutive positions of one segment read consecutive words.
+struct KetAoS {
+    const double* p;
+    const double* qcomp;
+    __device__ __forceinline__ const double* prim(int ik) const { return p + (size_t)ik * PP_STRIDE; }
+    __device__ __forceinline__ double at(const double* pk, int f) const { return pk[f]; }
+    __device__ __forceinline__ double q(const double* pk, int z) const { return qcomp[(long long)pk[PP_QOFF] + z]; }
+};
+template <int NF>
+struct KetSoA {
+    const double* p;
+    int len;
+    __device__ __forceinline__ const double* prim(int ik) const { return p + (size_t)ik * NF * len; }
+    __device__ __forceinline__ double at(const double* pk, int f) const { return pk[(size_t)f * len]; }
+    __device__ __forceinline__ double q(const double* pk, int z) const { return pk[(size_t)(PP_STRIDE + z) * len]; }
+};
+
+// Pair-level selection of the per-lane kernels from the ket position's
+// copies of pair_qhi and pair_sch (the same tests as quartet_selected).
+__device__ __forceinline__ bool quartet_selected_pos(const QuartetArgs& a, double bqhi, double bsch, int pos)
+{
+    if (a.qtau > 0.0 && bqhi * a.kpos_q[pos] < a.qtau) return false;
+    if (a.sch_cut > 0.0 || a.prec_sel != 0) {
+        const double bound = bsch * a.kpos_q[a.nket + pos];
+        if (bound < a.sch_cut) return false;
+        if (a.prec_sel == 1 && !(bound >= a.fp32_thr)) return false;
+        if (a.prec_sel == 2 && !(bound < a.fp32_thr)) return false;
+    }
+    return true;
+}
+
 // Kets of a task: its runs [x, y) of positions in the ket class list.
 __device__ __forceinline__ int task_ket_count(const QuartetArgs& a, const int4& task)
 {

@@ -327,6 +327,36 @@ unsigned grid_for(long long n, int block)
 
 }  // namespace
 
+// Position-major ket records (hfk_launch_ket_soa): one thread per position.
+__global__ void k_ket_soa(const double* __restrict__ pp, const double* __restrict__ qcomp,
+                          const int* __restrict__ pair_pp0, const int* __restrict__ klist, int nket,
+                          const long long* __restrict__ pos_soa, const int* __restrict__ pos_len,
+                          const int* __restrict__ pos_npp, int nf, int ncomp, double* __restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nket) return;
+    const int npp = pos_npp[j];
+    const long long len = pos_len[j];
+    double* o = out + pos_soa[j] + j;
+    const double* rec = pp + (size_t)pair_pp0[klist[j]] * PP_STRIDE;
+    for (int ik = 0; ik < npp; ++ik) {
+        const double* r = rec + (size_t)ik * PP_STRIDE;
+        double* w = o + (long long)ik * nf * len;
+        for (int f = 0; f < PP_STRIDE; ++f) w[f * len] = r[f];
+        const double* q = qcomp + (long long)r[PP_QOFF];
+        for (int z = 0; z < ncomp; ++z) w[(PP_STRIDE + z) * len] = q[z];
+    }
+}
+
+void hfk_launch_ket_soa(const double* pp, const double* qcomp, const int* pair_pp0, const int* klist, int nket,
+                        const long long* pos_soa, const int* pos_len, const int* pos_npp, int nf, int ncomp,
+                        double* out, hipStream_t s)
+{
+    if (nket <= 0) return;
+    hipLaunchKernelGGL(k_ket_soa, dim3((nket + 255) / 256), dim3(256), 0, s, pp, qcomp, pair_pp0, klist, nket, pos_soa,
+                       pos_len, pos_npp, nf, ncomp, out);
+}
+
 void hfk_launch_pair_setup(const ShellDev& sh, int npair, const int* pair_A, const int* pair_B,
                            const int* pair_pp0, const int* pair_q0, double* pp, double* qcomp,
                            double* pair_qhi, hipStream_t s)
