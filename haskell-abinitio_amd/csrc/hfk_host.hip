@@ -922,6 +922,29 @@ int hfk_ctx_plan(hfk_ctx* c)
         }
         HFK_TRY(entry_upload(c, e, &e.d_kcsr_off, koff));
         HFK_TRY(entry_upload(c, e, &e.d_kcsr, kcsr));
+        // K reduction jobs: HFK_KSEG consecutive entries of one shell
+        // component row (first pass), and each row's jobs (second pass)
+        {
+            std::vector<int4> kjob, krow;
+            for (int S = 0; S < nshell; ++S) {
+                const int q0 = koff[S], q1 = koff[S + 1];
+                if (q1 <= q0) continue;
+                for (int x = 0; x < HFK_MAXCOMP; ++x) {
+                    if (c->shells[S].bf[x] < 0) continue;
+                    const int j0 = (int)kjob.size();
+                    for (int q = q0; q < q1; q += HFK_KSEG)
+                        kjob.push_back(make_int4(S * HFK_MAXCOMP + x, q, std::min(q1, q + HFK_KSEG), 0));
+                    krow.push_back(make_int4(S * HFK_MAXCOMP + x, j0, (int)kjob.size() - j0, 0));
+                }
+            }
+            e.nkjob = (int)kjob.size();
+            e.nkrow = (int)krow.size();
+            if (!kjob.empty()) {
+                HFK_TRY(entry_upload(c, e, &e.d_kjob, kjob));
+                HFK_TRY(entry_upload(c, e, &e.d_krow, krow));
+                HFK_TRY(entry_alloc(c, e, &e.d_kseg, kjob.size() * (size_t)e.ncols));
+            }
+        }
         HFK_TRY(entry_upload(c, e, &e.d_bcsr_off, boff));
         HFK_TRY(entry_upload(c, e, &e.d_jab_slot, jab_slot));
         // J'_cd layout; the first reduction pass works on chunks of
@@ -978,7 +1001,7 @@ int hfk_ctx_plan(hfk_ctx* c)
         HFK_TRY(entry_alloc(c, e, &e.d_jab, std::max<size_t>(1, (size_t)e.ntask * nbra_comp)));
         HFK_TRY(entry_alloc(c, e, &e.d_jcd, std::max<size_t>(1, (size_t)njcd * nket_comp)));
         e.jl.jcd = e.d_jcd;
-        e.slot_bytes = sizeof(double) * ((size_t)e.ntask * (nrow * e.ncols + nbra_comp) +
+        e.slot_bytes = sizeof(double) * ((size_t)e.ntask * (nrow * e.ncols + nbra_comp) + (size_t)e.nkjob * e.ncols +
                                          ((size_t)njcd + (size_t)e.jl.nchunk * kl.size()) * nket_comp);
         c->slot_bytes += e.slot_bytes;
     }
@@ -998,6 +1021,11 @@ int hfk_ctx_plan(hfk_ctx* c)
             P.kslot = e.d_kslot;
             P.csr_off = e.d_kcsr_off;
             P.csr = e.d_kcsr;
+            P.kjob = e.d_kjob;
+            P.nkjob = e.nkjob;
+            P.kseg = e.d_kseg;
+            P.krow = e.d_krow;
+            P.nkrow = e.nkrow;
             P.collist = e.d_collist;
             P.ncols = e.ncols;
             P.nrow = ncart(e.la) + ncart(e.lb);

@@ -68,6 +68,10 @@ struct PlanEntry {
     JKPart jkp;                 // its partial J', K' (reduced on its stream after the launch)
     int* d_kcsr_off;            // [nshell+1]
     int2* d_kcsr;               // {task, row offset}
+    int4* d_kjob;               // K reduction jobs (JKPart.kjob)
+    int4* d_krow;               // K reduction rows (JKPart.krow)
+    double* d_kseg;             // [nkjob][ncols] first-pass sums
+    int nkjob, nkrow;
     int* d_bcsr_off;            // [nbra+1]: J'_ab slots of each bra position
     int* d_jab_slot;            // [ntask]
     size_t slot_bytes;          // device memory of the slots

@@ -7,6 +7,7 @@
 
 #define HFK_MAXCOMP 6        // Cartesian components of a d shell
 #define HFK_LMAX 2           // highest shell angular momentum with generated kernels
+#define HFK_KSEG 32          // K-slot rows summed per first-pass job of the K reduction
 
 // Ket run of a task: positions [x, y) of the ket class list, walked in
 // order; jcd is the J'_cd slot entry of position x (position p writes entry
@@ -225,6 +226,14 @@ struct JKPart {
     const int2* csr;            // {task, row offset (0: shell A of the bra, NA: shell B)}
     const int* collist;         // [ncols] basis function of each compact column
     int ncols, nrow, nshell;
+    // K rows in two passes: job = HFK_KSEG consecutive csr entries of one
+    // row (shell S, component x), summed in entry order into kseg; then each
+    // row sums its jobs in job order.
+    const int4* kjob;           // [nkjob] {S * HFK_MAXCOMP + x, first entry, end entry, -}
+    int nkjob;
+    double* kseg;               // [nkjob][ncols]
+    const int4* krow;           // [nkrow] {S * HFK_MAXCOMP + x, first job, job count, -}
+    int nkrow;
     const double* jab;          // [slot][NBRA] (bra side)
     const int* bcsr_off;        // [nbra + 1]: slots of each bra position
     const double* jpart;        // [nchunk][nket][NKET] ket-side chunk sums

@@ -434,16 +434,48 @@ __global__ void __launch_bounds__(256) k_reduce_jcd_part(JcdLayout L)
             s_base[b] = L.jbase[qi];
         }
         __syncthreads();
-        for (int b = 0; b < cnt; ++b) {
-            const int end = s_end[b];
-            if (end <= t0) {   // uniform: no later bra covers the tile
-                done = true;
-                break;
+        if (np == 1) {
+            // one piece per quartet: the loads of 8 bras are issued before
+            // their adds (in bra order; a bra that does not cover the
+            // position adds nothing)
+            constexpr int U = 8;
+            for (int b0 = 0; b0 < cnt; b0 += U) {
+                if (s_end[b0] <= t0) {   // uniform: no later bra covers the tile
+                    done = true;
+                    break;
+                }
+                double v[U];
+                bool in[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int b = b0 + u;
+                    in[u] = b < cnt && valid && pos < s_end[b] && (!L.same || pos <= base + b);
+                    v[u] = in[u] ? L.jcd[(s_base[b] + off) * nkc + k] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (in[u]) sum += v[u];
             }
-            if (valid && pos < end && (!L.same || pos <= base + b)) {
-                const long long len = end - first;
-                const long long e0 = s_base[b] + off;
-                for (int p = 0; p < np; ++p) sum += L.jcd[(e0 + p * len) * nkc + k];
+        } else {
+            for (int b = 0; b < cnt; ++b) {
+                const int end = s_end[b];
+                if (end <= t0) {   // uniform: no later bra covers the tile
+                    done = true;
+                    break;
+                }
+                if (valid && pos < end && (!L.same || pos <= base + b)) {
+                    const long long len = end - first;
+                    const long long e0 = s_base[b] + off;
+                    constexpr int U = 8;
+                    for (int p0 = 0; p0 < np; p0 += U) {
+                        double v[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) v[u] = p0 + u < np ? L.jcd[(e0 + (p0 + u) * len) * nkc + k] : 0.0;
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            if (p0 + u < np) sum += v[u];
+                    }
+                }
             }
         }
     }
@@ -458,93 +490,93 @@ void hfk_launch_reduce_jcd(const JcdLayout& L, hipStream_t s)
         hipLaunchKernelGGL(k_reduce_jcd_part, dim3((unsigned)L.ntile, (unsigned)L.nchunk), dim3(256), 0, s, L);
 }
 
-// The launch's partial J', K' in one kernel.
-// Blocks [0, nshell * HFK_MAXCOMP): K' partial, one workgroup per shell
-// component row.  Its entries (task, row offset) are staged in LDS 256 at a
-// time; the threads form G = 256 / (64 ceil(ncols / 64)) groups (G = 4 for
-// narrow launches), group g summing the entries g, g + G, ... in task order,
-// a thread owning compact columns in registers; the groups' rows are added
-// in group order.
+// The launch's partial J', K'.
+// k_jk_part, blocks [0, ceil(nkjob / G)): first pass of the K rows.  A job
+// is HFK_KSEG consecutive entries (task, row offset) of one shell component
+// row; its slot rows are summed in entry order.  The threads form G = 256 /
+// (64 ceil(ncols / 64)) groups, one job per group, a thread owning compact
+// columns in registers; the loads of 8 entries are issued before their adds
+// (in entry order), so a lane keeps 8 reads in flight.
 // Blocks above: J' partial, thread = (position in the bra class list, bra
 // component) (bra side, plus the ket side of the same pair when the classes
 // are one), then for two classes (position in the ket class list, ket
 // component); the ket side sums the J'_cd chunk sums in chunk order.
-__device__ __forceinline__ void k_part_rows(const JKPart& P, int n, const int* sh_bf, int blk)
+// k_krow_final: second pass, thread = (row, compact column): the row's job
+// sums in job order, stored to the row of the partial K'.
+__device__ __forceinline__ void k_part_kjobs(const JKPart& P, int blk)
 {
     constexpr int NCOL = 16;   // columns per thread: ncols <= 4096 (the planner checks)
-    __shared__ double red[3][256];
-    __shared__ int2 s_csr[256];
-    const int S = blk / HFK_MAXCOMP, x = blk - S * HFK_MAXCOMP;
-    const int r = sh_bf[S * HFK_MAXCOMP + x];
-    if (r < 0) return;
+    constexpr int U = 8;       // entries in flight per lane
     const int tid = threadIdx.x;
     const int width = 64 * ((P.ncols + 63) / 64);
     const int G = width >= 256 ? 1 : 256 / width;
     const int gw = G == 1 ? 256 : width;
     const int g = tid / gw, lane = tid - g * gw;
+    const int job = blk * G + g;
+    if (g >= G || job >= P.nkjob) return;
+    const int4 jb = P.kjob[job];
+    const int x = jb.x % HFK_MAXCOMP;
+    const int ncols = P.ncols, nc = (ncols - lane + gw - 1) / gw;   // columns of this lane
     double acc[NCOL];
 #pragma unroll
     for (int c = 0; c < NCOL; ++c) acc[c] = 0.0;
-    const int q0 = P.csr_off[S], q1 = P.csr_off[S + 1];
-    for (int base = q0; base < q1; base += 256) {
-        const int cnt = min(256, q1 - base);
-        __syncthreads();
-        if (tid < cnt) s_csr[tid] = P.csr[base + tid];
-        __syncthreads();
-        if (g < G) {
-#pragma unroll 4
-            for (int q = g; q < cnt; q += G) {
-                const int2 en = s_csr[q];
-                const double* src = P.kslot + ((size_t)en.x * P.nrow + en.y + x) * P.ncols;
+    for (int q0 = jb.y; q0 < jb.z; q0 += U) {
+        const double* src[U];
 #pragma unroll
-                for (int c = 0; c < NCOL; ++c) {
-                    const int j = lane + gw * c;
-                    if (j < P.ncols) acc[c] += src[j];
-                }
-            }
+        for (int u = 0; u < U; ++u) {
+            const int q = min(q0 + u, jb.z - 1);   // past the end: a valid entry, not added
+            const int2 en = P.csr[q];
+            src[u] = P.kslot + ((size_t)en.x * P.nrow + en.y + x) * ncols + lane;
         }
-    }
-    double* row = P.Kpart + (size_t)r * (n + 1);
-    if (G == 1) {
+        const int nu = min(U, jb.z - q0);
 #pragma unroll
         for (int c = 0; c < NCOL; ++c) {
-            const int j = lane + gw * c;
-            if (j < P.ncols) row[P.collist[j]] = acc[c];
+            if (c >= nc) break;
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = src[u][gw * c];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (u < nu) acc[c] += v[u];
         }
-        return;
     }
-    // G > 1: ncols <= 128, one column per thread (c = 0)
-    __syncthreads();
-    if (g > 0 && g < G) red[g - 1][lane] = acc[0];
-    __syncthreads();
-    if (g == 0 && lane < P.ncols) {
-        double v = acc[0];
-        for (int h = 1; h < G; ++h) v += red[h - 1][lane];
-        row[P.collist[lane]] = v;
-    }
+    double* out = P.kseg + (size_t)job * ncols + lane;
+#pragma unroll
+    for (int c = 0; c < NCOL; ++c)
+        if (c < nc) out[gw * c] = acc[c];
 }
 
 __device__ __forceinline__ double k_part_ket_sum(const JKPart& P, int pos, int c)
 {
+    constexpr int U = 8;
     const long long per = (long long)P.nket * P.nc * P.nd;
-    const long long t = (long long)pos * P.nc * P.nd + c;
+    const double* src = P.jpart + (long long)pos * P.nc * P.nd + c;
     double v = 0.0;
-    for (int q = 0; q < P.nchunk; ++q) v += P.jpart[(long long)q * per + t];
+    for (int q0 = 0; q0 < P.nchunk; q0 += U) {
+        double t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) t[u] = q0 + u < P.nchunk ? src[(q0 + u) * per] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (q0 + u < P.nchunk) v += t[u];
+    }
     return v;
 }
 
 __global__ void __launch_bounds__(256) k_jk_part(JKPart P, int n, const int* sh_bf, const int* sh_bfp,
                                                  const int* pair_A, const int* pair_B)
 {
-    const int nkrow = P.nshell * HFK_MAXCOMP;
-    if ((int)blockIdx.x < nkrow) {
-        k_part_rows(P, n, sh_bf, blockIdx.x);
+    const int width = 64 * ((P.ncols + 63) / 64);
+    const int G = width >= 256 ? 1 : 256 / width;
+    const int nkblk = (P.nkjob + G - 1) / G;
+    if ((int)blockIdx.x < nkblk) {
+        k_part_kjobs(P, blockIdx.x);
         return;
     }
     const int nbc = P.na * P.nb, nkc = P.nc * P.nd;
     const long long tb = (long long)P.nbra * nbc;
     const long long total = tb + (P.same ? 0 : (long long)P.nket * nkc);
-    const long long t = (long long)(blockIdx.x - nkrow) * blockDim.x + threadIdx.x;
+    const long long t = (long long)(blockIdx.x - nkblk) * blockDim.x + threadIdx.x;
     if (t >= total) return;
     int p, x, y;
     double v = 0.0;
@@ -567,13 +599,40 @@ __global__ void __launch_bounds__(256) k_jk_part(JKPart P, int n, const int* sh_
     if (ia < n && ib < n) P.Jpart[(size_t)ia * (n + 1) + ib] = v;
 }
 
+__global__ void __launch_bounds__(256) k_krow_final(JKPart P, int n, const int* sh_bf)
+{
+    constexpr int U = 8;
+    const long long nt = (long long)P.nkrow * P.ncols;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nt;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int row = (int)(t / P.ncols), col = (int)(t - (long long)row * P.ncols);
+        const int4 kr = P.krow[row];
+        const double* src = P.kseg + (size_t)kr.y * P.ncols + col;
+        double v = 0.0;
+        for (int j0 = 0; j0 < kr.z; j0 += U) {
+            double s[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) s[u] = j0 + u < kr.z ? src[(size_t)(j0 + u) * P.ncols] : 0.0;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (j0 + u < kr.z) v += s[u];
+        }
+        const int r = sh_bf[kr.x];
+        P.Kpart[(size_t)r * (n + 1) + P.collist[col]] = v;
+    }
+}
+
 void hfk_launch_jk_part(const JKPart& P, int n, const int* sh_bf, const int* sh_bfp, const int* pair_A,
                         const int* pair_B, hipStream_t s)
 {
     const long long total = (long long)P.nbra * P.na * P.nb + (P.same ? 0 : (long long)P.nket * P.nc * P.nd);
-    const long long blocks = (long long)P.nshell * HFK_MAXCOMP + (total + 255) / 256;
+    const int width = 64 * ((P.ncols + 63) / 64);
+    const int G = width >= 256 ? 1 : 256 / width;
+    const long long blocks = (P.nkjob + G - 1) / G + (total + 255) / 256;
     if (blocks > 0)
         hipLaunchKernelGGL(k_jk_part, dim3((unsigned)blocks), dim3(256), 0, s, P, n, sh_bf, sh_bfp, pair_A, pair_B);
+    const long long nt = (long long)P.nkrow * P.ncols;
+    if (nt > 0) hipLaunchKernelGGL(k_krow_final, dim3(grid_for(nt, 256)), dim3(256), 0, s, P, n, sh_bf);
 }
 
 // G_ij = (J'_ij + J'_ji) - 1/2 (K'_ij + K'_ji), J' and K' the sums over the
