@@ -547,8 +547,9 @@ static void free_plan(hfk_ctx* c)
     c->slot_bytes = 0;
 }
 
-// The launch plan: every class pair (bra class >= ket class), split into an
-// FP64 and an FP32 launch when the mixed-precision threshold is set.
+// The launch plan: every class pair (bra class >= ket class); with the
+// mixed-precision threshold set, its tasks form an FP64 and an FP32 part
+// (two launches over one set of slots).
 int hfk_ctx_plan(hfk_ctx* c)
 {
     if (const char* v = getenv("HFK_KETS_PER_LANE")) c->kets_per_lane = std::max(1, atoi(v));
@@ -562,23 +563,18 @@ int hfk_ctx_plan(hfk_ctx* c)
             hfk_pair_class_l(kb, &la, &lb);
             hfk_pair_class_l(kk, &lc, &ld);
             const bool coop = hfk_class_coop(la, lb, lc, ld);   // FP64 only
-            const bool split = c->opt.fp32_threshold > 0.0 && !coop;
-            for (int prec = 0; prec < 2; ++prec) {
-                if (prec == 1 && !split) continue;
-                PlanEntry e = {};
-                e.kb = kb;
-                e.kk = kk;
-                e.la = la;
-                e.lb = lb;
-                e.lc = lc;
-                e.ld = ld;
-                e.prec = prec;
-                e.prec_sel = split ? prec + 1 : 0;
-                const long long nb = c->class_count[kb], nk = c->class_count[kk];
-                e.nquart = (kb == kk) ? nb * (nb + 1) / 2 : nb * nk;
-                e.mode = ref ? 1 : 0;   // 1: kernels apply the reference screen
-                c->plan.push_back(e);
-            }
+            PlanEntry e = {};
+            e.kb = kb;
+            e.kk = kk;
+            e.la = la;
+            e.lb = lb;
+            e.lc = lc;
+            e.ld = ld;
+            e.split = c->opt.fp32_threshold > 0.0 && !coop;   // FP64 and FP32 task parts
+            const long long nb = c->class_count[kb], nk = c->class_count[kk];
+            e.nquart = (kb == kk) ? nb * (nb + 1) / 2 : nb * nk;
+            e.mode = ref ? 1 : 0;   // 1: kernels apply the reference screen
+            c->plan.push_back(e);
         }
     }
     const int nshell = (int)c->shells.size();
@@ -713,6 +709,32 @@ int hfk_ctx_plan(hfk_ctx* c)
         e.ncols = (int)collist.size();
         for (int f = 0; f <= c->n; ++f)
             if (colmap[f] < 0) colmap[f] = e.ncols;   // never a ket column (or the padding): discarded
+        // Precision parts.  With the mixed-precision split, the kets of bra i
+        // in segment s, positions [first, jend), divide at jmid: from jmid on
+        // every ket's Schwarz bound times the bra's is below the threshold
+        // (a suffix maximum within the segment, found by binary search), so
+        // [jmid, jend) runs in FP32 and [first, jmid) in FP64.  Each task
+        // holds kets of one part; the FP64 tasks come first in the task list,
+        // and both launches of the class write the same slots.
+        std::vector<int> jmid(jend);
+        if (e.split) {
+            std::vector<double> smax(kl.size());
+            for (int s2 = 0; s2 < nkseg; ++s2)
+                for (long long j = segs[s2].second - 1; j >= segs[s2].first; --j)
+                    smax[j] = std::max(c->h_pair_sch[kl[j]], j + 1 < segs[s2].second ? smax[j + 1] : 0.0);
+            const double thr = c->opt.fp32_threshold;
+            for (size_t i = 0; i < bl.size(); ++i) {
+                const double bs = c->h_pair_sch[bl[i]];
+                for (int s2 = 0; s2 < nkseg; ++s2) {
+                    long long lo = segs[s2].first, hi = jend[i * nkseg + s2];
+                    while (lo < hi) {   // first t with bs * smax[t] < thr (non-increasing)
+                        const long long mid = (lo + hi) / 2;
+                        if (bs * smax[mid] < thr) hi = mid; else lo = mid + 1;
+                    }
+                    jmid[i * nkseg + s2] = (int)lo;
+                }
+            }
+        }
         long long merged_kets = 0;
         for (size_t i = 0; i < bl.size(); ++i)
             for (int s = 0; s < nkseg; ++s)
@@ -746,66 +768,12 @@ int hfk_ctx_plan(hfk_ctx* c)
             r.npp = c->h_pair_npp[kl[segs[sg].first]];
             return r;
         };
-        for (size_t i = 0; i < bl.size(); ++i) {
-            const int nbp = c->h_pair_npp[bl[i]];
-            // kets of the unsplit segments, as one list of runs cut every cmerge kets
-            long long r0 = (long long)runs.size();
-            int len = 0;
-            for (int s = 0; s < nkseg; ++s) {
-                const auto& sg = segs[s];
-                const int nkp = c->h_pair_npp[kl[sg.first]];
-                if (!whole(nbp, nkp)) continue;
-                long long x = sg.first;
-                const long long x1 = jend[i * nkseg + s];
-                while (x < x1) {
-                    const long long take = std::min<long long>(x1 - x, cmerge - len);
-                    runs.push_back(run_of(x, x + take, jbase[i * nkseg + s] + (x - sg.first), s));
-                    cost_acc += take * nbp * nkp;
-                    x += take;
-                    len += (int)take;
-                    if (len == cmerge || (long long)runs.size() - r0 == HFK_TASK_RUNS) {
-                        if (!push(bl[i], r0, (nbp << 16) | 0xff, len)) return hfk_set_error(HFK_EUNSUP, "task list too long");
-                        r0 = (long long)runs.size();
-                        len = 0;
-                    }
-                }
-            }
-            if (len > 0 && !push(bl[i], r0, (nbp << 16) | 0xff, len)) return hfk_set_error(HFK_EUNSUP, "task list too long");
-            if (nbp > 0) e.bra_stage = std::max(e.bra_stage, coop ? nbp : std::min(nbp, HFK_BRA_STAGE));
-            // segments split into primitive ranges: one run per task
-            for (int s = 0; s < nkseg; ++s) {
-                const auto& sg = segs[s];
-                const int nkp = c->h_pair_npp[kl[sg.first]];
-                if (whole(nbp, nkp)) continue;
-                const long long k0 = sg.first, k1 = jend[i * nkseg + s];
-                if (k1 <= k0) continue;
-                int bstep, kstep;
-                steps(nbp, nkp, bstep, kstep);
-                int piece = 0;   // pieces in (bra range, ket range) order, as npiece counts them
-                for (int b0 = 0; b0 < nbp; b0 += bstep) {
-                    const int b1 = std::min(nbp, b0 + bstep);
-                    for (int q0 = 0; q0 < nkp; q0 += kstep, ++piece) {
-                        const int q1 = std::min(nkp, q0 + kstep);
-                        const int w = (b0 << 24) | (b1 << 16) | (q0 << 8) | q1;
-                        for (long long j0 = k0; j0 < k1; j0 += chunk) {
-                            const long long j1 = std::min<long long>(k1, j0 + chunk);
-                            const long long rr = (long long)runs.size();
-                            runs.push_back(run_of(j0, j1, jbase[i * nkseg + s] + (long long)piece * (k1 - k0) + (j0 - k0), s));
-                            cost_acc = (j1 - j0) * (long long)(b1 - b0) * (q1 - q0);
-                            if (!push(bl[i], rr, w, (int)(j1 - j0))) return hfk_set_error(HFK_EUNSUP, "task list too long");
-                        }
-                        e.bra_stage = std::max(e.bra_stage, b1 - b0);
-                    }
-                }
-            }
-        }
-        if ((long long)kl.size() > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "pair class too long");
-        // costliest tasks first: workgroups start roughly in index order, so
-        // the launch ends on short tasks and its tail is short.  A stable
-        // bucket sort on the cost's quarter-octave (4 log2 cost, from its two
-        // bits below the leading one), linear in the number of tasks
+        // costliest tasks first within each part: workgroups start roughly in
+        // index order, so a launch ends on short tasks and its tail is short.
+        // A stable bucket sort on the cost's quarter-octave (4 log2 cost, from
+        // its two bits below the leading one), linear in the number of tasks
         // (millions for the large clusters).
-        {
+        auto sort_costliest_first = [&](size_t t0) {
             auto bucket = [](long long v) {
                 if (v <= 0) return 0;
                 const int b = 63 - __builtin_clzll((unsigned long long)v);
@@ -813,11 +781,77 @@ int hfk_ctx_plan(hfk_ctx* c)
                 return 4 * b + sub;   // 0 .. 255
             };
             std::vector<size_t> start(257, 0);
-            for (long long v : cost) ++start[255 - bucket(v) + 1];
+            for (size_t t = t0; t < tasks.size(); ++t) ++start[255 - bucket(cost[t]) + 1];
             for (int b = 1; b <= 256; ++b) start[b] += start[b - 1];
-            std::vector<int4> sorted(tasks.size());
-            for (size_t t = 0; t < tasks.size(); ++t) sorted[start[255 - bucket(cost[t])]++] = tasks[t];
-            tasks.swap(sorted);
+            std::vector<int4> sorted(tasks.size() - t0);
+            for (size_t t = t0; t < tasks.size(); ++t) sorted[start[255 - bucket(cost[t])]++] = tasks[t];
+            std::copy(sorted.begin(), sorted.end(), tasks.begin() + t0);
+        };
+        const int nparts = e.split ? 2 : 1;
+        for (int part = 0; part < nparts; ++part) {
+            const size_t part0 = tasks.size();
+            // the part's kets of bra i in segment s: [lo, hi) within [first, jend)
+            auto part_lo = [&](size_t i, int s) -> long long { return part == 0 ? segs[s].first : jmid[i * nkseg + s]; };
+            auto part_hi = [&](size_t i, int s) -> long long { return part == 0 ? jmid[i * nkseg + s] : jend[i * nkseg + s]; };
+            for (size_t i = 0; i < bl.size(); ++i) {
+                const int nbp = c->h_pair_npp[bl[i]];
+                // kets of the unsplit segments, as one list of runs cut every cmerge kets
+                long long r0 = (long long)runs.size();
+                int len = 0;
+                for (int s = 0; s < nkseg; ++s) {
+                    const auto& sg = segs[s];
+                    const int nkp = c->h_pair_npp[kl[sg.first]];
+                    if (!whole(nbp, nkp)) continue;
+                    long long x = part_lo(i, s);
+                    const long long x1 = part_hi(i, s);
+                    while (x < x1) {
+                        const long long take = std::min<long long>(x1 - x, cmerge - len);
+                        runs.push_back(run_of(x, x + take, jbase[i * nkseg + s] + (x - sg.first), s));
+                        cost_acc += take * nbp * nkp;
+                        x += take;
+                        len += (int)take;
+                        if (len == cmerge || (long long)runs.size() - r0 == HFK_TASK_RUNS) {
+                            if (!push(bl[i], r0, (nbp << 16) | 0xff, len)) return hfk_set_error(HFK_EUNSUP, "task list too long");
+                            r0 = (long long)runs.size();
+                            len = 0;
+                        }
+                    }
+                }
+                if (len > 0 && !push(bl[i], r0, (nbp << 16) | 0xff, len)) return hfk_set_error(HFK_EUNSUP, "task list too long");
+                if (nbp > 0) e.bra_stage = std::max(e.bra_stage, coop ? nbp : std::min(nbp, HFK_BRA_STAGE));
+                // segments split into primitive ranges: one run per task
+                for (int s = 0; s < nkseg; ++s) {
+                    const auto& sg = segs[s];
+                    const int nkp = c->h_pair_npp[kl[sg.first]];
+                    if (whole(nbp, nkp)) continue;
+                    // J'_cd entries cover the whole survivor range [k0, k1) of the
+                    // quartet pieces; the part's kets are [p0, p1)
+                    const long long k0 = sg.first, k1 = jend[i * nkseg + s];
+                    const long long p0 = part_lo(i, s), p1 = part_hi(i, s);
+                    if (p1 <= p0) continue;
+                    int bstep, kstep;
+                    steps(nbp, nkp, bstep, kstep);
+                    int piece = 0;   // pieces in (bra range, ket range) order, as npiece counts them
+                    for (int b0 = 0; b0 < nbp; b0 += bstep) {
+                        const int b1 = std::min(nbp, b0 + bstep);
+                        for (int q0 = 0; q0 < nkp; q0 += kstep, ++piece) {
+                            const int q1 = std::min(nkp, q0 + kstep);
+                            const int w = (b0 << 24) | (b1 << 16) | (q0 << 8) | q1;
+                            for (long long j0 = p0; j0 < p1; j0 += chunk) {
+                                const long long j1 = std::min<long long>(p1, j0 + chunk);
+                                const long long rr = (long long)runs.size();
+                                runs.push_back(run_of(j0, j1, jbase[i * nkseg + s] + (long long)piece * (k1 - k0) + (j0 - k0), s));
+                                cost_acc = (j1 - j0) * (long long)(b1 - b0) * (q1 - q0);
+                                if (!push(bl[i], rr, w, (int)(j1 - j0))) return hfk_set_error(HFK_EUNSUP, "task list too long");
+                            }
+                            e.bra_stage = std::max(e.bra_stage, b1 - b0);
+                        }
+                    }
+                }
+            }
+            if ((long long)kl.size() > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "pair class too long");
+            sort_costliest_first(part0);
+            if (part == 0) e.ntask64 = (long long)tasks.size();
         }
         e.ntask = (long long)tasks.size();
         // LDS of one workgroup: the K' rows must fit; the bra shells' density
@@ -1052,7 +1086,7 @@ int hfk_ctx_plan(hfk_ctx* c)
     c->serial_runs = 0;
     const size_t np = c->plan.size();
     // event pool for per-launch timing
-    while (c->events.size() < 2 * np + 2) {
+    while (c->events.size() < 4 * np + 2) {
         hipEvent_t ev;
         HFK_HIP(hipEventCreate(&ev));
         c->events.push_back(ev);
@@ -1088,8 +1122,15 @@ void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a)
     a.nket = c->class_count[e.kk];
     a.same = (e.kb == e.kk);
     a.nquart = e.nquart;
-    a.prec_sel = e.prec_sel;
+    a.prec_sel = 0;   // the precision is the task part's (planned), not chosen per quartet
     a.fp32_thr = c->opt.fp32_threshold;
+}
+
+// Tasks of one precision part of a plan entry: [first, first + count).
+static void part_range(const PlanEntry& e, int part, long long& first, long long& count)
+{
+    first = part == 0 ? 0 : e.ntask64;
+    count = part == 0 ? e.ntask64 : e.ntask - e.ntask64;
 }
 
 // Run the plan for one operation (J/K or flat export); per-launch HIP events.
@@ -1099,36 +1140,47 @@ int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
     long long total = 0;
     const size_t np = c->plan.size();
     // J/K launches write only their own slots, so they may run concurrently.
-    // The longest launch (as measured in the context's second J/K run; the
+    // The longest class (as measured in the context's second J/K run; the
     // first two are serial and the first pays one-time costs) runs first,
     // alone on the context stream: it fills the GPU by itself and keeps a
     // clean time.  The others then run round-robin on the auxiliary streams,
     // forked from and joined back into the context stream, and fill each
-    // other's tails.  Each J/K launch is followed on its stream by the
-    // fixed-order reduction of its J'_cd entries.
+    // other's tails.  A class's FP64 and FP32 parts run one after the other
+    // on its stream, followed by the fixed-order reductions of its slots.
     const bool conc = op == HFK_OP_JK && c->concurrent && np > 2 && c->first_launch >= 0;
     const size_t first = conc ? (size_t)c->first_launch : 0;
     int slot = 0, missing = -1, lds_fail = -1;
     for (size_t oi = 0; oi < np && missing < 0 && lds_fail < 0; ++oi) {
-        // launch order: the first launch, then the others in plan order
+        // launch order: the first class, then the others in plan order
         const size_t i = !conc ? oi : oi == 0 ? first : oi <= first ? oi - 1 : oi;
         const PlanEntry& e = c->plan[i];
         if (conc && oi == 1) {
             HFK_HIP(hipEventRecord(c->ev_fork, c->stream));
             for (int k = 0; k < hfk_ctx::NAUX; ++k) HFK_HIP(hipStreamWaitEvent(c->aux[k], c->ev_fork, 0));
         }
-        if (op != HFK_OP_JK && e.prec == 1) continue;   // exports are FP64
         hfk_plan_args(c, e, a);
-        if (op != HFK_OP_JK) a.prec_sel = 0;
         total += e.nquart;
         hipStream_t s = (conc && oi > 0) ? c->aux[slot++ % hfk_ctx::NAUX] : c->stream;
-        HFK_HIP(hipEventRecord(c->events[2 * i], s));
-        const int rc = hfk_launch_quartets(e.la, e.lb, e.lc, e.ld, op, e.prec, c->opt.boys_mode,
-                                           op == HFK_OP_JK ? e.mode : 0, a, s);
+        int rc = 0;
+        // exports (flat array, Schwarz diagonal) walk the pair lists once, in FP64
+        const int nparts = (op == HFK_OP_JK && e.split) ? 2 : 1;
+        for (int part = 0; part < nparts && rc == 0; ++part) {
+            QuartetArgs b = a;
+            if (op == HFK_OP_JK) {
+                long long t0, cnt;
+                part_range(e, part, t0, cnt);
+                b.tasks = e.d_tasks + t0;
+                b.task0 = t0;
+                b.ntask = cnt;
+            }
+            HFK_HIP(hipEventRecord(c->events[4 * i + 2 * part], s));
+            rc = hfk_launch_quartets(e.la, e.lb, e.lc, e.ld, op, part, c->opt.boys_mode,
+                                     op == HFK_OP_JK ? e.mode : 0, b, s);
+            HFK_HIP(hipEventRecord(c->events[4 * i + 2 * part + 1], s));
+        }
         if (rc == -3) lds_fail = (int)i;
         else if (rc) missing = (int)i;
         c->plan[i].solo = (!conc || oi == 0) ? 1 : 0;
-        HFK_HIP(hipEventRecord(c->events[2 * i + 1], s));
         if (op == HFK_OP_JK && rc == 0) {
             hfk_launch_reduce_jcd(e.jl, s);
             hfk_launch_jk_part(e.jkp, c->n, c->d_sh_bf, c->d_sh_bfp, c->d_pair_A, c->d_pair_B, s);
@@ -1160,17 +1212,21 @@ int hfk_ctx_collect_times(hfk_ctx* c)
 {
     for (size_t i = 0; i < c->plan.size(); ++i) {
         PlanEntry& e = c->plan[i];
-        if (c->timed_op != HFK_OP_JK && e.prec == 1) { e.ms = 0.0; continue; }
-        float ms = 0.f;
-        HFK_HIP(hipEventElapsedTime(&ms, c->events[2 * i], c->events[2 * i + 1]));
-        e.ms = ms;
+        for (int part = 0; part < 2; ++part) {
+            e.ms[part] = 0.0;
+            if (part == 1 && !(c->timed_op == HFK_OP_JK && e.split)) continue;
+            float ms = 0.f;
+            HFK_HIP(hipEventElapsedTime(&ms, c->events[4 * i + 2 * part], c->events[4 * i + 2 * part + 1]));
+            e.ms[part] = ms;
+        }
     }
-    // the second serial J/K run ranks the launches for the concurrent ones that follow
+    // the second serial J/K run ranks the classes for the concurrent runs that follow
     if (c->timed_op == HFK_OP_JK && c->timed_serial && c->first_launch < 0 && !c->plan.empty() &&
         ++c->serial_runs >= 2) {
         size_t first = 0;
+        auto t = [&](size_t i) { return c->plan[i].ms[0] + c->plan[i].ms[1]; };
         for (size_t i = 1; i < c->plan.size(); ++i)
-            if (c->plan[i].ms > c->plan[first].ms) first = i;
+            if (t(i) > t(first)) first = i;
         c->first_launch = (int)first;
     }
     return HFK_OK;
@@ -1183,54 +1239,73 @@ static int hfk_ctx_count(hfk_ctx* c)
 {
     if (c->counted) return HFK_OK;
     unsigned long long* d_cnt = nullptr;
-    const size_t np = c->plan.size();
-    HFK_TRY(dev_alloc(&d_cnt, 3 * std::max<size_t>(np, 1)));
-    hfk_launch_fill((double*)d_cnt, 0.0, (long long)(3 * std::max<size_t>(np, 1)), c->stream);
+    const size_t np = c->plan.size(), nslot = 6 * std::max<size_t>(np, 1);   // 3 counts x 2 parts
+    HFK_TRY(dev_alloc(&d_cnt, nslot));
+    hfk_launch_fill((double*)d_cnt, 0.0, (long long)nslot, c->stream);
     QuartetArgs a = hfk_ctx_args(c);
     a.sch_cut = c->opt.schwarz_cut;
     for (size_t i = 0; i < np; ++i) {
         PlanEntry& e = c->plan[i];
         hfk_plan_args(c, e, a);
-        hfk_launch_count_tasks(a, ncart(e.la), ncart(e.lb), ncart(e.lc), ncart(e.ld), d_cnt + 3 * i, c->stream);
+        for (int part = 0; part < (e.split ? 2 : 1); ++part) {
+            QuartetArgs b = a;
+            long long t0, cnt;
+            part_range(e, part, t0, cnt);
+            b.tasks = e.d_tasks + t0;
+            b.task0 = t0;
+            b.ntask = cnt;
+            hfk_launch_count_tasks(b, ncart(e.la), ncart(e.lb), ncart(e.lc), ncart(e.ld), d_cnt + 6 * i + 3 * part,
+                                   c->stream);
+        }
     }
     HFK_HIP(hipGetLastError());
-    std::vector<unsigned long long> h(3 * std::max<size_t>(np, 1));
+    std::vector<unsigned long long> h(nslot);
     HFK_HIP(hipMemcpyAsync(h.data(), d_cnt, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
     HFK_HIP(hipStreamSynchronize(c->stream));
     dev_free(c, d_cnt);
     long long fq = 0;
-    for (size_t i = 0; i < np; ++i) {
-        c->plan[i].shell_q = (long long)h[3 * i];
-        c->plan[i].prim_q = (long long)h[3 * i + 1];
-        c->plan[i].func_q = (long long)h[3 * i + 2];
-        fq += c->plan[i].func_q;
-    }
+    for (size_t i = 0; i < np; ++i)
+        for (int part = 0; part < 2; ++part) {
+            PlanEntry& e = c->plan[i];
+            e.shell_q[part] = (long long)h[6 * i + 3 * part];
+            e.prim_q[part] = (long long)h[6 * i + 3 * part + 1];
+            e.func_q[part] = (long long)h[6 * i + 3 * part + 2];
+            fq += e.func_q[part];
+        }
     c->stats.func_quartets = fq;
     c->counted = true;
     return HFK_OK;
 }
 
+// One record per launch: each plan entry's FP64 part, then its FP32 part
+// when the entry is split.
 extern "C" int hfk_get_class_stats(hfk_ctx* c, hfk_class_stat* out, int32_t cap, int32_t* n)
 {
     if (!c || !n) return hfk_set_error(HFK_EINVAL, "hfk_get_class_stats: null argument");
     std::lock_guard<std::mutex> g(c->mu);
     HFK_TRY(hfk_ctx_count(c));
-    *n = (int32_t)c->plan.size();
-    for (size_t i = 0; i < c->plan.size() && (int32_t)i < cap && out; ++i) {
+    int32_t k = 0;
+    for (size_t i = 0; i < c->plan.size(); ++i) {
         const PlanEntry& e = c->plan[i];
-        hfk_class_stat& s = out[i];
-        s.la = e.la;
-        s.lb = e.lb;
-        s.lc = e.lc;
-        s.ld = e.ld;
-        s.prec = e.prec;
-        s.solo = e.solo;
-        s.shell_quartets = e.shell_q;
-        s.prim_quartets = e.prim_q;
-        s.func_quartets = e.func_q;
-        s.ms = e.ms;
-        s.tasks = e.ntask;
+        for (int part = 0; part < (e.split ? 2 : 1); ++part, ++k) {
+            if (!out || k >= cap) continue;
+            hfk_class_stat& s = out[k];
+            long long t0, cnt;
+            part_range(e, part, t0, cnt);
+            s.la = e.la;
+            s.lb = e.lb;
+            s.lc = e.lc;
+            s.ld = e.ld;
+            s.prec = part;
+            s.solo = e.solo;
+            s.shell_quartets = e.shell_q[part];
+            s.prim_quartets = e.prim_q[part];
+            s.func_quartets = e.func_q[part];
+            s.ms = e.ms[part];
+            s.tasks = cnt;
+        }
     }
+    *n = k;
     return HFK_OK;
 }
 

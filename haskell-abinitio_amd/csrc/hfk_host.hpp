@@ -38,16 +38,15 @@ struct HostShell {
 struct PlanEntry {
     int kb, kk;                 // bra / ket pair class
     int la, lb, lc, ld;
-    int prec;                   // 0 FP64, 1 FP32
-    int prec_sel;               // QuartetArgs.prec_sel for this launch
+    int split;                  // tasks [0, ntask64) run in FP64, [ntask64, ntask) in FP32
     long long nquart;           // launched quartet slots
-    long long shell_q, prim_q, func_q;   // counted survivors
-    double ms;                  // last timed duration
+    long long shell_q[2], prim_q[2], func_q[2];   // counted survivors of the FP64 / FP32 part
+    double ms[2];               // last timed duration of each part's launch
     int mode;                   // 0 main pass, 1 screen-correction pass (shares the main tasks)
     int4* d_tasks;              // tasks {bra pair, first run, end run, primitive ranges}
     KetRun* d_runs;             // runs [begin, end) of positions in d_kets, with their J'_cd entries
     int* d_kets;                // ket pair indices the tasks refer to (the ket class list)
-    long long ntask;
+    long long ntask, ntask64;
     int bra_stage;              // longest bra primitive-pair range of the tasks
     int task_len_max;           // longest ket sequence of the tasks
     int solo;                   // its last timed run had the GPU to itself

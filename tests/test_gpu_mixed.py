@@ -41,3 +41,22 @@ def test_mixed_fock_close_to_fp64(require_gpu, water8_fp64):
     with hfk.Context(mol, hfk.options(fp32_threshold=1e-4)) as c32:
         G32 = c32.calcGmatrix(D)
     assert np.max(np.abs(G32 - G64)) < 1e-5
+
+
+def test_mixed_parts_partition_the_quartets(require_gpu, water8_fp64):
+    """With the precision split, each class's FP64 and FP32 task parts
+    together hold exactly the quartets of the all-FP64 plan: the surviving
+    shell, primitive and function quartets per class add up, so no quartet
+    is evaluated twice or dropped."""
+    print("case: (H2O)8 mixed task parts", flush=True)
+    mol, _ = water8_fp64
+    key = lambda c: (c["la"], c["lb"], c["lc"], c["ld"])
+    with hfk.Context(mol) as c64:
+        ref = {key(c): c for c in c64.class_stats()}
+    with hfk.Context(mol, hfk.options(fp32_threshold=1e-4)) as c32:
+        parts = c32.class_stats()
+    assert {c["prec"] for c in parts} == {0, 1}
+    for k, r in ref.items():
+        got = [c for c in parts if key(c) == k]
+        for field in ("shell_quartets", "prim_quartets", "func_quartets"):
+            assert sum(c[field] for c in got) == r[field], (k, field)
