@@ -100,7 +100,8 @@ def gen_class(la, lb, lc, ld):
     out.append("// bra: nbp primitive-pair records (LDS or global); bq: their per-component")
     out.append("// screen values, bq[ib * %d + ab] (MODE 1)." % nbra)
     out.append("// kr: the ket's primitive-pair records (KetAoS or KetSoA).")
-    out.append("template <typename T, int BOYS, int MODE, typename KR>")
+    out.append("// BRAK = 1: the bra records' PP_K field already holds 2 pi^(5/2) K.")
+    out.append("template <typename T, int BOYS, int MODE, int BRAK, typename KR>")
     out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, const double* __restrict__ bq, int nbp," % name)
     out.append("    const KR& kr, int nkp, const BoysTables& bt, T* __restrict__ V, double qtau)")
     out.append("{")
@@ -142,7 +143,7 @@ def gen_class(la, lb, lc, ld):
         Eb = [hermite_E(em2, la, lb, "bA" + ax + sx, "bB" + ax + sx, "boo2p" + sx, "b" + ax) for ax in "xyz"]
         em2.raw("      T F%s[%d];" % (sx, L + 1))
         em2.raw("      boys_fast<T, %d, BOYS>(Targ%s, bt, F%s);" % (L, sx, sx))
-        em2.raw("      const T pref%s = T(HFK_TWO_PI_2_5) * bK%s * kK * rs%s;" % (sx, sx, sx))
+        em2.raw("      const T pref%s = (BRAK ? bK%s : T(HFK_TWO_PI_2_5) * bK%s) * kK * rs%s;" % (sx, sx, sx, sx))
         # base R^n_000 = pref * (-2 alpha)^n * F_n
         em2.raw("      const T m2a%s = T(-2) * alpha%s;" % (sx, sx))
         R = {}
@@ -244,13 +245,20 @@ def gen_class(la, lb, lc, ld):
     out.append("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
     out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
     vals = body(em2, "", "ib")
-    for vi in sorted(vals):
-        name_, bi, ki = vals[vi]
-        if multi:
-            em2.raw("      if constexpr (MODE == 0) V[%d] += %s;" % (vi, name_))
-            em2.raw("      else V[%d] += (allin || !(bqv[%d] * kqv[%d] < qtau)) ? %s : T(0);" % (vi, bi, ki, name_))
-        else:
-            em2.raw("      V[%d] += %s;" % (vi, name_))
+    if multi:
+        # every active lane keeps every component: plain sums (a uniform
+        # branch); otherwise the per-component selects
+        em2.raw("      if (MODE == 0 || __all(allin)) {")
+        for vi in sorted(vals):
+            em2.raw("        V[%d] += %s;" % (vi, vals[vi][0]))
+        em2.raw("      } else {")
+        for vi in sorted(vals):
+            name_, bi, ki = vals[vi]
+            em2.raw("        V[%d] += (allin || !(bqv[%d] * kqv[%d] < qtau)) ? %s : T(0);" % (vi, bi, ki, name_))
+        em2.raw("      }")
+    else:
+        for vi in sorted(vals):
+            em2.raw("      V[%d] += %s;" % (vi, vals[vi][0]))
     out.extend(em2.lines)
     out.append("    }")
     out.append("  }")
@@ -528,10 +536,10 @@ def main():
         body.append('#include "hfk_internal.hpp"')
         body.append("namespace {")
         body.append(src)
-        body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, typename KR>")
+        body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, int BRAK, typename KR>")
         body.append("__device__ __forceinline__ void eri_class(const double* __restrict__ bra, const double* __restrict__ bq, int nbp,")
         body.append("    const KR& kr, int nkp, const BoysTables& bt, T* __restrict__ V, double qtau)")
-        body.append("{ eri_%s<T, BOYS, MODE>(bra, bq, nbp, kr, nkp, bt, V, qtau); }" % tag)
+        body.append("{ eri_%s<T, BOYS, MODE, BRAK>(bra, bq, nbp, kr, nkp, bt, V, qtau); }" % tag)
         if not grad_coop:
             body.append(gen_grad_class(*q))
             body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, typename KR>")

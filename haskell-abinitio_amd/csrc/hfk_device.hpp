@@ -122,21 +122,25 @@ __device__ __forceinline__ void atomic_add_lds(double* p, double v) { unsafeAtom
 
 // floor(x / 0.1) of the reference (TableBoys.hs:47-48) without a division:
 // x*10 and x/0.1 differ by less than 2^-51 relative, so their floors can
-// differ only when x*10 lies within that distance of an integer; those rare
-// arguments take the exact division.
+// differ only when y = x*10 lies within y*2^-49 of an integer.  The cell
+// matters only up to y = 1100 (x_k <= 100 decides between the table and the
+// asymptote; above, x itself is used), where that distance is below 2e-12;
+// arguments whose fractional part lies within 2e-12 of an integer take the
+// exact division.  Beyond y = 1100 a cell off by one changes nothing.
 __device__ __forceinline__ double boys_ref_cell(double x)
 {
     const double y = x * 10.0;
     const double fl = floor(y);
-    const double gap = fmin(y - fl, (fl + 1.0) - y);
-    return (gap <= y * 0x1p-49) ? floor(x / 0.1) : fl;
+    const double fr = y - fl;   // exact
+    return (fabs(fr - 0.5) >= 0.5 - 2e-12) ? floor(x / 0.1) : fl;
 }
+// FP32: the same with x*10 and x/0.1 within 2^-22 relative (1100 * 2^-20 < 1.1e-3)
 __device__ __forceinline__ float boys_ref_cell(float x)
 {
     const float y = x * 10.0f;
     const float fl = floorf(y);
-    const float gap = fminf(y - fl, (fl + 1.0f) - y);
-    return (gap <= y * 0x1p-20f) ? floorf(x / 0.1f) : fl;
+    const float fr = y - fl;
+    return (fabsf(fr - 0.5f) >= 0.5f - 1.1e-3f) ? floorf(x / 0.1f) : fl;
 }
 
 // 1/sqrt(x): hardware estimate refined by Newton steps to full precision.
@@ -171,15 +175,17 @@ __device__ __forceinline__ void boys_fast(T x, const BoysTables& bt, T* F)
         if (xk <= T(100)) {
             int idx = (int)fl;
             idx = idx < 0 ? 0 : idx;   // NaN guard
-            const T* tab = boys_ref_tab<T>(bt) + (size_t)idx * BOYS_REF_STRIDE;
+            const T* tab = boys_ref_tab<T>(bt) + (unsigned)idx * BOYS_REF_STRIDE;
             T f[L + 6];
 #pragma unroll
             for (int m = 0; m < L + 6; ++m) f[m] = tab[m];
+            // sum_k (-dx)^k / k! f_{m+k}, k = 0..5, in Horner form with the
+            // shared factors d/k
             const T d = xk - x;   // -dx
-            const T w2 = (T(0.5) * d) * d, w3 = (T(1.0 / 3.0) * d) * w2, w4 = (T(0.25) * d) * w3, w5 = (T(0.2) * d) * w4;
+            const T d2 = T(0.5) * d, d3 = T(1.0 / 3.0) * d, d4 = T(0.25) * d, d5 = T(0.2) * d;
 #pragma unroll
             for (int m = 0; m <= L; ++m)
-                F[m] = f[m] + d * f[m + 1] + w2 * f[m + 2] + w3 * f[m + 3] + w4 * f[m + 4] + w5 * f[m + 5];
+                F[m] = f[m] + d * (f[m + 1] + d2 * (f[m + 2] + d3 * (f[m + 3] + d4 * (f[m + 4] + d5 * f[m + 5]))));
         } else {
             const T rs = rsqrt_full(x * T(1.0 / HFK_PI));   // sqrt(pi / x)
             const T inv = (rs * rs) * T(1.0 / HFK_PI);      // 1 / x

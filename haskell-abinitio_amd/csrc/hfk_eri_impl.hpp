@@ -31,7 +31,7 @@ __device__ __forceinline__ void eval_quartet(const QuartetArgs& a, int bp, int k
     const int nkp = (k1 < 0 ? a.pair_npp[kp] : k1) - k0;
     const double* bq = a.qcomp + (MODE == 1 ? (long long)bra[PP_QOFF] : 0LL);
     const KetAoS kr{ket, a.qcomp};
-    eri_class<LA, LB, LC, LD, T, BOYS, MODE>(bra, bq, nbp, kr, nkp, a.bt, V, a.qtau);
+    eri_class<LA, LB, LC, LD, T, BOYS, MODE, 0>(bra, bq, nbp, kr, nkp, a.bt, V, a.qtau);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -220,12 +220,17 @@ k_jk_tiles(QuartetArgs a)
         }
     }
     // the task's bra primitive-pair records and their per-component screen
-    // values, read by every lane in every ket iteration: staged once in LDS
+    // values, read by every lane in every ket iteration: staged once in LDS,
+    // the K field premultiplied by 2 pi^(5/2) (the product the ERI code
+    // forms first, so the rounding is the same)
     const int nbs = bprim1 - bprim0;
     double* sbq = sbra + nbs * PP_STRIDE;
     {
         const double* g = a.pp + (size_t)(a.pair_pp0[bp] + bprim0) * PP_STRIDE;
-        for (int t = tid; t < nbs * PP_STRIDE; t += nthr) sbra[t] = g[t];
+        for (int t = tid; t < nbs * PP_STRIDE; t += nthr) {
+            const double v = g[t];
+            sbra[t] = (t & (PP_STRIDE - 1)) == PP_K ? HFK_TWO_PI_2_5 * v : v;
+        }
         if constexpr (MODE == 1) {
             const double* q = a.qcomp + (long long)g[PP_QOFF];
             for (int t = tid; t < nbs * NBRA; t += nthr) sbq[t] = q[t];
@@ -281,7 +286,7 @@ k_jk_tiles(QuartetArgs a)
             {
                 const int slen = rlen[r];
                 const KetSoA<NF> kr{a.ksoa + rsoa[r] + pos + (size_t)kprim0 * NF * slen, slen};
-                eri_class<LA, LB, LC, LD, T, BOYS, MODE>(sbra, sbq, nbs, kr, min(kprim1, rnpp[r]) - kprim0, a.bt, Vt,
+                eri_class<LA, LB, LC, LD, T, BOYS, MODE, 1>(sbra, sbq, nbs, kr, min(kprim1, rnpp[r]) - kprim0, a.bt, Vt,
                                                          a.qtau);
             }
             int ic[NC], id[ND];
