@@ -112,7 +112,13 @@ def gen_class(la, lb, lc, ld):
     out.append("  for (int ik = 0; ik < nkp; ++ik) {")
     out.append("    const double* pk = kr.prim(ik);")
     em = Emitter()
-    em.raw("    const double kqlo = kr.at(pk, PP_QLO), kqhi = kr.at(pk, PP_QHI);")
+    # a single-component ket uses QHI for QLO and for its component's screen
+    # value: they equal it except that NaN is +inf there, and !(q_ab * v <
+    # tau) holds for v = NaN and v = +inf alike (q_ab >= 0 or NaN)
+    if nket > 1:
+        em.raw("    const double kqlo = kr.at(pk, PP_QLO), kqhi = kr.at(pk, PP_QHI);")
+    else:
+        em.raw("    const double kqhi = kr.at(pk, PP_QHI), kqlo = kqhi;")
     em.raw("    const T kq = T(kr.at(pk, PP_P));")
     em.raw("    const T kQx = T(kr.at(pk, PP_PX)), kQy = T(kr.at(pk, PP_PY)), kQz = T(kr.at(pk, PP_PZ));")
     em.raw("    const T kCx = T(kr.at(pk, PP_AX)), kCy = T(kr.at(pk, PP_AY)), kCz = T(kr.at(pk, PP_AZ));")
@@ -122,9 +128,16 @@ def gen_class(la, lb, lc, ld):
     if nket > 1 or nbra > 1:
         em.raw("    double kqv[%d];" % nket)
         em.raw("    if constexpr (MODE == 1) {")
-        em.raw("#pragma unroll")
-        em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = kr.q(pk, z);" % nket)
+        if nket > 1:
+            em.raw("#pragma unroll")
+            em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = kr.q(pk, z);" % nket)
+        else:
+            em.raw("      kqv[0] = kqhi;")
         em.raw("    }")
+    # the bra records are in descending order of QHI: a ket primitive screened
+    # against the first one is screened against all, and so are the later ket
+    # primitives (descending too)
+    em.raw("    if (MODE == 1 && kqhi * bra[PP_QHI] < qtau) break;")
     Ek = [hermite_E(em, lc, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
     out.extend(em.lines)
     def body(em2, sx, ibx):
@@ -240,7 +253,9 @@ def gen_class(la, lb, lc, ld):
     out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
     out.append("      // reference screen: component (ab|cd) of this primitive quartet is kept iff")
     out.append("      // !(q_ab*q_cd < tau).  MODE 0: no screen; MODE 1: the screen, branch-free.")
-    out.append("      if (MODE == 1 && bqhi * kqhi < qtau) continue;   // every component screened")
+    out.append("      // A pair's records are in descending order of QHI, so once every component")
+    out.append("      // is screened it stays screened for the rest of the bra primitives.")
+    out.append("      if (MODE == 1 && bqhi * kqhi < qtau) break;")
     out.append("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
     out.append("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
     out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
@@ -303,12 +318,13 @@ def gen_grad_class(la, lb, lc, ld):
         em.raw("#pragma unroll")
         em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = kr.q(pk, z);" % nket)
         em.raw("    }")
+    em.raw("    if (MODE == 1 && kqhi * bra[PP_QHI] < qtau) break;   // records by descending QHI")
     Ek = [hermite_E(em, lc + 1, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
     out.extend(em.lines)
     out.append("    for (int ib = 0; ib < nbp; ++ib) {")
     out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
     out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
-    out.append("      if (MODE == 1 && bqhi * kqhi < qtau) continue;")
+    out.append("      if (MODE == 1 && bqhi * kqhi < qtau) break;   // records by descending QHI")
     out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
     em2 = Emitter()
     em2.n = em.n

@@ -416,6 +416,20 @@ int hfk_ctx_build_pairs(hfk_ctx* c)
     hfk_launch_pair_setup(sd, c->npair, c->d_pair_A, c->d_pair_B, c->d_pair_pp0, d_q0, c->d_pp, c->d_qcomp,
                           c->d_pair_qhi, c->stream);
     HFK_HIP(hipGetLastError());
+    // records of each pair by descending largest screen value
+    {
+        double *sorted = nullptr, *qsorted = nullptr;
+        HFK_TRY(dev_alloc(&sorted, (size_t)npp_tot * PP_STRIDE));
+        HFK_TRY(dev_alloc(&qsorted, (size_t)std::max(1LL, nq_tot)));
+        hfk_launch_pair_sort(c->npair, c->d_pair_pp0, c->d_pair_npp, d_q0, nq_tot, c->d_pp, c->d_qcomp, sorted, qsorted,
+                             c->stream);
+        HFK_HIP(hipGetLastError());
+        HFK_HIP(hipStreamSynchronize(c->stream));
+        dev_free(c, c->d_pp);
+        dev_free(c, c->d_qcomp);
+        c->d_pp = sorted;
+        c->d_qcomp = qsorted;
+    }
     c->h_pair_qhi.resize(c->npair);
     HFK_HIP(hipMemcpyAsync(c->h_pair_qhi.data(), c->d_pair_qhi, sizeof(double) * c->npair, hipMemcpyDeviceToHost,
                            c->stream));

@@ -171,6 +171,10 @@ struct ShellDev {
     const double* scale;    // [nshell][HFK_MAXCOMP]
 };
 
+// Each pair's records sorted by descending QHI (stable) into out, their
+// per-component screen values into qout in the same order.
+void hfk_launch_pair_sort(int npair, const int* pair_pp0, const int* pair_npp, const int* pair_q0, long long nq_tot,
+                          const double* pp, const double* qcomp, double* out, double* qout, hipStream_t s);
 void hfk_launch_pair_setup(const ShellDev& sh, int npair, const int* pair_A, const int* pair_B,
                            const int* pair_pp0, const int* pair_q0, double* pp, double* qcomp,
                            double* pair_qhi, hipStream_t s);

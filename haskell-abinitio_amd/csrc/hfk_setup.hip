@@ -357,6 +357,48 @@ void hfk_launch_ket_soa(const double* pp, const double* qcomp, const int* pair_p
                        pos_len, pos_npp, nf, ncomp, out);
 }
 
+// Each pair's primitive-pair records in descending order of their largest
+// screen value QHI (stable: ties keep their index order), written to out,
+// with their per-component screen values moved along (qout), so that a
+// pair's value blocks stay contiguous in record order (the J/K kernel stages
+// a task's blocks in one copy).  With that order the reference screen
+// !(q_ab q_cd < tau) fails, for a ket primitive, on a suffix of the bra
+// primitives, and the per-lane ERI loops stop at the first failure.
+__global__ void k_pair_sort(int npair, const int* pair_pp0, const int* pair_npp, const int* pair_q0, long long nq_tot,
+                            const double* pp, const double* qcomp, double* out, double* qout)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npair) return;
+    const size_t base = (size_t)pair_pp0[p];
+    const int n = pair_npp[p];
+    if (n <= 0) return;
+    const long long q0 = pair_q0[p], q1 = p + 1 < npair ? pair_q0[p + 1] : nq_tot;
+    const int nc = (int)((q1 - q0) / n);   // screen values per record
+    for (int k = 0; k < n; ++k) {
+        const double qk = pp[(base + k) * PP_STRIDE + PP_QHI];
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            const double qj = pp[(base + j) * PP_STRIDE + PP_QHI];
+            rank += (qj > qk || (qj == qk && j < k)) ? 1 : 0;
+        }
+        const double* src = pp + (base + k) * PP_STRIDE;
+        double* dst = out + (base + rank) * PP_STRIDE;
+#pragma unroll
+        for (int f = 0; f < PP_STRIDE; ++f) dst[f] = src[f];
+        const long long qsrc = (long long)src[PP_QOFF], qdst = q0 + (long long)rank * nc;
+        for (int z = 0; z < nc; ++z) qout[qdst + z] = qcomp[qsrc + z];
+        dst[PP_QOFF] = (double)qdst;
+    }
+}
+
+void hfk_launch_pair_sort(int npair, const int* pair_pp0, const int* pair_npp, const int* pair_q0, long long nq_tot,
+                          const double* pp, const double* qcomp, double* out, double* qout, hipStream_t s)
+{
+    if (npair <= 0) return;
+    hipLaunchKernelGGL(k_pair_sort, dim3((npair + 127) / 128), dim3(128), 0, s, npair, pair_pp0, pair_npp, pair_q0,
+                       nq_tot, pp, qcomp, out, qout);
+}
+
 void hfk_launch_pair_setup(const ShellDev& sh, int npair, const int* pair_A, const int* pair_B,
                            const int* pair_pp0, const int* pair_q0, double* pp, double* qcomp,
                            double* pair_qhi, hipStream_t s)
