@@ -326,6 +326,138 @@ def fock_rate(ctx, D, steps, warmup, workload):
     }
 
 
+def serial_context(mol, opts=None):
+    """A context whose class launches run one after another (HFK_CONCURRENT=0
+    at creation), so every per-class time is the launch's own."""
+    import hfk
+    old = os.environ.get("HFK_CONCURRENT")
+    os.environ["HFK_CONCURRENT"] = "0"
+    try:
+        return hfk.Context(mol, opts)
+    finally:
+        if old is None:
+            del os.environ["HFK_CONCURRENT"]
+        else:
+            os.environ["HFK_CONCURRENT"] = old
+
+
+def gemm_record(mol, reps=5):
+    """The SCF's FP64 matrix-core GEMMs at the workload's size: X^T F X
+    (hfk_unitary, two n^3 GEMMs) and D = 2 C_occ C_occ^T (hfk_density,
+    n x n x occ), timed by the library's HIP events around the GEMM
+    launches (the host copies of the ABI call excluded); roofline against
+    the FP64 MFMA peak."""
+    import numpy as np
+    import hfk
+    n = mol.nbf
+    occ = int(round(mol.Z.sum() - mol.charge)) // 2
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((n, n))
+    A = rng.standard_normal((n, n))
+    Ff = (A + A.T)[np.triu_indices(n)]
+    Cm = rng.standard_normal((n, n))
+    out = {"n": n, "occ": occ, "peak": PEAK_FP64, "unit": "TFLOP/s", "bound": "fp64_mfma"}
+    with hfk.Context(mol) as ctx:
+        ctx.stats()   # the one-time quartet count, outside the timed calls
+        for name, call in (("unitary_XtFX", lambda: ctx.unitaryTransf(X, Ff)),
+                           ("density_2CCt", lambda: ctx.calcDensity(Cm, occ))):
+            call()
+            ms = []
+            for _ in range(reps):
+                call()
+                st = ctx.stats()
+                ms.append(st["last_gemm_ms"])
+            t = sum(ms) / len(ms)
+            achieved = st["last_gemm_flops"] / (t * 1e-3) / 1e12
+            out[name] = {"ms": t, "flops": st["last_gemm_flops"], "achieved": achieved, "frac": achieved / PEAK_FP64}
+            log("bench: gemm %s n=%d %.3f ms %.2f TFLOP/s (%.1f%% of FP64 MFMA peak)"
+                % (name, n, t, achieved, 100 * achieved / PEAK_FP64))
+    return out
+
+
+def grad_roofline(mol, D, desc):
+    """Per-class gradient launches (derivative integrals contracted with the
+    two-particle density) on a context whose classes run one after another:
+    each launch's time is its own.  Algorithmic flops from hfk.flops
+    (f_grad_prim); roofline of the longest launch against the FP64 vector
+    peak."""
+    from hfk.flops import grad_launch_flops
+    ctx = serial_context(mol)
+    try:
+        ctx.gradient(D)
+        t0 = time.perf_counter()
+        ctx.gradient(D)
+        wall = time.perf_counter() - t0
+        cs = [c for c in ctx.class_stats() if c["prec"] == 0]
+    finally:
+        ctx.close()
+    classes = []
+    for c in cs:
+        fl = grad_launch_flops(c)
+        ms = c["grad_ms"]
+        classes.append({"class": "(%s%s|%s%s)" % tuple("spdf"[c[k]] for k in ("la", "lb", "lc", "ld")), "ms": ms,
+                        "prim_quartets": c["prim_quartets"], "alg_flops": fl,
+                        "alg_tflops": fl / (ms * 1e-3) / 1e12 if ms > 0 else None})
+    dom = max(classes, key=lambda x: x["ms"])
+    ach = dom["alg_flops"] / (dom["ms"] * 1e-3) / 1e12
+    tot_ms = sum(x["ms"] for x in classes)
+    tot_fl = sum(x["alg_flops"] for x in classes)
+    log("bench: gradient kernels %s: %.1f ms serial, dominant %s %.1f ms %.1f%% of FP64 peak"
+        % (desc, tot_ms, dom["class"], dom["ms"], 100 * ach / PEAK_FP64))
+    return {"workload": desc, "gradient_wall_s_serial_classes": wall, "kernel_ms_sum": tot_ms,
+            "alg_tflops_all_classes": tot_fl / (tot_ms * 1e-3) / 1e12, "classes": classes,
+            "roofline": {"bound": "fp64_valu", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                         "frac": ach / PEAK_FP64, "kernel": "gradient %s (derivative ERIs x 2-PDM)" % dom["class"],
+                         "launch_ms": dom["ms"], "alg_flops_per_launch": dom["alg_flops"]}}
+
+
+def dclass_record(steps=3):
+    """C12H26/6-31G* (Cartesian d, 232 functions): per-class J/K launches
+    run one after another, with the cooperative d-class kernels; each
+    class's algorithmic rate against the FP64 vector peak."""
+    import numpy as np
+    from hfk import systems
+    from hfk.flops import launch_flops
+    mol = systems.alkane(12, "6-31g*")
+    n = mol.nbf
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((n, n)) * 0.01
+    D = (A + A.T)[np.triu_indices(n)]
+    ctx = serial_context(mol)
+    try:
+        ctx.stats()
+        ms = {}
+        for _ in range(steps):
+            ctx.calcGmatrix(D)
+            for c in ctx.class_stats():
+                k = "(%s%s|%s%s)" % tuple("spdf"[c[x]] for x in ("la", "lb", "lc", "ld"))
+                ms.setdefault(k, []).append((c["ms"], c))
+        eri_ms = ctx.stats()["last_eri_ms"]
+    finally:
+        ctx.close()
+    classes = []
+    for k, v in ms.items():
+        t = sum(x[0] for x in v) / len(v)
+        c = v[-1][1]
+        fl = launch_flops(c)
+        coop = max(c["la"], c["lb"], c["lc"], c["ld"]) == 2 and \
+            (c["la"] + 1) * (c["la"] + 2) // 2 * ((c["lb"] + 1) * (c["lb"] + 2) // 2) * \
+            ((c["lc"] + 1) * (c["lc"] + 2) // 2) * ((c["ld"] + 1) * (c["ld"] + 2) // 2) > 54
+        classes.append({"class": k, "kernel": "k_coop" if coop else "k_jk_tiles", "ms": t,
+                        "prim_quartets": c["prim_quartets"], "alg_flops": fl,
+                        "alg_tflops": fl / (t * 1e-3) / 1e12 if t > 0 else None})
+    co = [x for x in classes if x["kernel"] == "k_coop"]
+    dom = max(co, key=lambda x: x["ms"])
+    ach = dom["alg_flops"] / (dom["ms"] * 1e-3) / 1e12
+    log("bench: c12h26 Fock build %.2f ms (classes serial); d-class %s %.2f ms %.1f%% of FP64 peak"
+        % (eri_ms, dom["class"], dom["ms"], 100 * ach / PEAK_FP64))
+    return {"workload": "C12H26 RHF/6-31G* (Cartesian d), 232 basis functions, all FP64; random density",
+            "eri_ms_serial": eri_ms, "classes": classes,
+            "roofline": {"bound": "fp64_valu", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                         "frac": ach / PEAK_FP64, "kernel": "k_coop %s" % dom["class"], "launch_ms": dom["ms"],
+                         "alg_flops_per_launch": dom["alg_flops"]}}
+
+
 def headline_density(mol, big):
     """Converged density of the workload: from the fragment guess with
     incremental builds for the large clusters, else from the core guess."""
@@ -368,6 +500,9 @@ def main():
     ap.add_argument("--no-target", action="store_true", help="skip the (H2O)_64 FP64-vs-mixed energy+gradient record")
     ap.add_argument("--grad-workloads", default="h2o16,c12h26",
                     help="comma list of gradient/SCF workloads (h2oN or c12h26)")
+    ap.add_argument("--no-gemm", action="store_true", help="skip the GEMM roofline record")
+    ap.add_argument("--no-gradroof", action="store_true", help="skip the per-class gradient roofline record")
+    ap.add_argument("--no-dclass", action="store_true", help="skip the C12H26 d-class record")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before timing, run one device copy of this size (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -446,6 +581,12 @@ def main():
                        "ms_per_step": r8["ms_per_step"], "roofline": r8["roofline"], "classes": r8["classes"],
                        "scf": scf8}
         log("bench: h2o8 Fock build %.3f ms, %.3e quartets/s" % (r8["eri_ms"], r8["value"]))
+    if not args.no_gemm:
+        out["gemm"] = gemm_record(mol)
+    if not args.no_gradroof:
+        out["gradient_roofline"] = grad_roofline(mol, D, desc)
+    if not args.no_dclass:
+        out["dclass_c12h26"] = dclass_record()
     if not args.no_mixed:
         out["mixed_vs_fp64"] = mixed_vs_fp64(args.mixed_workload, args.mixed_thr)
     if not args.no_grad:

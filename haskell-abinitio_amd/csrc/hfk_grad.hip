@@ -117,16 +117,20 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
         for (int k = 0; k < hfk_ctx::NAUX; ++k) HFK_HIP(hipStreamWaitEvent(c->aux[k], c->ev_fork, 0));
     }
     int slot = 0;
-    for (const PlanEntry& e : c->plan) {
-        hfk_plan_args(c, e, a);   // FP32 entries select their quartets; all run in FP64 here
+    const size_t np = c->plan.size();
+    for (size_t i = 0; i < np; ++i) {
+        const PlanEntry& e = c->plan[i];
+        hfk_plan_args(c, e, a);   // the whole task list (both precision parts) in FP64
         // the bra shells' density rows in LDS while four waves per SIMD still fit
         a.drows = !hfk_class_grad_coop(e.la, e.lb, e.lc, e.ld) &&
                   hfk_grad_lds_bytes(e.la, e.lb, e.ncols, natom, 1) <= hfk_lds_budget(a.block);
         hipStream_t s = conc ? c->aux[slot++ % hfk_ctx::NAUX] : c->stream;
+        HFK_HIP(hipEventRecord(c->events[4 * np + 2 * i], s));
         if (hfk_launch_grad(e.la, e.lb, e.lc, e.ld, c->opt.boys_mode, e.mode, a, s)) {
             rc = hfk_set_error(HFK_EUNSUP, "no gradient kernel for class (%d%d|%d%d)", e.la, e.lb, e.lc, e.ld);
             break;
         }
+        HFK_HIP(hipEventRecord(c->events[4 * np + 2 * i + 1], s));
     }
     if (conc)
         for (int k = 0; k < hfk_ctx::NAUX; ++k) {
@@ -142,6 +146,12 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
     }
     dev_free(c, d_grad);
     if (rc != HFK_OK) return rc;
+    // per-class launch times (their own when the classes ran serially)
+    for (size_t i = 0; i < np; ++i) {
+        float ms = 0.f;
+        HFK_HIP(hipEventElapsedTime(&ms, c->events[4 * np + 2 * i], c->events[4 * np + 2 * i + 1]));
+        c->plan[i].grad_ms = ms;
+    }
     // nuclear repulsion: d/dR_A sum_{B<C} Z_B Z_C / |R_B - R_C|
     for (int A = 0; A < natom; ++A)
         for (int B = 0; B < natom; ++B) {

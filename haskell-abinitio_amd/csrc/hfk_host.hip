@@ -1100,7 +1100,7 @@ int hfk_ctx_plan(hfk_ctx* c)
     c->serial_runs = 0;
     const size_t np = c->plan.size();
     // event pool for per-launch timing
-    while (c->events.size() < 4 * np + 2) {
+    while (c->events.size() < 6 * np + 2) {   // J/K parts, then the gradient launches
         hipEvent_t ev;
         HFK_HIP(hipEventCreate(&ev));
         c->events.push_back(ev);
@@ -1317,6 +1317,7 @@ extern "C" int hfk_get_class_stats(hfk_ctx* c, hfk_class_stat* out, int32_t cap,
             s.func_quartets = e.func_q[part];
             s.ms = e.ms[part];
             s.tasks = cnt;
+            s.grad_ms = part == 0 ? e.grad_ms : 0.0;
         }
     }
     *n = k;

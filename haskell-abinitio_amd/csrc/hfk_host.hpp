@@ -42,6 +42,7 @@ struct PlanEntry {
     long long nquart;           // launched quartet slots
     long long shell_q[2], prim_q[2], func_q[2];   // counted survivors of the FP64 / FP32 part
     double ms[2];               // last timed duration of each part's launch
+    double grad_ms;             // its launch in the last gradient (all tasks, FP64)
     int mode;                   // 0 main pass, 1 screen-correction pass (shares the main tasks)
     int4* d_tasks;              // tasks {bra pair, first run, end run, primitive ranges}
     KetRun* d_runs;             // runs [begin, end) of positions in d_kets, with their J'_cd entries

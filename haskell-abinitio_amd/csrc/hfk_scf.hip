@@ -234,6 +234,17 @@ extern "C" void hfk_default_scf_options(hfk_scf_options* o)
     o->charge = 0.0;
 }
 
+// After a synchronised call: the GEMM time between the context's ev0 and
+// ev1 and its flop count into the statistics.
+static int record_gemm_time(hfk_ctx* c, double flops)
+{
+    float ms = 0.f;
+    HFK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->stats.last_gemm_ms = ms;
+    c->stats.last_gemm_flops = flops;
+    return HFK_OK;
+}
+
 extern "C" int hfk_density(hfk_ctx* c, const double* C, int32_t occ, double* D_flat)
 {
     if (!c || !C || !D_flat || occ < 0 || occ > c->n) return hfk_set_error(HFK_EINVAL, "hfk_density: bad argument");
@@ -243,10 +254,13 @@ extern "C" int hfk_density(hfk_ctx* c, const double* C, int32_t occ, double* D_f
     const int n = c->n;
     if (rc == HFK_OK) {
         hipMemcpyAsync(W.C, C, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream);
+        hipEventRecord(c->ev0, c->stream);
         density_dev(W, W.C, occ, W.T1);
+        hipEventRecord(c->ev1, c->stream);
         hfk_launch_full_to_flat(n, W.T1, c->d_flatA, c->stream);
         hipMemcpyAsync(D_flat, c->d_flatA, sizeof(double) * n * (n + 1) / 2, hipMemcpyDeviceToHost, c->stream);
         if (hipStreamSynchronize(c->stream) != hipSuccess) rc = hfk_set_error(HFK_EHIP, "hfk_density: HIP failure");
+        else rc = record_gemm_time(c, 2.0 * n * n * (double)occ);
     }
     work_free(W);
     return rc;
@@ -282,9 +296,12 @@ extern "C" int hfk_unitary(hfk_ctx* c, const double* X, const double* F_flat, do
         hipMemcpyAsync(W.X, X, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream);
         hipMemcpyAsync(c->d_flatA, F_flat, sizeof(double) * n * (n + 1) / 2, hipMemcpyHostToDevice, c->stream);
         hfk_launch_flat_to_full(n, c->d_flatA, W.Ffull, c->stream);
+        hipEventRecord(c->ev0, c->stream);
         unitary_dev(W, W.X, W.Ffull, W.T1);
+        hipEventRecord(c->ev1, c->stream);
         hipMemcpyAsync(Fp, W.T1, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->stream);
         if (hipStreamSynchronize(c->stream) != hipSuccess) rc = hfk_set_error(HFK_EHIP, "hfk_unitary: HIP failure");
+        else rc = record_gemm_time(c, 4.0 * n * (double)n * n);
     }
     work_free(W);
     return rc;

@@ -61,7 +61,8 @@ class HfkStats(C.Structure):
     _fields_ = [
         ("nshell", C.c_int64), ("nshell_pair", C.c_int64), ("nprim_pair", C.c_int64),
         ("quartets_launched", C.c_int64), ("func_quartets", C.c_int64),
-        ("last_fock_ms", C.c_double), ("last_eri_ms", C.c_double),
+        ("last_fock_ms", C.c_double), ("last_eri_ms", C.c_double), ("last_gemm_ms", C.c_double),
+        ("last_gemm_flops", C.c_double),
     ]
 
 
@@ -70,7 +71,7 @@ class HfkClassStat(C.Structure):
         ("la", C.c_int32), ("lb", C.c_int32), ("lc", C.c_int32), ("ld", C.c_int32),
         ("prec", C.c_int32), ("solo", C.c_int32), ("shell_quartets", C.c_int64),
         ("prim_quartets", C.c_int64), ("func_quartets", C.c_int64), ("ms", C.c_double),
-        ("tasks", C.c_int64),
+        ("tasks", C.c_int64), ("grad_ms", C.c_double),
     ]
 
 

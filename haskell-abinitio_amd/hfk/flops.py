@@ -37,3 +37,22 @@ def launch_flops(st):
     """Algorithmic flops of one class launch record (hfk.Context.class_stats)."""
     q = (st["la"], st["lb"], st["lc"], st["ld"])
     return st["prim_quartets"] * f_prim(*q) + st["shell_quartets"] * f_hrr(*q)
+
+
+def f_grad_prim(la, lb, lc, ld):
+    """Algorithmic flops of the nuclear-gradient contraction per surviving
+    primitive quartet, as a lower bound: the derivatives with respect to
+    three centres (the fourth follows from translational invariance), each
+    counted at the cost of the energy class itself, f_prim(la, lb, lc, ld)
+    (a derivative class has one angular momentum raised, so it costs more),
+    plus 4 flops (derivative combination and the two-particle density
+    product) for each of the 9 Cartesian derivatives of each component
+    quartet."""
+    return 3 * f_prim(la, lb, lc, ld) + 36 * nc(la) * nc(lb) * nc(lc) * nc(ld)
+
+
+def grad_launch_flops(st):
+    """Algorithmic flops of one class's gradient launch (the same quartets
+    as its J/K launch: a class_stats record)."""
+    q = (st["la"], st["lb"], st["lc"], st["ld"])
+    return st["prim_quartets"] * f_grad_prim(*q)

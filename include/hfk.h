@@ -200,6 +200,8 @@ typedef struct hfk_stats {
     int64_t func_quartets;        /* canonical function quartets they hold               */
     double last_fock_ms;          /* wall time of the last Fock build (ms, HIP events)   */
     double last_eri_ms;           /* ERI+digestion kernel time of the last Fock build    */
+    double last_gemm_ms;          /* MFMA GEMM time of the last hfk_unitary / hfk_density call (HIP events) */
+    double last_gemm_flops;       /* its floating-point operations (2mnk per GEMM)       */
 } hfk_stats;
 
 int hfk_get_stats(hfk_ctx *ctx, hfk_stats *st);
@@ -216,6 +218,8 @@ typedef struct hfk_class_stat {
     int64_t func_quartets;        /* canonical function quartets they hold              */
     double ms;                    /* kernel time of the last Fock build / export        */
     int64_t tasks;                /* workgroups of its J/K launch (bra pair x ket run)   */
+    double grad_ms;               /* its derivative-integral launch in the last hfk_gradient
+                                     (FP64 records; 0 for the FP32 part)                  */
 } hfk_class_stat;
 
 int hfk_get_class_stats(hfk_ctx *ctx, hfk_class_stat *out, int32_t cap, int32_t *n);

@@ -41,6 +41,10 @@ def test_flops_model_anchors():
     assert (flops.f_prim(1, 1, 1, 0), flops.f_hrr(1, 1, 1, 0)) == (606, 54)
     assert (flops.f_prim(1, 1, 1, 1), flops.f_hrr(1, 1, 1, 1)) == (1695, 324)
     assert (flops.f_prim(2, 2, 2, 2), flops.f_hrr(2, 2, 2, 2)) == (30355, 11256)
+    # gradient model: three centres at the energy class's cost plus 9 x 4
+    # contraction flops per component quartet
+    assert flops.f_grad_prim(0, 0, 0, 0) == 3 * 43 + 36
+    assert flops.f_grad_prim(1, 0, 1, 0) == 3 * 225 + 36 * 9
 
 
 def test_basis_parser_keeps_sp_exponents():
