@@ -430,47 +430,9 @@ int hfk_ctx_build_pairs(hfk_ctx* c)
         c->d_pp = sorted;
         c->d_qcomp = qsorted;
     }
-    c->h_pair_qhi.resize(c->npair);
-    HFK_HIP(hipMemcpyAsync(c->h_pair_qhi.data(), c->d_pair_qhi, sizeof(double) * c->npair, hipMemcpyDeviceToHost,
-                           c->stream));
-    HFK_HIP(hipStreamSynchronize(c->stream));
     c->stats.nshell = ns;
     c->stats.nprim_pair = npp_tot;
     return hfk_ctx_classify_pairs(c);
-}
-
-// Pair classes and the reference-exact pair screen: a pair whose largest
-// screen value times the largest over all pairs is below tau can never pass
-// the primitive screen, so it is dropped.
-int hfk_ctx_classify_pairs(hfk_ctx* c)
-{
-    const bool ref = c->opt.screen_mode == HFK_SCREEN_REFERENCE && c->opt.screen_tau > 0.0;
-    double gmax = 0.0;
-    for (double q : c->h_pair_qhi) gmax = std::max(gmax, q);   // NaN screened as +inf upstream
-    std::vector<std::vector<int>> lists(HFK_NPC);
-    long long kept = 0;
-    for (int p = 0; p < c->npair; ++p) {
-        if (ref && c->h_pair_qhi[p] * gmax < c->opt.screen_tau) continue;
-        const int la = c->shells[c->h_pair_A[p]].L, lb = c->shells[c->h_pair_B[p]].L;
-        lists[pair_class(la, lb)].push_back(p);
-        ++kept;
-    }
-    // by primitive-pair count, then largest screen value first (the task
-    // lists of hfk_ctx_plan rely on this order)
-    for (int k = 0; k < HFK_NPC; ++k)
-        std::stable_sort(lists[k].begin(), lists[k].end(), [&](int p, int q) {
-            if (c->h_pair_npp[p] != c->h_pair_npp[q]) return c->h_pair_npp[p] < c->h_pair_npp[q];
-            return c->h_pair_qhi[p] > c->h_pair_qhi[q];
-        });
-    for (int k = 0; k < HFK_NPC; ++k) {
-        if (c->d_class[k]) { dev_free(c, c->d_class[k]); c->d_class[k] = nullptr; }
-        if (c->d_ksoa[k]) { dev_free(c, c->d_ksoa[k]); c->d_ksoa[k] = nullptr; }
-        c->class_count[k] = (long long)lists[k].size();
-        c->h_class[k] = lists[k];
-        if (!lists[k].empty()) HFK_TRY(dev_upload(c, &c->d_class[k], lists[k]));
-    }
-    c->stats.nshell_pair = kept;
-    return HFK_OK;
 }
 
 QuartetArgs hfk_ctx_args(hfk_ctx* c)
@@ -512,14 +474,8 @@ int hfk_ctx_schwarz(hfk_ctx* c)
         int rc = hfk_launch_quartets(la, lb, la, lb, HFK_OP_DIAG, 0, c->opt.boys_mode, 0, a, c->stream);
         if (rc) return hfk_set_error(HFK_EUNSUP, "no kernel for class (%d%d|%d%d)", la, lb, la, lb);
     }
+    hfk_launch_sqrt(c->npair, c->d_pair_diag, c->d_pair_sch, c->stream);
     HFK_HIP(hipGetLastError());
-    std::vector<double> diag(c->npair);
-    HFK_HIP(hipMemcpyAsync(diag.data(), c->d_pair_diag, sizeof(double) * c->npair, hipMemcpyDeviceToHost, c->stream));
-    HFK_HIP(hipStreamSynchronize(c->stream));
-    c->h_pair_sch.resize(c->npair);
-    for (int p = 0; p < c->npair; ++p) c->h_pair_sch[p] = sqrt(diag[p]);
-    HFK_HIP(hipMemcpyAsync(c->d_pair_sch, c->h_pair_sch.data(), sizeof(double) * c->npair, hipMemcpyHostToDevice,
-                           c->stream));
     HFK_HIP(hipStreamSynchronize(c->stream));
     return HFK_OK;
 }
@@ -592,13 +548,13 @@ int hfk_ctx_plan(hfk_ctx* c)
         }
     }
     const int nshell = (int)c->shells.size();
-    // Task lists of the J/K (and gradient) kernels.  Class lists are ordered
-    // by primitive-pair count, then by descending screen value, so for each
-    // bra pair the kets surviving the pair screen are a prefix of each
-    // segment of equal primitive-pair count.  A task is a bra pair, a list of
-    // runs (such prefixes, as position ranges in the ket class list) and
-    // primitive ranges of bra and ket.  Lanes walk the runs in order, so a
-    // wave mostly meets one primitive-pair count.
+    // Task lists of the J/K (and gradient) kernels, built on the device
+    // (hfk_plan.hip).  Class lists are ordered by primitive-pair count, then
+    // by descending screen value, so for each bra pair the kets surviving the
+    // pair screen are a prefix of each segment of equal primitive-pair count.
+    // A task is a bra pair, a list of runs (such prefixes, as position ranges
+    // in the ket class list) and primitive ranges of bra and ket.  Lanes walk
+    // the runs in order, so a wave mostly meets one primitive-pair count.
     //
     // Slots.  Each task stores its K' rows (the bra shells' rows over the
     // launch's compact columns) and its J'_ab; each (bra, ket, primitive
@@ -612,101 +568,6 @@ int hfk_ctx_plan(hfk_ctx* c)
         const int block = hfk_plan_block(c, e);
         const int NA = ncart(e.la), NB = ncart(e.lb), NC = ncart(e.lc), ND = ncart(e.ld);
         const int nrow = NA + NB, nbra_comp = NA * NB, nket_comp = NC * ND;
-        // cooperative kernel: S = 64 / (na nb) kets at a time
-        const int coop_s = std::max(1, 64 / nbra_comp);
-        // kets per task of the primitive-split (heavy) segments: every task
-        // stores its K' rows, so large systems take longer ones
-        const int chunk = coop ? 16 * coop_s : block * c->kets_per_lane * (c->n >= 200 ? 16 : 1);
-        const std::vector<int>& bl = c->h_class[e.kb];
-        const std::vector<int>& kl = c->h_class[e.kk];
-        std::vector<double> kq(kl.size());
-        for (size_t j = 0; j < kl.size(); ++j) kq[j] = c->h_pair_qhi[kl[j]];
-        auto segments = [&](const std::vector<int>& lst) {
-            std::vector<std::pair<long long, long long>> out;   // runs of equal primitive-pair count
-            for (size_t j = 0; j < lst.size();) {
-                size_t k = j + 1;
-                while (k < lst.size() && c->h_pair_npp[lst[k]] == c->h_pair_npp[lst[j]]) ++k;
-                out.push_back({(long long)j, (long long)k});
-                j = k;
-            }
-            return out;
-        };
-        const auto segs = segments(kl), bsegs = segments(bl);
-        const int nkseg = (int)segs.size(), nbseg = (int)bsegs.size();
-        // position-major ket records of the class: segment s at soa_base[s],
-        // field f of primitive ik of position p at soa_base[s] + (p - first)
-        // + (ik * NF + f) * len
-        const int NF = hfk_ket_fields(e.lc, e.ld);
-        std::vector<long long> soa_base(nkseg + 1, 0);
-        for (int sg = 0; sg < nkseg; ++sg)
-            soa_base[sg + 1] = soa_base[sg] + (long long)NF * c->h_pair_npp[kl[segs[sg].first]] *
-                                                  (segs[sg].second - segs[sg].first);
-        // surviving kets of bra i in segment sg: [sg.first, hi) by binary search
-        auto staircase = [&](size_t i, const std::pair<long long, long long>& sg) -> long long {
-            const double qi = c->h_pair_qhi[bl[i]];
-            // kets need qhi_j >= tau / qhi_i; relaxed so that rounding never drops a survivor
-            const double need = (!ref || std::isinf(qi)) ? -1.0
-                              : (qi > 0.0 ? c->opt.screen_tau / qi * (1.0 - 1e-12) : INFINITY);
-            long long hi_end = sg.second;
-            if (e.kb == e.kk) hi_end = std::min<long long>(hi_end, (long long)i + 1);
-            if (hi_end <= sg.first) return sg.first;
-            long long lo = sg.first, hi = hi_end;
-            while (lo < hi) {
-                const long long mid = (lo + hi) / 2;
-                if (kq[mid] >= need) lo = mid + 1; else hi = mid;
-            }
-            return lo;
-        };
-        // staircase ends, bra-major [i][kseg]
-        std::vector<int> jend(bl.size() * (size_t)nkseg);
-        long long est_prim = 0;
-        for (size_t i = 0; i < bl.size(); ++i)
-            for (int s = 0; s < nkseg; ++s) {
-                const long long end = staircase(i, segs[s]);
-                jend[i * nkseg + s] = (int)end;
-                est_prim += (long long)c->h_pair_npp[bl[i]] * c->h_pair_npp[kl[segs[s].first]] * (end - segs[s].first);
-            }
-        // Primitive ranges: a lane's share of a task is about per_lane primitive
-        // quartets (at most 128), sized from the launch's total so that small
-        // launches still fill the GPU (~1.3e5 lanes of work) and large ones
-        // keep quartets whole.  Bra primitives are wave-uniform (LDS
-        // broadcast), so a lane keeps as many of them as fit and the ket range
-        // is split.  Segments that need no split join the long multi-run tasks.
-        const int L = e.la + e.lb + e.lc + e.ld;
-        const long long floor_l = std::max(4, 32 >> L);   // cheap classes: amortise the digestion
-        const int per_lane = (int)std::max(floor_l, std::min(128LL, est_prim / 131072));
-        const bool single = coop || hfk_class_grad_coop(e.la, e.lb, e.lc, e.ld);   // cooperative kernels read these
-        auto whole = [&](int nbp, int nkp) { return !single && nbp <= HFK_BRA_STAGE && nbp * nkp <= per_lane; };
-        auto steps = [&](int nbp, int nkp, int& bstep, int& kstep) {
-            bstep = nbp;
-            kstep = nkp;
-            if (!coop && nbp * nkp > per_lane) {
-                bstep = std::max(1, std::min(nbp, per_lane));
-                kstep = std::max(1, std::min(nkp, per_lane / bstep));
-            }
-            if (!coop) bstep = std::min(bstep, HFK_BRA_STAGE);   // staged in LDS by the kernel
-        };
-        // pieces per (bra segment, ket segment) and the J'_cd entry layout
-        std::vector<int> npiece((size_t)nbseg * nkseg, 1);
-        for (int b = 0; b < nbseg; ++b)
-            for (int s = 0; s < nkseg; ++s) {
-                const int nbp = c->h_pair_npp[bl[bsegs[b].first]], nkp = c->h_pair_npp[kl[segs[s].first]];
-                if (whole(nbp, nkp)) continue;
-                int bstep, kstep;
-                steps(nbp, nkp, bstep, kstep);
-                npiece[(size_t)b * nkseg + s] = ((nbp + bstep - 1) / bstep) * ((nkp + kstep - 1) / kstep);
-            }
-        std::vector<int> bseg_of(bl.size());
-        for (int b = 0; b < nbseg; ++b)
-            for (long long i = bsegs[b].first; i < bsegs[b].second; ++i) bseg_of[i] = b;
-        std::vector<long long> jbase(bl.size() * (size_t)nkseg);
-        long long njcd = 0;
-        for (size_t i = 0; i < bl.size(); ++i)
-            for (int s = 0; s < nkseg; ++s) {
-                jbase[i * nkseg + s] = njcd;
-                njcd += (long long)npiece[(size_t)bseg_of[i] * nkseg + s] * (jend[i * nkseg + s] - segs[s].first);
-            }
-        e.njcd = njcd;
         // compact K columns: the functions of the ket class's shells
         std::vector<int> colmap(c->n + 1, -1), collist;
         for (int S = 0; S < nshell; ++S) {
@@ -723,151 +584,10 @@ int hfk_ctx_plan(hfk_ctx* c)
         e.ncols = (int)collist.size();
         for (int f = 0; f <= c->n; ++f)
             if (colmap[f] < 0) colmap[f] = e.ncols;   // never a ket column (or the padding): discarded
-        // Precision parts.  With the mixed-precision split, the kets of bra i
-        // in segment s, positions [first, jend), divide at jmid: from jmid on
-        // every ket's Schwarz bound times the bra's is below the threshold
-        // (a suffix maximum within the segment, found by binary search), so
-        // [jmid, jend) runs in FP32 and [first, jmid) in FP64.  Each task
-        // holds kets of one part; the FP64 tasks come first in the task list,
-        // and both launches of the class write the same slots.
-        std::vector<int> jmid(jend);
-        if (e.split) {
-            std::vector<double> smax(kl.size());
-            for (int s2 = 0; s2 < nkseg; ++s2)
-                for (long long j = segs[s2].second - 1; j >= segs[s2].first; --j)
-                    smax[j] = std::max(c->h_pair_sch[kl[j]], j + 1 < segs[s2].second ? smax[j + 1] : 0.0);
-            const double thr = c->opt.fp32_threshold;
-            for (size_t i = 0; i < bl.size(); ++i) {
-                const double bs = c->h_pair_sch[bl[i]];
-                for (int s2 = 0; s2 < nkseg; ++s2) {
-                    long long lo = segs[s2].first, hi = jend[i * nkseg + s2];
-                    while (lo < hi) {   // first t with bs * smax[t] < thr (non-increasing)
-                        const long long mid = (lo + hi) / 2;
-                        if (bs * smax[mid] < thr) hi = mid; else lo = mid + 1;
-                    }
-                    jmid[i * nkseg + s2] = (int)lo;
-                }
-            }
-        }
-        long long merged_kets = 0;
-        for (size_t i = 0; i < bl.size(); ++i)
-            for (int s = 0; s < nkseg; ++s)
-                if (whole(c->h_pair_npp[bl[i]], c->h_pair_npp[kl[segs[s].first]]))
-                    merged_kets += jend[i * nkseg + s] - segs[s].first;
-        // long tasks: about 40960 waves of work per launch, at least 256 and at
-        // most 8192 kets each (each task stores its K' rows once, so longer
-        // tasks store less; the costliest-first order keeps the tail short)
-        const long long target = std::max(1, 40960 * 64 / block);
-        const long long cmerge = std::max<long long>(std::max(chunk, 128),
-                                                     std::min<long long>(8192, (merged_kets + target - 1) / target));
-        std::vector<int4> tasks;
-        std::vector<KetRun> runs;
-        std::vector<long long> cost;   // estimated primitive quartets per task
-        long long cost_acc = 0;
-        auto push = [&](int bp, long long r0, int w, int len) {
-            if (r0 > INT32_MAX || (long long)runs.size() > INT32_MAX) return false;
-            tasks.push_back(make_int4(bp, (int)r0, (int)runs.size(), w));
-            cost.push_back(cost_acc);
-            cost_acc = 0;
-            e.task_len_max = std::max(e.task_len_max, len);
-            return true;
-        };
-        auto run_of = [&](long long x, long long y, long long jcd, int sg) {
-            KetRun r;
-            r.x = (int)x;
-            r.y = (int)y;
-            r.jcd = jcd;
-            r.soa = soa_base[sg] - segs[sg].first;
-            r.len = (int)(segs[sg].second - segs[sg].first);
-            r.npp = c->h_pair_npp[kl[segs[sg].first]];
-            return r;
-        };
-        // costliest tasks first within each part: workgroups start roughly in
-        // index order, so a launch ends on short tasks and its tail is short.
-        // A stable bucket sort on the cost's quarter-octave (4 log2 cost, from
-        // its two bits below the leading one), linear in the number of tasks
-        // (millions for the large clusters).
-        auto sort_costliest_first = [&](size_t t0) {
-            auto bucket = [](long long v) {
-                if (v <= 0) return 0;
-                const int b = 63 - __builtin_clzll((unsigned long long)v);
-                const int sub = b >= 2 ? (int)((v >> (b - 2)) & 3) : (int)((v << (2 - b)) & 3);
-                return 4 * b + sub;   // 0 .. 255
-            };
-            std::vector<size_t> start(257, 0);
-            for (size_t t = t0; t < tasks.size(); ++t) ++start[255 - bucket(cost[t]) + 1];
-            for (int b = 1; b <= 256; ++b) start[b] += start[b - 1];
-            std::vector<int4> sorted(tasks.size() - t0);
-            for (size_t t = t0; t < tasks.size(); ++t) sorted[start[255 - bucket(cost[t])]++] = tasks[t];
-            std::copy(sorted.begin(), sorted.end(), tasks.begin() + t0);
-        };
-        const int nparts = e.split ? 2 : 1;
-        for (int part = 0; part < nparts; ++part) {
-            const size_t part0 = tasks.size();
-            // the part's kets of bra i in segment s: [lo, hi) within [first, jend)
-            auto part_lo = [&](size_t i, int s) -> long long { return part == 0 ? segs[s].first : jmid[i * nkseg + s]; };
-            auto part_hi = [&](size_t i, int s) -> long long { return part == 0 ? jmid[i * nkseg + s] : jend[i * nkseg + s]; };
-            for (size_t i = 0; i < bl.size(); ++i) {
-                const int nbp = c->h_pair_npp[bl[i]];
-                // kets of the unsplit segments, as one list of runs cut every cmerge kets
-                long long r0 = (long long)runs.size();
-                int len = 0;
-                for (int s = 0; s < nkseg; ++s) {
-                    const auto& sg = segs[s];
-                    const int nkp = c->h_pair_npp[kl[sg.first]];
-                    if (!whole(nbp, nkp)) continue;
-                    long long x = part_lo(i, s);
-                    const long long x1 = part_hi(i, s);
-                    while (x < x1) {
-                        const long long take = std::min<long long>(x1 - x, cmerge - len);
-                        runs.push_back(run_of(x, x + take, jbase[i * nkseg + s] + (x - sg.first), s));
-                        cost_acc += take * nbp * nkp;
-                        x += take;
-                        len += (int)take;
-                        if (len == cmerge || (long long)runs.size() - r0 == HFK_TASK_RUNS) {
-                            if (!push(bl[i], r0, (nbp << 16) | 0xff, len)) return hfk_set_error(HFK_EUNSUP, "task list too long");
-                            r0 = (long long)runs.size();
-                            len = 0;
-                        }
-                    }
-                }
-                if (len > 0 && !push(bl[i], r0, (nbp << 16) | 0xff, len)) return hfk_set_error(HFK_EUNSUP, "task list too long");
-                if (nbp > 0) e.bra_stage = std::max(e.bra_stage, coop ? nbp : std::min(nbp, HFK_BRA_STAGE));
-                // segments split into primitive ranges: one run per task
-                for (int s = 0; s < nkseg; ++s) {
-                    const auto& sg = segs[s];
-                    const int nkp = c->h_pair_npp[kl[sg.first]];
-                    if (whole(nbp, nkp)) continue;
-                    // J'_cd entries cover the whole survivor range [k0, k1) of the
-                    // quartet pieces; the part's kets are [p0, p1)
-                    const long long k0 = sg.first, k1 = jend[i * nkseg + s];
-                    const long long p0 = part_lo(i, s), p1 = part_hi(i, s);
-                    if (p1 <= p0) continue;
-                    int bstep, kstep;
-                    steps(nbp, nkp, bstep, kstep);
-                    int piece = 0;   // pieces in (bra range, ket range) order, as npiece counts them
-                    for (int b0 = 0; b0 < nbp; b0 += bstep) {
-                        const int b1 = std::min(nbp, b0 + bstep);
-                        for (int q0 = 0; q0 < nkp; q0 += kstep, ++piece) {
-                            const int q1 = std::min(nkp, q0 + kstep);
-                            const int w = (b0 << 24) | (b1 << 16) | (q0 << 8) | q1;
-                            for (long long j0 = p0; j0 < p1; j0 += chunk) {
-                                const long long j1 = std::min<long long>(p1, j0 + chunk);
-                                const long long rr = (long long)runs.size();
-                                runs.push_back(run_of(j0, j1, jbase[i * nkseg + s] + (long long)piece * (k1 - k0) + (j0 - k0), s));
-                                cost_acc = (j1 - j0) * (long long)(b1 - b0) * (q1 - q0);
-                                if (!push(bl[i], rr, w, (int)(j1 - j0))) return hfk_set_error(HFK_EUNSUP, "task list too long");
-                            }
-                            e.bra_stage = std::max(e.bra_stage, b1 - b0);
-                        }
-                    }
-                }
-            }
-            if ((long long)kl.size() > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "pair class too long");
-            sort_costliest_first(part0);
-            if (part == 0) e.ntask64 = (long long)tasks.size();
-        }
-        e.ntask = (long long)tasks.size();
+        HFK_TRY(entry_upload(c, e, &e.d_colmap, colmap));
+        HFK_TRY(entry_upload(c, e, &e.d_collist, collist));
+        e.d_kets = c->d_class[e.kk];   // runs index the ket class list itself (not owned)
+        HFK_TRY(hfk_plan_entry_device(c, e));
         // LDS of one workgroup: the K' rows must fit; the bra shells' density
         // rows join them while the workgroups of four waves per SIMD still fit
         // in the 160 KB of a CU
@@ -881,176 +601,15 @@ int hfk_ctx_plan(hfk_ctx* c)
             e.drows = !coop && hfk_jk_lds_bytes(e.la, e.lb, e.ncols, e.bra_stage, 1) <= hfk_lds_budget(block) &&
                       !getenv("HFK_NO_DROWS");
         }
-        // reduction lists: the tasks of each bra shell (K' rows) and of each bra pair (J'_ab), in task order
-        std::vector<int> pos_of(c->npair, -1);
-        for (size_t i = 0; i < bl.size(); ++i) pos_of[bl[i]] = (int)i;
-        std::vector<int> koff(nshell + 1, 0), boff(bl.size() + 1, 0);
-        for (const int4& t : tasks) {
-            ++koff[c->h_pair_A[t.x] + 1];
-            ++koff[c->h_pair_B[t.x] + 1];
-            ++boff[pos_of[t.x] + 1];
-        }
-        for (int S = 0; S < nshell; ++S) koff[S + 1] += koff[S];
-        for (size_t i = 0; i < bl.size(); ++i) boff[i + 1] += boff[i];
-        std::vector<int2> kcsr(koff[nshell]);
-        std::vector<int> jab_slot(tasks.size());   // the tasks of one bra pair: consecutive J'_ab slots
-        {
-            std::vector<int> kfill(koff.begin(), koff.end() - 1), bfill(boff.begin(), boff.end() - 1);
-            for (size_t t = 0; t < tasks.size(); ++t) {
-                const int bp = tasks[t].x;
-                kcsr[kfill[c->h_pair_A[bp]]++] = make_int2((int)t, 0);
-                kcsr[kfill[c->h_pair_B[bp]]++] = make_int2((int)t, NA);
-                jab_slot[t] = bfill[pos_of[bp]]++;
-            }
-        }
-        if (e.ntask > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "too many tasks (%lld)", e.ntask);
-        e.d_kets = c->d_class[e.kk];   // runs index the ket class list itself (not owned)
-        if (!tasks.empty()) {
-            HFK_TRY(entry_upload(c, e, &e.d_tasks, tasks));
-            HFK_TRY(entry_upload(c, e, &e.d_runs, runs));
-        }
-        HFK_TRY(entry_upload(c, e, &e.d_colmap, colmap));
-        HFK_TRY(entry_upload(c, e, &e.d_collist, collist));
-        if (!coop) {
-            // the class's position-major records (once per class)
-            if (!c->d_ksoa[e.kk]) {
-                std::vector<long long> pos_soa(kl.size());
-                std::vector<int> pos_len(kl.size()), pos_npp(kl.size());
-                for (int sg = 0; sg < nkseg; ++sg)
-                    for (long long j = segs[sg].first; j < segs[sg].second; ++j) {
-                        pos_soa[j] = soa_base[sg] - segs[sg].first;
-                        pos_len[j] = (int)(segs[sg].second - segs[sg].first);
-                        pos_npp[j] = c->h_pair_npp[kl[j]];
-                    }
-                long long* d_ps = nullptr;
-                int *d_pl = nullptr, *d_pn = nullptr;
-                HFK_TRY(dev_upload(c, &d_ps, pos_soa));
-                HFK_TRY(dev_upload(c, &d_pl, pos_len));
-                HFK_TRY(dev_upload(c, &d_pn, pos_npp));
-                HFK_TRY(dev_alloc(&c->d_ksoa[e.kk], (size_t)std::max(1LL, soa_base[nkseg])));
-                hfk_launch_ket_soa(c->d_pp, c->d_qcomp, c->d_pair_pp0, c->d_class[e.kk], (int)kl.size(), d_ps, d_pl, d_pn,
-                                   NF, nket_comp, c->d_ksoa[e.kk], c->stream);
-                HFK_HIP(hipGetLastError());
-                dev_free(c, d_pn);
-                dev_free(c, d_pl);
-                dev_free(c, d_ps);
-            }
-            // per-position descriptors, screen values and component scales
-            const int W = hfk_kdesc_words(NC, ND);
-            const size_t nk = kl.size();
-            std::vector<int> kdesc((size_t)W * nk);
-            std::vector<double> kposq(2 * nk), kscale((size_t)(NC + ND) * nk);
-            bool unit = true;
-            for (size_t j = 0; j < nk; ++j) {
-                const int p = kl[j], C = c->h_pair_A[p], Dd = c->h_pair_B[p];
-                const HostShell &hc = c->shells[C], &hd = c->shells[Dd];
-                for (int x = 0; x < NC; ++x) {
-                    const int f = hc.bf[x] >= 0 ? hc.bf[x] : c->n;
-                    kdesc[(size_t)x * nk + j] = f;
-                    kdesc[(size_t)(NC + ND + x) * nk + j] = colmap[f];
-                    kscale[(size_t)x * nk + j] = hc.scale[x];
-                    unit = unit && hc.scale[x] == 1.0;
-                }
-                for (int x = 0; x < ND; ++x) {
-                    const int f = hd.bf[x] >= 0 ? hd.bf[x] : c->n;
-                    kdesc[(size_t)(NC + x) * nk + j] = f;
-                    kdesc[(size_t)(2 * NC + ND + x) * nk + j] = colmap[f];
-                    kscale[(size_t)(NC + x) * nk + j] = hd.scale[x];
-                    unit = unit && hd.scale[x] == 1.0;
-                }
-                kdesc[(size_t)(2 * (NC + ND)) * nk + j] = C == Dd ? 1 : 0;
-                kdesc[(size_t)(2 * (NC + ND) + 1) * nk + j] = hc.atom;
-                kdesc[(size_t)(2 * (NC + ND) + 2) * nk + j] = hd.atom;
-                kposq[j] = c->h_pair_qhi[p];
-                kposq[nk + j] = c->h_pair_sch[p];
-            }
-            HFK_TRY(entry_upload(c, e, &e.d_kdesc, kdesc));
-            HFK_TRY(entry_upload(c, e, &e.d_kpos_q, kposq));
-            if (!unit) HFK_TRY(entry_upload(c, e, &e.d_kscale, kscale));
-        }
-        HFK_TRY(entry_upload(c, e, &e.d_kcsr_off, koff));
-        HFK_TRY(entry_upload(c, e, &e.d_kcsr, kcsr));
-        // K reduction jobs: HFK_KSEG consecutive entries of one shell
-        // component row (first pass), and each row's jobs (second pass)
-        {
-            std::vector<int4> kjob, krow;
-            for (int S = 0; S < nshell; ++S) {
-                const int q0 = koff[S], q1 = koff[S + 1];
-                if (q1 <= q0) continue;
-                for (int x = 0; x < HFK_MAXCOMP; ++x) {
-                    if (c->shells[S].bf[x] < 0) continue;
-                    const int j0 = (int)kjob.size();
-                    for (int q = q0; q < q1; q += HFK_KSEG)
-                        kjob.push_back(make_int4(S * HFK_MAXCOMP + x, q, std::min(q1, q + HFK_KSEG), 0));
-                    krow.push_back(make_int4(S * HFK_MAXCOMP + x, j0, (int)kjob.size() - j0, 0));
-                }
-            }
-            e.nkjob = (int)kjob.size();
-            e.nkrow = (int)krow.size();
-            if (!kjob.empty()) {
-                HFK_TRY(entry_upload(c, e, &e.d_kjob, kjob));
-                HFK_TRY(entry_upload(c, e, &e.d_krow, krow));
-                HFK_TRY(entry_alloc(c, e, &e.d_kseg, kjob.size() * (size_t)e.ncols));
-            }
-        }
-        HFK_TRY(entry_upload(c, e, &e.d_bcsr_off, boff));
-        HFK_TRY(entry_upload(c, e, &e.d_jab_slot, jab_slot));
-        // J'_cd layout; the first reduction pass works on chunks of
-        // consecutive bras of one bra segment (enough chunks to fill the GPU
-        // for small launches, at most ~128 plus one per segment for large ones)
-        {
-            std::vector<int> pos_kseg(kl.size()), kfirst(nkseg + 1);
-            for (int s = 0; s < nkseg; ++s) {
-                kfirst[s] = (int)segs[s].first;
-                for (long long j = segs[s].first; j < segs[s].second; ++j) pos_kseg[j] = s;
-            }
-            kfirst[nkseg] = (int)kl.size();
-            const long long ch = std::max<long long>(16, ((long long)bl.size() + 127) / 128);
-            std::vector<int4> chunks;
-            for (int b = 0; b < nbseg; ++b)
-                for (long long i0 = bsegs[b].first; i0 < bsegs[b].second; i0 += ch)
-                    chunks.push_back(make_int4(b, (int)i0, (int)std::min<long long>(bsegs[b].second, i0 + ch), 0));
-            // tiles of ptile consecutive positions within one ket segment
-            const int ptile = std::max(1, 256 / nket_comp);
-            std::vector<int2> tiles;
-            for (int s = 0; s < nkseg; ++s)
-                for (long long t0 = segs[s].first; t0 < segs[s].second; t0 += ptile) tiles.push_back(make_int2(s, (int)t0));
-            JcdLayout& J = e.jl;
-            memset(&J, 0, sizeof J);
-            int2* d_tl = nullptr;
-            HFK_TRY(entry_upload(c, e, &d_tl, tiles));
-            J.tiles = d_tl;
-            J.ntile = (int)tiles.size();
-            J.ptile = ptile;
-            int *d_pk = nullptr, *d_kf = nullptr, *d_je = nullptr, *d_np = nullptr;
-            int4* d_ch = nullptr;
-            long long* d_jb = nullptr;
-            HFK_TRY(entry_upload(c, e, &d_pk, pos_kseg));
-            HFK_TRY(entry_upload(c, e, &d_kf, kfirst));
-            HFK_TRY(entry_upload(c, e, &d_ch, chunks));
-            HFK_TRY(entry_upload(c, e, &d_je, jend));
-            HFK_TRY(entry_upload(c, e, &d_jb, jbase));
-            HFK_TRY(entry_upload(c, e, &d_np, npiece));
-            J.pos_kseg = d_pk;
-            J.kseg_first = d_kf;
-            J.chunks = d_ch;
-            J.jend = d_je;
-            J.jbase = d_jb;
-            J.npiece = d_np;
-            J.nket = (int)kl.size();
-            J.nkseg = nkseg;
-            J.nchunk = (int)chunks.size();
-            J.nket_comp = nket_comp;
-            J.same = e.kb == e.kk;
-            HFK_TRY(entry_alloc(c, e, &J.part, std::max<size_t>(1, chunks.size() * kl.size() * (size_t)nket_comp)));
-        }
+        const long long nk = c->class_count[e.kk];
+        HFK_TRY(entry_alloc(c, e, &e.jl.part, std::max<size_t>(1, (size_t)e.jl.nchunk * nk * nket_comp)));
         // the slots
         HFK_TRY(entry_alloc(c, e, &e.d_kslot, std::max<size_t>(1, (size_t)e.ntask * nrow * e.ncols)));
         HFK_TRY(entry_alloc(c, e, &e.d_jab, std::max<size_t>(1, (size_t)e.ntask * nbra_comp)));
-        HFK_TRY(entry_alloc(c, e, &e.d_jcd, std::max<size_t>(1, (size_t)njcd * nket_comp)));
+        HFK_TRY(entry_alloc(c, e, &e.d_jcd, std::max<size_t>(1, (size_t)e.njcd * nket_comp)));
         e.jl.jcd = e.d_jcd;
         e.slot_bytes = sizeof(double) * ((size_t)e.ntask * (nrow * e.ncols + nbra_comp) + (size_t)e.nkjob * e.ncols +
-                                         ((size_t)njcd + (size_t)e.jl.nchunk * kl.size()) * nket_comp);
+                                         ((size_t)e.njcd + (size_t)e.jl.nchunk * nk) * nket_comp);
         c->slot_bytes += e.slot_bytes;
     }
     // the launches' partial J', K'

@@ -35,6 +35,12 @@ struct HostShell {
     double scale[HFK_MAXCOMP];  // coefficient of the component / base coefficient
 };
 
+// Segment of a pair-class list: positions [first, end), npp primitive pairs each.
+struct ClassSeg {
+    long long first, end;
+    int npp;
+};
+
 struct PlanEntry {
     int kb, kk;                 // bra / ket pair class
     int la, lb, lc, ld;
@@ -92,9 +98,10 @@ struct hfk_ctx {
     // pairs
     int npair = 0;
     std::vector<int> h_pair_A, h_pair_B, h_pair_npp;
-    std::vector<double> h_pair_qhi, h_pair_sch;
-    std::vector<int> h_class[HFK_NPC];
+    // pair classes (built on the device, hfk_plan.hip): their lengths and
+    // their segments (runs of one primitive-pair count)
     long long class_count[HFK_NPC] = {0};
+    std::vector<ClassSeg> class_segs[HFK_NPC];
     int* d_class[HFK_NPC] = {nullptr};
     // position-major copies of the classes' primitive-pair records, for the
     // classes that are kets of per-lane launches (KetRun.soa); built with the plan
@@ -218,6 +225,10 @@ void hfk_pair_class_l(int k, int* la, int* lb);
 QuartetArgs hfk_ctx_args(hfk_ctx* c);
 int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op);
 int hfk_ctx_plan(hfk_ctx* c);
+// hfk_plan.hip: the task and reduction lists of one plan entry, built on the
+// device (colmap / ncols set before); sqrt of n doubles on the device
+int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e);
+void hfk_launch_sqrt(int n, const double* in, double* out, hipStream_t s);
 void hfk_plan_args(hfk_ctx* c, const PlanEntry& e, QuartetArgs& a);
 int hfk_ctx_collect_times(hfk_ctx* c);
 // calcIntegrals into a host array of out_len = M(M+1)/2 (caller holds the lock)
