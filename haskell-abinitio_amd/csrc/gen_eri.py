@@ -281,30 +281,77 @@ def gen_class(la, lb, lc, ld):
     return "\n".join(out), (na, nb, nc, nd)
 
 
+def prod_sum(factors_by_axis, inner):
+    """sum over (t,u,v) of fx[t] fy[u] fz[v] inner(t,u,v), factorised per
+    axis; factors are lists of (index, expr) ("T(1)" is a unit factor),
+    inner returns an expression or None (no term).  None when empty."""
+    tx = []
+    for t, ex in factors_by_axis[0]:
+        ty = []
+        for u, ey in factors_by_axis[1]:
+            tz = []
+            for v, ez in factors_by_axis[2]:
+                val = inner(t, u, v)
+                if val is None:
+                    continue
+                tz.append(val if ez == "T(1)" else "%s*%s" % (ez, val))
+            if tz:
+                s = " + ".join(tz)
+                ty.append("(%s)" % s if ey == "T(1)" else "%s*(%s)" % (ey, s))
+        if ty:
+            s = " + ".join(ty)
+            tx.append("(%s)" % s if ex == "T(1)" else "%s*(%s)" % (ex, s))
+    return " + ".join(tx).replace("+ -", "- ") if tx else None
+
+
+# Classes whose gradient code forms R, W and X inside each branch of the
+# all-components test (shorter live ranges; measured faster for these), the
+# others form them once before the branch.
+GRAD_LAZY = {(1, 1, 1, 0)}
+
+
 def gen_grad_class(la, lb, lc, ld):
     """Derivative-integral contraction for the nuclear gradient of class
-    (la lb|lc ld): per primitive quartet, with the two-particle density
-    G[ab][cd] of the shell quartet,
-        gA_k += sum_abcd G (d/dA_k a, b|c d),  dA_k g_a = 2 alpha g_{a+1_k} - a_k g_{a-1_k}
-    and likewise B (exponent beta) and C (exponent gamma); D follows from
-    translational invariance.  McMurchie-Davidson: the shifted functions
-    enter through Hermite coefficients E^{i+-1,j}, R_tuv runs to L+1."""
+    (la lb|lc ld) over one bra pair and one ket pair, with the two-particle
+    density G[ab][cd] of the shell quartet.  Per primitive quartet
+    (McMurchie-Davidson, bra Omega_ab = K sum_t E^ab_t Lambda_t(P)):
+
+      dA:   2 alpha g_{a+1} - a g_{a-1}   (Hermite coefficients
+            DA^{ab}_t = 2 alpha E^{a+1,b}_t - a E^{a-1,b}_t, t <= a+b+1)
+      dA + dB = dP: E^ab_tuv with R shifted by one along the axis
+      dC = (gamma/q) dQ + d/dX_CD, dQ = -dP, and d/dX_CD of the ket's
+            coefficients dE^cd_t = 2 gamma E^{c+1,d}_t - c E^{c-1,d}_t
+            - (gamma/q) E^{cd}_{t-1} (t <= c+d)
+      dD = -(dA + dB + dC)   (translational invariance; formed by the caller)
+
+    The ket tables depend on the ket primitive only, so the d/dX_CD part is
+    applied once per ket primitive to the bra-loop sums of
+    Z^cd_tuv = sum_ab G X^ab_tuv.  Outputs g[0..2] = dA, g[3..5] = dB,
+    g[6..8] = dC (each contracted with G, summed over the primitive
+    quartets).  Spec of the derivative: Derivatives.hs:246-257."""
     L = la + lb + lc + ld
     ca, cb, cc, cd = comps(la), comps(lb), comps(lc), comps(ld)
     na, nb, nc, nd = len(ca), len(cb), len(cc), len(cd)
     nbra, nket = na * nb, nc * nd
+    lab, lcd = la + lb, lc + ld
+    multi = nket > 1 or nbra > 1
     name = "grad_%d%d%d%d" % (la, lb, lc, ld)
     out = []
+    out.append("// bra: nbp staged primitive-pair records (PP_K holds 2 pi^(5/2) K); bq: their")
+    out.append("// per-component screen values, bq[ib * %d + ab] (MODE 1)." % nbra)
     out.append("template <typename T, int BOYS, int MODE, typename KR>")
-    out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, int nbp, const KR& kr, int nkp," % name)
-    out.append("    const double* __restrict__ qcomp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
+    out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, const double* __restrict__ bq, int nbp," % name)
+    out.append("    const KR& kr, int nkp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
     out.append("{")
-    out.append("#pragma unroll")
-    out.append("  for (int z = 0; z < 9; ++z) g[z] = T(0);")
+    out.append("  T gA0 = T(0), gA1 = T(0), gA2 = T(0), gP0 = T(0), gP1 = T(0), gP2 = T(0);")
+    out.append("  T gC0 = T(0), gC1 = T(0), gC2 = T(0);")
     out.append("  for (int ik = 0; ik < nkp; ++ik) {")
     out.append("    const double* pk = kr.prim(ik);")
     em = Emitter()
-    em.raw("    const double kqlo = kr.at(pk, PP_QLO), kqhi = kr.at(pk, PP_QHI);")
+    if nket > 1:
+        em.raw("    const double kqlo = kr.at(pk, PP_QLO), kqhi = kr.at(pk, PP_QHI);")
+    else:
+        em.raw("    const double kqhi = kr.at(pk, PP_QHI), kqlo = kqhi;")
     em.raw("    const T kq = T(kr.at(pk, PP_P));")
     em.raw("    const T kQx = T(kr.at(pk, PP_PX)), kQy = T(kr.at(pk, PP_PY)), kQz = T(kr.at(pk, PP_PZ));")
     em.raw("    const T kCx = T(kr.at(pk, PP_AX)), kCy = T(kr.at(pk, PP_AY)), kCz = T(kr.at(pk, PP_AZ));")
@@ -312,53 +359,75 @@ def gen_grad_class(la, lb, lc, ld):
     em.raw("    const T kK = T(kr.at(pk, PP_K));")
     em.raw("    const T koo2q = T(kr.at(pk, PP_OO2P));")
     em.raw("    const T kgam2 = T(2) * T(kr.at(pk, PP_EA));")
-    if nket > 1 or nbra > 1:
+    em.raw("    const T kgoq = kgam2 * koo2q;   // gamma / q")
+    if multi:
         em.raw("    double kqv[%d];" % nket)
         em.raw("    if constexpr (MODE == 1) {")
-        em.raw("#pragma unroll")
-        em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = kr.q(pk, z);" % nket)
+        if nket > 1:
+            em.raw("#pragma unroll")
+            em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = kr.q(pk, z);" % nket)
+        else:
+            em.raw("      kqv[0] = kqhi;")
         em.raw("    }")
     em.raw("    if (MODE == 1 && kqhi * bra[PP_QHI] < qtau) break;   // records by descending QHI")
     Ek = [hermite_E(em, lc + 1, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
+
+    def Efac(E, i, j):
+        return [(t, E[(i, j, t)]) for t in range(i + j + 1)]
+
+    # ket d/dX_CD tables, per axis and (c, d) exponent pair
+    dEk = [dict() for _ in range(3)]
+    for x in range(3):
+        for c in cc:
+            for d in cd:
+                key = (c[x], d[x])
+                if key in dEk[x]:
+                    continue
+                ci, di = key
+                lst = []
+                for t in range(ci + di + 1):
+                    terms = []
+                    e = Ek[x][(ci + 1, di, t)]
+                    terms.append("kgam2*%s" % e if e != "T(1)" else "kgam2")
+                    if ci > 0 and t <= ci - 1 + di:
+                        e = Ek[x][(ci - 1, di, t)]
+                        terms.append("- T(%d)*%s" % (ci, e) if e != "T(1)" else "- T(%d)" % ci)
+                    if t >= 1:
+                        e = Ek[x][(ci, di, t - 1)]
+                        terms.append("- kgoq*%s" % e if e != "T(1)" else "- kgoq")
+                    lst.append((t, em.tmp(" ".join(terms), "dk")))
+                dEk[x][key] = lst
+    # bra-loop sums of Z^cd_tuv (tuv within the ket's Hermite box)
+    zbox = {}
+    for ic, c in enumerate(cc):
+        for id_, d in enumerate(cd):
+            kidx = ic * nd + id_
+            zbox[kidx] = [(t, u, v) for t in range(c[0] + d[0] + 1) for u in range(c[1] + d[1] + 1)
+                          for v in range(c[2] + d[2] + 1)]
+            for (t, u, v) in zbox[kidx]:
+                em.raw("    T za%d_%d%d%d = T(0);" % (kidx, t, u, v))
+    em.raw("    T sp0 = T(0), sp1 = T(0), sp2 = T(0);")
     out.extend(em.lines)
+
     out.append("    for (int ib = 0; ib < nbp; ++ib) {")
     out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
     out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
     out.append("      if (MODE == 1 && bqhi * kqhi < qtau) break;   // records by descending QHI")
+    out.append("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
+    out.append("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
     out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
     em2 = Emitter()
     em2.n = em.n
-    em2.raw("      const T bp = T(pb[PP_P]);")
-    em2.raw("      const T bPx = T(pb[PP_PX]), bPy = T(pb[PP_PY]), bPz = T(pb[PP_PZ]);")
     em2.raw("      const T bAx = T(pb[PP_AX]), bAy = T(pb[PP_AY]), bAz = T(pb[PP_AZ]);")
     em2.raw("      const T bBx = T(pb[PP_BX]), bBy = T(pb[PP_BY]), bBz = T(pb[PP_BZ]);")
     em2.raw("      const T bK = T(pb[PP_K]);")
     em2.raw("      const T boo2p = T(pb[PP_OO2P]);")
     em2.raw("      const T balp2 = T(2) * T(pb[PP_EA]);")
-    em2.raw("      const T bbet2 = T(2) * (T(pb[PP_P]) - T(pb[PP_EA]));")
-    if nket > 1 or nbra > 1:
-        em2.raw("      double bqv[%d];" % nbra)
-        em2.raw("      if constexpr (MODE == 1) {")
-        em2.raw("        const int bqoff = (int)pb[PP_QOFF];")
-        em2.raw("#pragma unroll")
-        em2.raw("        for (int z = 0; z < %d; ++z) bqv[z] = qcomp[bqoff + z];" % nbra)
-        em2.raw("      }")
-    Eb = [hermite_E(em2, la + 1, lb + 1, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
-    em2.raw("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
-    em2.raw("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
+    Eb = [hermite_E(em2, la + 1, lb, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
     em2.raw("      T F[%d];" % (L + 2))
     em2.raw("      boys_fast<T, %d, BOYS>(Targ, bt, F);" % (L + 1))
-    em2.raw("      const T pref = T(HFK_TWO_PI_2_5) * bK * kK * rs;")
+    em2.raw("      const T pref = bK * kK * rs;")
     em2.raw("      const T m2a = T(-2) * alpha;")
-    # masked density per component quartet
-    Gm = {}
-    for ab in range(nbra):
-        for cdi in range(nket):
-            k = ab * nket + cdi
-            if nket > 1 or nbra > 1:
-                Gm[(ab, cdi)] = em2.tmp("(MODE == 0 || allin || !(bqv[%d] * kqv[%d] < qtau)) ? G[%d] : T(0)" % (ab, cdi, k), "gm")
-            else:
-                Gm[(ab, cdi)] = "G[%d]" % k
     R = {}
     pw = "pref"
     for n in range(0, L + 2):
@@ -368,134 +437,164 @@ def gen_grad_class(la, lb, lc, ld):
             pw = em2.tmp("%s * m2a" % pw, "w")
             R[(n, 0, 0, 0)] = em2.tmp("%s * F[%d]" % (pw, n), "r")
 
-    def getR(n, t, u, v):
+    def getR(Rm, n, t, u, v):
+        """R^n_tuv, memoised in Rm (one memo per branch: a value formed
+        inside one branch is not visible in the other)."""
         if t < 0 or u < 0 or v < 0:
             return None
         key = (n, t, u, v)
-        if key in R:
-            return R[key]
+        if key in Rm:
+            return Rm[key]
         if t >= 1:
-            a = getR(n + 1, t - 2, u, v); b = getR(n + 1, t - 1, u, v); X = "Xpq"; c = t - 1
+            a_ = getR(Rm, n + 1, t - 2, u, v); b_ = getR(Rm, n + 1, t - 1, u, v); X = "Xpq"; c_ = t - 1
         elif u >= 1:
-            a = getR(n + 1, t, u - 2, v); b = getR(n + 1, t, u - 1, v); X = "Ypq"; c = u - 1
+            a_ = getR(Rm, n + 1, t, u - 2, v); b_ = getR(Rm, n + 1, t, u - 1, v); X = "Ypq"; c_ = u - 1
         else:
-            a = getR(n + 1, t, u, v - 2); b = getR(n + 1, t, u, v - 1); X = "Zpq"; c = v - 1
+            a_ = getR(Rm, n + 1, t, u, v - 2); b_ = getR(Rm, n + 1, t, u, v - 1); X = "Zpq"; c_ = v - 1
         terms = []
-        if a is not None and c != 0:
-            terms.append("T(%d)*%s" % (c, a))
-        terms.append("%s*%s" % (X, b))
-        R[key] = em2.tmp(" + ".join(terms), "r")
-        return R[key]
+        if a_ is not None and c_ != 0:
+            terms.append("T(%d)*%s" % (c_, a_))
+        terms.append("%s*%s" % (X, b_))
+        Rm[key] = em2.tmp(" + ".join(terms), "r")
+        return Rm[key]
 
-    def prod_sum(factors_by_axis, inner):
-        """sum over (t,u,v) of fx[t] fy[u] fz[v] inner(t,u,v); factors are lists of (index, expr)."""
-        tx = []
-        for t, ex in factors_by_axis[0]:
-            ty = []
-            for u, ey in factors_by_axis[1]:
-                tz = []
-                for v, ez in factors_by_axis[2]:
-                    val = inner(t, u, v)
-                    if val is None:
-                        continue
-                    tz.append(val if ez == "T(1)" else "%s*%s" % (ez, val))
-                if tz:
-                    s = " + ".join(tz)
-                    ty.append("(%s)" % s if ey == "T(1)" else "%s*(%s)" % (ey, s))
-            if ty:
-                s = " + ".join(ty)
-                tx.append("(%s)" % s if ex == "T(1)" else "%s*(%s)" % (ex, s))
-        return " + ".join(tx) if tx else None
+    # W^{cd}_{tuv} = sum_{tau nu phi} (-1)^{tau+nu+phi} E^{cd}_{tau nu phi} R_{t+tau,u+nu,v+phi}
+    def getW(Rm, Wc, kidx, t, u, v):
+        key = (kidx, t, u, v)
+        if key in Wc:
+            return Wc[key]
+        c, d = cc[kidx // nd], cd[kidx % nd]
+        fac = [Efac(Ek[x], c[x], d[x]) for x in range(3)]
 
-    def Efac(E, i, j, sign=False):
-        return [(t, E[(i, j, t)]) for t in range(i + j + 1)]
+        def inner(tau, nu, phi):
+            r = getR(Rm, 0, t + tau, u + nu, v + phi)
+            return ("-%s" % r) if (tau + nu + phi) % 2 else r
+        Wc[key] = em2.tmp(prod_sum(fac, inner), "W")
+        return Wc[key]
 
-    def dEfac(E, i, j, w, shift_first):
-        """2w E^{i+1,j} - i E^{i-1,j} (shift_first) or 2w E^{i,j+1} - j E^{i,j-1}."""
-        outl = []
-        if shift_first:
-            up, dn, m = (i + 1, j), (i - 1, j), i
-        else:
-            up, dn, m = (i, j + 1), (i, j - 1), j
-        for t in range(up[0] + up[1] + 1):
-            terms = ["%s*%s" % (w, E[(up[0], up[1], t)]) if E[(up[0], up[1], t)] != "T(1)" else w]
-            if m > 0 and t <= dn[0] + dn[1]:
-                e = E[(dn[0], dn[1], t)]
-                terms.append("T(%d)*%s" % (m, e) if e != "T(1)" else "T(%d)" % m)
-                expr = "(%s - %s)" % (terms[0], terms[1])
-            else:
-                expr = terms[0]
-            outl.append((t, em2.tmp(expr, "de")))
-        return outl
+    # bra dA tables per axis and (a, b) exponent pair: t <= a+b+1
+    DA = [dict() for _ in range(3)]
+    for x in range(3):
+        for a in ca:
+            for b in cb:
+                key = (a[x], b[x])
+                if key in DA[x]:
+                    continue
+                ai, bi = key
+                lst = []
+                for t in range(ai + bi + 2):
+                    e = Eb[x][(ai + 1, bi, t)]
+                    terms = ["balp2*%s" % e if e != "T(1)" else "balp2"]
+                    if ai > 0 and t <= ai - 1 + bi:
+                        e = Eb[x][(ai - 1, bi, t)]
+                        terms.append("- T(%d)*%s" % (ai, e) if e != "T(1)" else "- T(%d)" % ai)
+                    lst.append((t, em2.tmp(" ".join(terms), "da")))
+                DA[x][key] = lst
+    # X^{ab}_{tau nu phi} = sum_tuv E^{ab}_tuv R_{t+tau,...}, over the union of the ket boxes
+    kunion = sorted(set(k for b_ in zbox.values() for k in b_))
 
-    lab, lcd = la + lb, lc + ld
-    # W^{cd}_{tuv} = sum (-1)^{tau+nu+phi} E^{cd} R_{t+tau,...}, t+u+v <= lab+1
-    btuv = [(t, u, v) for t in range(lab + 2) for u in range(lab + 2 - t) for v in range(lab + 2 - t - u)]
-    W = {}
-    for ic, c in enumerate(cc):
-        for id_, d in enumerate(cd):
-            kidx = ic * nd + id_
-            for (t, u, v) in btuv:
-                fac = [[(tau, Ek[x][(c[x], d[x], tau)]) for tau in range(c[x] + d[x] + 1)] for x in range(3)]
-                def inner(tau, nu, phi, t=t, u=u, v=v):
-                    r = getR(0, t + tau, u + nu, v + phi)
-                    return ("-%s" % r) if (tau + nu + phi) % 2 else r
-                val = prod_sum(fac, inner)
-                W[(kidx, t, u, v)] = em2.tmp(val.replace("+ -", "- "), "W")
-    # Y^{ab}_{tuv} = sum_cd Gm[ab][cd] W^{cd}_tuv, then bra derivatives
+    def getX(Rm, Xv, bidx, tau, nu, phi):
+        key = (bidx, tau, nu, phi)
+        if key not in Xv:
+            a, b = ca[bidx // nb], cb[bidx % nb]
+            fac = [Efac(Eb[x], a[x], b[x]) for x in range(3)]
+            Xv[key] = em2.tmp(prod_sum(fac, lambda t, u, v: getR(Rm, 0, t + tau, u + nu, v + phi)), "X")
+        return Xv[key]
+    # Y index sets; W and X are formed where first used inside each branch,
+    # which keeps their live ranges short
+    ysets = {}
     for ia, a in enumerate(ca):
         for ib_, b in enumerate(cb):
             bidx = ia * nb + ib_
-            Y = {}
-            for (t, u, v) in btuv:
-                if t > a[0] + b[0] + 1 or u > a[1] + b[1] + 1 or v > a[2] + b[2] + 1:
-                    continue
-                terms = ["%s*%s" % (Gm[(bidx, k)], W[(k, t, u, v)]) for k in range(nket)]
-                Y[(t, u, v)] = em2.tmp(" + ".join(terms), "Y")
-            for k in range(3):   # direction
-                for which, w, base in ((0, "balp2", 0), (1, "bbet2", 3)):
-                    fac = []
-                    for x in range(3):
-                        if x == k:
-                            fac.append(dEfac(Eb[x], a[x], b[x], w, which == 0))
-                        else:
-                            fac.append(Efac(Eb[x], a[x], b[x]))
+            box = [a[x] + b[x] for x in range(3)]
+            s = []
+            for t in range(box[0] + 2):
+                for u in range(box[1] + 2):
+                    for v in range(box[2] + 2):
+                        over = max(0, t - box[0]) + max(0, u - box[1]) + max(0, v - box[2])
+                        if over <= 1:
+                            s.append((t, u, v))
+            ysets[bidx] = s
+
+    lazy = (la, lb, lc, ld) in GRAD_LAZY
+    shared = (R, {}, {})
+    if not lazy:
+        for bidx in range(nbra):
+            for (t, u, v) in ysets[bidx]:
+                for k in range(nket):
+                    getW(R, shared[1], k, t, u, v)
+            for (t, u, v) in kunion:
+                getX(R, shared[2], bidx, t, u, v)
+
+    def body(Gname, ind):
+        """Y, the dA and dP sums and the Z sums with density names Gname(ab, cd)."""
+        Rm, Wc, Xv = (dict(R), {}, {}) if lazy else shared
+        for ia, a in enumerate(ca):
+            for ib_, b in enumerate(cb):
+                bidx = ia * nb + ib_
+                Y = {}
+                for (t, u, v) in ysets[bidx]:
+                    terms = ["%s*%s" % (Gname(bidx, k), getW(Rm, Wc, k, t, u, v)) for k in range(nket)]
+                    Y[(t, u, v)] = em2.tmp(" + ".join(terms), "Y")
+                for k in range(3):
+                    fac = [DA[x][(a[x], b[x])] if x == k else Efac(Eb[x], a[x], b[x]) for x in range(3)]
                     val = prod_sum(fac, lambda t, u, v: Y.get((t, u, v)))
                     if val:
-                        em2.raw("      g[%d] += %s;" % (base + k, val))
-    # ket derivatives: X^{ab}_{tnp} = sum_tuv E^{ab}_tuv R_{t+tau,...}, tau+nu+phi <= lcd+1
-    ktuv = [(t, u, v) for t in range(lcd + 2) for u in range(lcd + 2 - t) for v in range(lcd + 2 - t - u)]
-    Xv = {}
-    for ia, a in enumerate(ca):
-        for ib_, b in enumerate(cb):
-            bidx = ia * nb + ib_
-            for (tau, nu, phi) in ktuv:
-                fac = [Efac(Eb[x], a[x], b[x]) for x in range(3)]
-                val = prod_sum(fac, lambda t, u, v, tau=tau, nu=nu, phi=phi: getR(0, t + tau, u + nu, v + phi))
-                Xv[(bidx, tau, nu, phi)] = em2.tmp(val, "X")
-    for ic, c in enumerate(cc):
-        for id_, d in enumerate(cd):
-            kidx = ic * nd + id_
-            Z = {}
-            for (tau, nu, phi) in ktuv:
-                if tau > c[0] + d[0] + 1 or nu > c[1] + d[1] + 1 or phi > c[2] + d[2] + 1:
-                    continue
-                terms = ["%s*%s" % (Gm[(bb, kidx)], Xv[(bb, tau, nu, phi)]) for bb in range(nbra)]
-                z = em2.tmp(" + ".join(terms), "Z")
-                Z[(tau, nu, phi)] = ("-%s" % z) if (tau + nu + phi) % 2 else z
-            for k in range(3):
-                fac = []
-                for x in range(3):
-                    if x == k:
-                        fac.append(dEfac(Ek[x], c[x], d[x], "kgam2", True))
-                    else:
-                        fac.append(Efac(Ek[x], c[x], d[x]))
-                val = prod_sum(fac, lambda t, u, v: Z.get((t, u, v)))
-                if val:
-                    em2.raw("      g[%d] += %s;" % (6 + k, val.replace("+ -", "- ")))
+                        em2.raw("%sgA%d += %s;" % (ind, k, val))
+                    fac = [Efac(Eb[x], a[x], b[x]) for x in range(3)]
+                    sh = [1 if x == k else 0 for x in range(3)]
+                    val = prod_sum(fac, lambda t, u, v: Y.get((t + sh[0], u + sh[1], v + sh[2])))
+                    if val:
+                        em2.raw("%ssp%d += %s;" % (ind, k, val))
+        for kidx in range(nket):
+            for (t, u, v) in zbox[kidx]:
+                terms = ["%s*%s" % (Gname(bb, kidx), getX(Rm, Xv, bb, t, u, v)) for bb in range(nbra)]
+                em2.raw("%sza%d_%d%d%d += %s;" % (ind, kidx, t, u, v, " + ".join(terms)))
+
+    if multi:
+        em2.raw("      double bqv[%d];" % nbra)
+        em2.raw("      if constexpr (MODE == 1) {")
+        em2.raw("#pragma unroll")
+        em2.raw("        for (int z = 0; z < %d; ++z) bqv[z] = bq[ib * %d + z];" % (nbra, nbra))
+        em2.raw("      }")
+        # every active lane keeps every component: plain density (a uniform
+        # branch); otherwise the per-component selects
+        em2.raw("      if (MODE == 0 || __all(allin)) {")
+        body(lambda ab, k: "T(G[%d])" % (ab * nket + k), "        ")
+        em2.raw("      } else {")
+        Gm = {}
+        for ab in range(nbra):
+            for k in range(nket):
+                Gm[(ab, k)] = em2.tmp("(allin || !(bqv[%d] * kqv[%d] < qtau)) ? T(G[%d]) : T(0)" % (ab, k, ab * nket + k), "gm")
+        body(lambda ab, k: Gm[(ab, k)], "        ")
+        em2.raw("      }")
+    else:
+        body(lambda ab, k: "T(G[0])", "      ")
     out.extend(em2.lines)
     out.append("    }")
+    # per ket primitive: dP and the d/dX_CD part of dC
+    em3 = Emitter()
+    em3.n = em2.n
+    for k in range(3):
+        terms = []
+        for kidx in range(nket):
+            c, d = cc[kidx // nd], cd[kidx % nd]
+            fac = [dEk[x][(c[x], d[x])] if x == k else Efac(Ek[x], c[x], d[x]) for x in range(3)]
+
+            def inner(t, u, v, kidx=kidx):
+                z = "za%d_%d%d%d" % (kidx, t, u, v)
+                return ("-%s" % z) if (t + u + v) % 2 else z
+            val = prod_sum(fac, inner)
+            if val:
+                terms.append(val)
+        scd = " + ".join(terms).replace("+ -", "- ") if terms else "T(0)"
+        em3.raw("    gP%d += sp%d;" % (k, k))
+        em3.raw("    gC%d += (%s) - kgoq * sp%d;" % (k, scd, k))
+    out.extend(em3.lines)
     out.append("  }")
+    out.append("  g[0] = gA0; g[1] = gA1; g[2] = gA2;")
+    out.append("  g[3] = gP0 - gA0; g[4] = gP1 - gA1; g[5] = gP2 - gA2;")
+    out.append("  g[6] = gC0; g[7] = gC1; g[8] = gC2;")
     out.append("}")
     return "\n".join(out)
 
@@ -559,9 +658,9 @@ def main():
         if not grad_coop:
             body.append(gen_grad_class(*q))
             body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, typename KR>")
-            body.append("__device__ __forceinline__ void grad_class(const double* __restrict__ bra, int nbp, const KR& kr, int nkp,")
-            body.append("    const double* __restrict__ qcomp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
-            body.append("{ grad_%s<T, BOYS, MODE>(bra, nbp, kr, nkp, qcomp, bt, G, qtau, g); }" % tag)
+            body.append("__device__ __forceinline__ void grad_class(const double* __restrict__ bra, const double* __restrict__ bq, int nbp,")
+            body.append("    const KR& kr, int nkp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
+            body.append("{ grad_%s<T, BOYS, MODE>(bra, bq, nbp, kr, nkp, bt, G, qtau, g); }" % tag)
         body.append("}  // namespace")
         body.append('#include "hfk_eri_impl.hpp"')
         if grad_coop:

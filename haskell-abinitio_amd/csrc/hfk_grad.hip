@@ -86,7 +86,7 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
     const int n = c->n, natom = (int)c->Z.size();
     if (natom == 0 || (int)c->atom_of_bf.size() != n)
         return hfk_set_error(HFK_EINVAL, "hfk_gradient: basis needs natom, Z, R and atom_of_bf");
-    if (hfk_grad_lds_bytes(0, 0, 0, natom, 0) > 64 * 1024)
+    if (hfk_grad_lds_bytes(0, 0, 0, natom, 0, 0) > 64 * 1024)
         return hfk_set_error(HFK_EUNSUP, "hfk_gradient: %d atoms exceed the per-workgroup accumulator", natom);
     HFK_TRY(grad_upload_maps(c));
     const size_t nf = (size_t)n * (n + 1) / 2;
@@ -123,7 +123,12 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
         hfk_plan_args(c, e, a);   // the whole task list (both precision parts) in FP64
         // the bra shells' density rows in LDS while four waves per SIMD still fit
         a.drows = !hfk_class_grad_coop(e.la, e.lb, e.lc, e.ld) &&
-                  hfk_grad_lds_bytes(e.la, e.lb, e.ncols, natom, 1) <= hfk_lds_budget(a.block);
+                  hfk_grad_lds_bytes(e.la, e.lb, e.ncols, natom, 1, e.bra_stage) <= hfk_lds_budget(a.block);
+        if (!hfk_class_grad_coop(e.la, e.lb, e.lc, e.ld) &&
+            hfk_grad_lds_bytes(e.la, e.lb, e.ncols, natom, 0, e.bra_stage) > 64 * 1024) {
+            rc = hfk_set_error(HFK_EUNSUP, "hfk_gradient: %d atoms exceed the per-workgroup accumulator", natom);
+            break;
+        }
         hipStream_t s = conc ? c->aux[slot++ % hfk_ctx::NAUX] : c->stream;
         HFK_HIP(hipEventRecord(c->events[4 * np + 2 * i], s));
         if (hfk_launch_grad(e.la, e.lb, e.lc, e.ld, c->opt.boys_mode, e.mode, a, s)) {

@@ -20,7 +20,7 @@ namespace {
 template <int LA, int LB, int LC, int LD>
 constexpr int grad_min_waves()
 {
-    return (LA + LB + LC + LD == 2) ? 2 : 1;
+    return (LA + LB + LC + LD <= 1) ? 3 : ((LA + LB + LC + LD == 2) ? 2 : 1);
 }
 
 template <int LA, int LB, int LC, int LD, int BOYS, int MODE>
@@ -41,7 +41,7 @@ k_grad_tiles(QuartetArgs a)
     const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & (WAVE - 1);
     const bool lds = a.grad_lds != 0, drows = a.drows != 0;
     // LDS: [natom][3] accumulator (grad_lds), [NR][ldk] density rows of the
-    // bra shells (drows), the task's runs
+    // bra shells (drows), the task's runs, the staged bra primitive pairs
     double* gacc = gsm;
     double* drow = gacc + (lds ? 3 * a.natom : 0);
     long long* rsoa = (long long*)(drow + (drows ? NR * ldk : 0));
@@ -49,6 +49,7 @@ k_grad_tiles(QuartetArgs a)
     int* rstart = rx + HFK_TASK_RUNS + 1;
     int* rlen = rstart + HFK_TASK_RUNS + 1;
     int* rnpp = rlen + HFK_TASK_RUNS + 1;
+    double* sbra = (double*)(rnpp + HFK_TASK_RUNS + 1);
     double* __restrict__ acc = lds ? gacc : a.grad;
     if (lds)
         for (int t = tid; t < 3 * a.natom; t += nthr) gacc[t] = 0.0;
@@ -82,6 +83,22 @@ k_grad_tiles(QuartetArgs a)
             for (int col = tid; col < ldk; col += nthr) drow[r * ldk + col] = col < ncols ? Drow[a.collist[col]] : 0.0;
         }
     }
+    // the task's bra primitive-pair records and their per-component screen
+    // values, read by every lane for every ket primitive: staged once, the K
+    // field premultiplied by 2 pi^(5/2)
+    const int nbp = bprim1 - bprim0;   // ranges are explicit: 0 <= b0 < b1 <= npp
+    double* sbq = sbra + nbp * PP_STRIDE;
+    {
+        const double* gp = a.pp + (size_t)(a.pair_pp0[bp] + bprim0) * PP_STRIDE;
+        for (int t = tid; t < nbp * PP_STRIDE; t += nthr) {
+            const double v = gp[t];
+            sbra[t] = (t & (PP_STRIDE - 1)) == PP_K ? HFK_TWO_PI_2_5 * v : v;
+        }
+        if constexpr (MODE == 1) {
+            const double* q = a.qcomp + (long long)gp[PP_QOFF];
+            for (int t = tid; t < nbp * NBRA; t += nthr) sbq[t] = q[t];
+        }
+    }
     __syncthreads();
 
     double Dab[NBRA];
@@ -89,8 +106,6 @@ k_grad_tiles(QuartetArgs a)
     for (int x = 0; x < NA; ++x)
 #pragma unroll
         for (int y = 0; y < NB; ++y) Dab[x * NB + y] = D[(size_t)ia[x] * ld + ib[y]];
-    const double* bra = a.pp + (size_t)(a.pair_pp0[bp] + bprim0) * PP_STRIDE;
-    const int nbp = bprim1 - bprim0;   // ranges are explicit: 0 <= b0 < b1 <= npp
     const int* __restrict__ kd = a.kdesc;
     const int nket = (int)a.nket;
     const bool kscl = a.kscale != nullptr;
@@ -154,8 +169,8 @@ k_grad_tiles(QuartetArgs a)
         const int slen = rlen[r];
         const KetSoA<NF> kr{a.ksoa + rsoa[r] + pos + (size_t)kprim0 * NF * slen, slen};
         double g[9];
-        grad_class<LA, LB, LC, LD, double, BOYS, MODE>(bra, nbp, kr, min(kprim1, rnpp[r]) - kprim0, a.qcomp, a.bt, G,
-                                                       a.qtau, g);
+        grad_class<LA, LB, LC, LD, double, BOYS, MODE>(sbra, sbq, nbp, kr, min(kprim1, rnpp[r]) - kprim0, a.bt, G, a.qtau,
+                                                       g);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             gA[k] += g[k];
@@ -194,7 +209,7 @@ template <int LA, int LB, int LC, int LD, int BOYS, int MODE>
 void launch_grad_tiles(const QuartetArgs& a, hipStream_t s)
 {
     if (a.ntask <= 0) return;
-    const size_t shm = hfk_grad_lds_bytes(LA, LB, a.ncols, a.grad_lds ? a.natom : 0, a.drows);
+    const size_t shm = hfk_grad_lds_bytes(LA, LB, a.ncols, a.grad_lds ? a.natom : 0, a.drows, a.bra_stage);
     long long done = 0;
     while (done < a.ntask) {
         const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);

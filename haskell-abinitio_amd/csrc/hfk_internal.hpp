@@ -138,11 +138,13 @@ inline size_t hfk_jk_lds_bytes(int la, int lb, int ncols, int bra_stage, int dro
            hfk_run_stage_bytes() + hfk_bra_stage_bytes(la, lb, bra_stage);
 }
 // LDS bytes of a per-lane gradient workgroup: the per-atom accumulator, the
-// bra shells' density rows (drows), the task's runs.
-inline size_t hfk_grad_lds_bytes(int la, int lb, int ncols, int natom, int drows)
+// bra shells' density rows (drows), the task's runs, the staged bra
+// primitive pairs.
+inline size_t hfk_grad_lds_bytes(int la, int lb, int ncols, int natom, int drows, int bra_stage)
 {
     const int na = (la + 1) * (la + 2) / 2, nb = (lb + 1) * (lb + 2) / 2;
-    return sizeof(double) * (3 * (size_t)natom + (drows ? (size_t)(na + nb) * (ncols + 1) : 0)) + hfk_run_stage_bytes();
+    return sizeof(double) * (3 * (size_t)natom + (drows ? (size_t)(na + nb) * (ncols + 1) : 0)) + hfk_run_stage_bytes() +
+           hfk_bra_stage_bytes(la, lb, bra_stage);
 }
 // Dynamic LDS of a cooperative J/K workgroup (its static tables come on top).
 inline size_t hfk_coop_jk_lds_bytes(int la, int lb, int ncols)
