@@ -156,6 +156,9 @@ struct CoopLds {
     double R[SH::S][2][SH::NH];         // Hermite integrals, two levels
     double kq[SH::S][SH::NKET];         // ket per-component screen values
     double jcd[SH::S][SH::NKET];        // J'_cd partial sums
+    // gradient: the ket's X_CD-derivative coefficients, dE^{cd}_t = 2 gamma E^{c+1,d}_t
+    // - c E^{c-1,d}_t - (gamma/q) E^{cd}_{t-1}, indexed like ec
+    double dc[SH::G ? SH::S : 1][3][SH::G ? SH::EC_SZ : 1];
 };
 
 // Stage the Hermite tables and scalars of bra primitive pairs [bc0, bc0+nbc).
@@ -239,6 +242,25 @@ __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopSha
                 const double* r = ket0 + (size_t)ik * PP_STRIDE;
                 if (li < 3)
                     herm_table<SH::EC_I - 1, SH::EC_J - 1>(r[PP_AX + li], r[PP_BX + li], r[PP_OO2P], &sm.ec[slot][li][0]);
+                if constexpr (OP == HFK_OP_GRAD) {
+                    if (li < 3) {
+                        const double* E = &sm.ec[slot][li][0];
+                        double* dE = &sm.dc[slot][li][0];
+                        const double gam2 = 2.0 * r[PP_EA], goq = r[PP_EA] / r[PP_P];
+                        for (int i = 0; i <= LC; ++i)
+                            for (int j = 0; j <= LD; ++j)
+                                for (int t = 0; t < SH::EC_T; ++t) {
+                                    const int o = (i * SH::EC_J + j) * SH::EC_T;
+                                    double v = 0.0;
+                                    if (t <= i + j) {
+                                        v = gam2 * E[o + SH::EC_J * SH::EC_T + t];
+                                        if (i > 0) v -= (double)i * E[o - SH::EC_J * SH::EC_T + t];
+                                        if (t >= 1) v -= goq * E[o + t - 1];
+                                    }
+                                    dE[o + t] = v;
+                                }
+                    }
+                }
                 if (li == 3) {
                     sm.ks[slot][0] = r[PP_PX];
                     sm.ks[slot][1] = r[PP_PY];
@@ -288,7 +310,7 @@ __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopSha
                         for (int x = 0; x < 3; ++x)
 #pragma unroll
                             for (int t = 0; t < LE; ++t) e1[x][t] = (t < SH::EB_T) ? EB(x, ea[x], eb[x], t) : 0.0;
-                        // X_{tau nu phi} = sum_tuv E^{ab}_tuv R_{t+tau,u+nu,v+phi}, tau+nu+phi <= LCD+G
+                        // X_{tau nu phi} = sum_tuv E^{ab}_tuv R_{t+tau,u+nu,v+phi}, tau+nu+phi <= LCD
                         double Xv[SH::NX];
 #pragma unroll
                         for (int e = 0; e < SH::NX; ++e) Xv[e] = 0.0;
@@ -300,11 +322,11 @@ __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopSha
                                 for (int v = 0; v <= LAB - t - u; ++v) {
                                     const double c = (e1[0][t] * e1[1][u]) * e1[2][v];
 #pragma unroll
-                                    for (int tau = 0; tau <= LCD + G; ++tau)
+                                    for (int tau = 0; tau <= LCD; ++tau)
 #pragma unroll
-                                        for (int nu = 0; nu <= LCD + G - tau; ++nu)
+                                        for (int nu = 0; nu <= LCD - tau; ++nu)
 #pragma unroll
-                                            for (int ph = 0; ph <= LCD + G - tau - nu; ++ph)
+                                            for (int ph = 0; ph <= LCD - tau - nu; ++ph)
                                                 Xv[hidx(tau, nu, ph)] += c * R[hidx(t + tau, u + nu, v + ph)];
                                 }
                         double bq = 0.0;
@@ -331,10 +353,23 @@ __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopSha
                                 else V[cd] += s;
                             }
                         } else {
-                            double Zv[SH::NZ];
+                            // dA from the shifted bra coefficients; dA + dB = dP, the contraction
+                            // with R shifted by one along the axis; dC = -(gamma/q) dP plus the
+                            // ket's X_CD-derivative coefficients (dc); dD by translational
+                            // invariance.  Zv: the ket coefficients contracted with the density,
+                            // Zd[k]: the same with the derivative along k.
+                            double Zv[SH::NZ], Zd[3][SH::NZ];
 #pragma unroll
-                            for (int e = 0; e < SH::NZ; ++e) Zv[e] = 0.0;
-                            const double gam2 = 2.0 * sm.ks[slot][5];
+                            for (int e = 0; e < SH::NZ; ++e) {
+                                Zv[e] = 0.0;
+                                Zd[0][e] = 0.0;
+                                Zd[1][e] = 0.0;
+                                Zd[2][e] = 0.0;
+                            }
+                            const double* Dc = &sm.dc[slot][0][0];
+                            auto DC = [&](int axis, int i, int j, int t) -> double {
+                                return Dc[axis * SH::EC_SZ + (i * SH::EC_J + j) * SH::EC_T + t];
+                            };
 #pragma unroll
                             for (int cd = 0; cd < NKET; ++cd) {
                                 const int c = cd / ND, d = cd - (cd / ND) * ND;
@@ -342,42 +377,36 @@ __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopSha
                                 const int dd[3] = {cart_e(LD, d, 0), cart_e(LD, d, 1), cart_e(LD, d, 2)};
                                 double w = Gw[cd];
                                 if constexpr (MODE == 1) w = !(bq * sm.kq[slot][cd] < a.qtau) ? w : 0.0;
+                                double e[3][LCD + 1], de[3][LCD + 1];
+#pragma unroll
+                                for (int x = 0; x < 3; ++x)
+#pragma unroll
+                                    for (int t = 0; t <= LCD; ++t) {
+                                        e[x][t] = t <= cc[x] + dd[x] ? EC(x, cc[x], dd[x], t) : 0.0;
+                                        de[x][t] = t <= cc[x] + dd[x] ? DC(x, cc[x], dd[x], t) : 0.0;
+                                    }
 #pragma unroll
                                 for (int tau = 0; tau <= cc[0] + dd[0]; ++tau)
 #pragma unroll
                                     for (int nu = 0; nu <= cc[1] + dd[1]; ++nu)
 #pragma unroll
                                         for (int ph = 0; ph <= cc[2] + dd[2]; ++ph) {
-                                            const double e = EC(0, cc[0], dd[0], tau) * EC(1, cc[1], dd[1], nu) * EC(2, cc[2], dd[2], ph);
-                                            Zv[hidx(tau, nu, ph)] += ((tau + nu + ph) & 1) ? -w * e : w * e;
+                                            const double ws = ((tau + nu + ph) & 1) ? -w : w;
+                                            const int h = hidx(tau, nu, ph);
+                                            const double eyz = e[1][nu] * e[2][ph], exz = e[0][tau] * e[2][ph];
+                                            const double exy = e[0][tau] * e[1][nu];
+                                            Zv[h] += ws * (e[0][tau] * eyz);
+                                            Zd[0][h] += ws * (de[0][tau] * eyz);
+                                            Zd[1][h] += ws * (de[1][nu] * exz);
+                                            Zd[2][h] += ws * (de[2][ph] * exy);
                                         }
+                            }
+                            double scd[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-                                for (int k = 0; k < 3; ++k) {
-                                    double sacc = 0.0;
-#pragma unroll
-                                    for (int tau = 0; tau <= cc[0] + dd[0] + (k == 0); ++tau)
-#pragma unroll
-                                        for (int nu = 0; nu <= cc[1] + dd[1] + (k == 1); ++nu)
-#pragma unroll
-                                            for (int ph = 0; ph <= cc[2] + dd[2] + (k == 2); ++ph) {
-                                                const int idx[3] = {tau, nu, ph};
-                                                double f = 1.0;
-#pragma unroll
-                                                for (int x = 0; x < 3; ++x) {
-                                                    if (x == k) {
-                                                        double dv = gam2 * EC(x, cc[x] + 1, dd[x], idx[x]);
-                                                        if (cc[x] > 0 && idx[x] <= cc[x] - 1 + dd[x])
-                                                            dv -= (double)cc[x] * EC(x, cc[x] - 1, dd[x], idx[x]);
-                                                        f *= dv;
-                                                    } else {
-                                                        f *= EC(x, cc[x], dd[x], idx[x]);
-                                                    }
-                                                }
-                                                const double term = f * Xv[hidx(tau, nu, ph)];
-                                                sacc += ((tau + nu + ph) & 1) ? -term : term;
-                                            }
-                                    gc[k] += w * sacc;
-                                }
+                            for (int e = 0; e < SH::NZ; ++e) {
+                                scd[0] += Zd[0][e] * Xv[e];
+                                scd[1] += Zd[1][e] * Xv[e];
+                                scd[2] += Zd[2][e] * Xv[e];
                             }
                             // Y_tuv = sum_{tau nu phi} Z R_{t+tau,...}, t+u+v <= LAB+1
                             double Yv[nherm(LAB + 1)];
@@ -397,10 +426,9 @@ __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopSha
                                                     y += Zv[hidx(tau, nu, ph)] * R[hidx(t + tau, u + nu, v + ph)];
                                         Yv[hidx(t, u, v)] = y;
                                     }
-                            // 1-D derivative coefficients of the lane's component
+                            // dA coefficients of the lane's component (dense; zero beyond a+b+1)
                             const double alp2 = 2.0 * sm.bs[ib][5];
-                            const double bet2 = 2.0 * (sm.bs[ib][3] - sm.bs[ib][5]);
-                            double dA1[3][LE], dB1[3][LE];
+                            double dA1[3][LE];
 #pragma unroll
                             for (int x = 0; x < 3; ++x) {
                                 const int i = ea[x], j = eb[x];
@@ -408,26 +436,46 @@ __device__ __forceinline__ void coop_group(const QuartetArgs& a, CoopLds<CoopSha
                                 for (int t = 0; t < LE; ++t) {
                                     double va = alp2 * EB(x, i + 1, j, t);
                                     if (i > 0) va -= (double)i * EB(x, i - 1, j, t);
-                                    double vb = bet2 * EB(x, i, j + 1, t);
-                                    if (j > 0) vb -= (double)j * EB(x, i, j - 1, t);
                                     dA1[x][t] = va;
-                                    dB1[x][t] = vb;
                                 }
                             }
+                            // factorised sums over v, then u, then t:
+                            //   gA_x = sum dA_x(t) E_y(u) E_z(v) Y_tuv, and likewise y, z;
+                            //   dP_x = sum E_x(t) E_y(u) E_z(v) Y_{t+1,u,v}, and likewise y, z
+                            double gq[3] = {0.0, 0.0, 0.0}, sq[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-                            for (int t = 0; t <= LAB + 1; ++t)
+                            for (int t = 0; t <= LAB + 1; ++t) {
+                                double At = 0.0, Bt = 0.0, Ct = 0.0, Dt = 0.0, Et = 0.0;
 #pragma unroll
-                                for (int u = 0; u <= LAB + 1 - t; ++u)
+                                for (int u = 0; u <= LAB + 1 - t; ++u) {
+                                    double pz = 0.0, qz = 0.0, sz = 0.0;
 #pragma unroll
                                     for (int v = 0; v <= LAB + 1 - t - u; ++v) {
                                         const double y = Yv[hidx(t, u, v)];
-                                        ga[0] += dA1[0][t] * e1[1][u] * e1[2][v] * y;
-                                        ga[1] += e1[0][t] * dA1[1][u] * e1[2][v] * y;
-                                        ga[2] += e1[0][t] * e1[1][u] * dA1[2][v] * y;
-                                        gb[0] += dB1[0][t] * e1[1][u] * e1[2][v] * y;
-                                        gb[1] += e1[0][t] * dB1[1][u] * e1[2][v] * y;
-                                        gb[2] += e1[0][t] * e1[1][u] * dB1[2][v] * y;
+                                        pz += e1[2][v] * y;
+                                        qz += dA1[2][v] * y;
+                                        if (v >= 1) sz += e1[2][v - 1] * y;
                                     }
+                                    At += e1[1][u] * pz;
+                                    Bt += dA1[1][u] * pz;
+                                    Ct += e1[1][u] * qz;
+                                    if (u >= 1) Dt += e1[1][u - 1] * pz;
+                                    Et += e1[1][u] * sz;
+                                }
+                                gq[0] += dA1[0][t] * At;
+                                gq[1] += e1[0][t] * Bt;
+                                gq[2] += e1[0][t] * Ct;
+                                if (t >= 1) sq[0] += e1[0][t - 1] * At;
+                                sq[1] += e1[0][t] * Dt;
+                                sq[2] += e1[0][t] * Et;
+                            }
+                            const double goq = sm.ks[slot][5] / sm.ks[slot][3];
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                ga[k] += gq[k];
+                                gb[k] += sq[k] - gq[k];
+                                gc[k] += scd[k] - goq * sq[k];
+                            }
                         }
                     }
                 } else {
