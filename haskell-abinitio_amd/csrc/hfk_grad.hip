@@ -108,15 +108,9 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
     a.natom = natom;
     a.grad_lds = 1;
     int rc = HFK_OK;
-    // the class launches only add (atomically) into the per-atom sums, so
-    // they run concurrently, round-robin over the auxiliary streams (forked
-    // from and joined back into the context stream)
-    const bool conc = c->concurrent && c->plan.size() > 1;
-    if (conc) {
-        HFK_HIP(hipEventRecord(c->ev_fork, c->stream));
-        for (int k = 0; k < hfk_ctx::NAUX; ++k) HFK_HIP(hipStreamWaitEvent(c->aux[k], c->ev_fork, 0));
-    }
-    int slot = 0;
+    // the class launches run one after another on the context stream: each
+    // fills the GPU, and at (H2O)_64 launching them on several streams was
+    // slower (1.107 s against 1.091 s)
     const size_t np = c->plan.size();
     for (size_t i = 0; i < np; ++i) {
         const PlanEntry& e = c->plan[i];
@@ -129,7 +123,7 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
             rc = hfk_set_error(HFK_EUNSUP, "hfk_gradient: %d atoms exceed the per-workgroup accumulator", natom);
             break;
         }
-        hipStream_t s = conc ? c->aux[slot++ % hfk_ctx::NAUX] : c->stream;
+        hipStream_t s = c->stream;
         HFK_HIP(hipEventRecord(c->events[4 * np + 2 * i], s));
         if (hfk_launch_grad(e.la, e.lb, e.lc, e.ld, c->opt.boys_mode, e.mode, a, s)) {
             rc = hfk_set_error(HFK_EUNSUP, "no gradient kernel for class (%d%d|%d%d)", e.la, e.lb, e.lc, e.ld);
@@ -137,11 +131,6 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
         }
         HFK_HIP(hipEventRecord(c->events[4 * np + 2 * i + 1], s));
     }
-    if (conc)
-        for (int k = 0; k < hfk_ctx::NAUX; ++k) {
-            HFK_HIP(hipEventRecord(c->ev_join[k], c->aux[k]));
-            HFK_HIP(hipStreamWaitEvent(c->stream, c->ev_join[k], 0));
-        }
     if (rc == HFK_OK) {
         hfk_launch_grad_onee(hfk_onee_args(c), c->d_atom_of_bf, c->d_flatA, c->d_flatB, d_grad, c->stream);
         if (hipGetLastError() != hipSuccess ||
@@ -151,7 +140,7 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
     }
     dev_free(c, d_grad);
     if (rc != HFK_OK) return rc;
-    // per-class launch times (their own when the classes ran serially)
+    // per-class launch times
     for (size_t i = 0; i < np; ++i) {
         float ms = 0.f;
         HFK_HIP(hipEventElapsedTime(&ms, c->events[4 * np + 2 * i], c->events[4 * np + 2 * i + 1]));
