@@ -308,6 +308,10 @@ def prod_sum(factors_by_axis, inner):
 # all-components test (shorter live ranges; measured faster for these), the
 # others form them once before the branch.
 GRAD_LAZY = {(1, 1, 1, 0)}
+# Classes whose gradient splits the bra components over this many launches
+# (gen_grad_class_z): the density block of all components does not fit one
+# lane's registers.
+GRAD_SPLIT = {(1, 1, 1, 1): 3}
 
 
 def gen_grad_class(la, lb, lc, ld):
@@ -599,6 +603,272 @@ def gen_grad_class(la, lb, lc, ld):
     return "\n".join(out)
 
 
+def gen_grad_class_z(la, lb, lc, ld, sub, fname):
+    """Gradient contraction for the bra components `sub` (indices into the
+    bra's Cartesian pairs) of class (la lb|lc ld), for classes whose ket
+    holds too many components for the W route.  G is the lane's density
+    block G[s][cd] over s in sub.  Per ket primitive the density is
+    contracted with the ket's Hermite coefficients,
+        Z^s_tuv = sum_cd G[s][cd] (-1)^{t+u+v} E^cd_tuv,
+    so per primitive quartet Y^s_tuv = sum_tau Z^s_tau R_{t+tau} (no W);
+    dA, dP and the Z^cd sums for dC follow gen_grad_class.  The outputs are
+    the sub-block's share of dA, dB, dC: partial sums over bra components,
+    which the caller adds."""
+    L = la + lb + lc + ld
+    ca, cb, cc, cd = comps(la), comps(lb), comps(lc), comps(ld)
+    na, nb, nc, nd = len(ca), len(cb), len(cc), len(cd)
+    nbra, nket = na * nb, nc * nd
+    nsub = len(sub)
+    lab, lcd = la + lb, lc + ld
+    out = []
+    out.append("// bra: nbp staged primitive-pair records (PP_K holds 2 pi^(5/2) K); bq: their")
+    out.append("// per-component screen values, bq[ib * %d + ab] (MODE 1).  G: [%d][%d], bra" % (nbra, nsub, nket))
+    out.append("// components %s." % ", ".join(str(x) for x in sub))
+    out.append("template <typename T, int BOYS, int MODE, typename KR>")
+    out.append("__device__ __forceinline__ void %s(const double* __restrict__ bra, const double* __restrict__ bq, int nbp," % fname)
+    out.append("    const KR& kr, int nkp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
+    out.append("{")
+    out.append("  T gA0 = T(0), gA1 = T(0), gA2 = T(0), gP0 = T(0), gP1 = T(0), gP2 = T(0);")
+    out.append("  T gC0 = T(0), gC1 = T(0), gC2 = T(0);")
+    out.append("  for (int ik = 0; ik < nkp; ++ik) {")
+    out.append("    const double* pk = kr.prim(ik);")
+    em = Emitter()
+    em.raw("    const double kqlo = kr.at(pk, PP_QLO), kqhi = kr.at(pk, PP_QHI);")
+    em.raw("    const T kq = T(kr.at(pk, PP_P));")
+    em.raw("    const T kQx = T(kr.at(pk, PP_PX)), kQy = T(kr.at(pk, PP_PY)), kQz = T(kr.at(pk, PP_PZ));")
+    em.raw("    const T kCx = T(kr.at(pk, PP_AX)), kCy = T(kr.at(pk, PP_AY)), kCz = T(kr.at(pk, PP_AZ));")
+    em.raw("    const T kDx = T(kr.at(pk, PP_BX)), kDy = T(kr.at(pk, PP_BY)), kDz = T(kr.at(pk, PP_BZ));")
+    em.raw("    const T kK = T(kr.at(pk, PP_K));")
+    em.raw("    const T koo2q = T(kr.at(pk, PP_OO2P));")
+    em.raw("    const T kgam2 = T(2) * T(kr.at(pk, PP_EA));")
+    em.raw("    const T kgoq = kgam2 * koo2q;   // gamma / q")
+    em.raw("    double kqv[%d];" % nket)
+    em.raw("    if constexpr (MODE == 1) {")
+    em.raw("#pragma unroll")
+    em.raw("      for (int z = 0; z < %d; ++z) kqv[z] = kr.q(pk, z);" % nket)
+    em.raw("    }")
+    em.raw("    if (MODE == 1 && kqhi * bra[PP_QHI] < qtau) break;   // records by descending QHI")
+    Ek = [hermite_E(em, lc, ld, "kC" + ax, "kD" + ax, "koo2q", "k" + ax) for ax in "xyz"]
+
+    def Efac(E, i, j):
+        return [(t, E[(i, j, t)]) for t in range(i + j + 1)]
+
+    zbox = {}
+    for ic, c in enumerate(cc):
+        for id_, d in enumerate(cd):
+            kidx = ic * nd + id_
+            zbox[kidx] = [(t, u, v) for t in range(c[0] + d[0] + 1) for u in range(c[1] + d[1] + 1)
+                          for v in range(c[2] + d[2] + 1)]
+    kunion = sorted(set(k for b_ in zbox.values() for k in b_))
+    # ket products (-1)^{t+u+v} E^cd_tuv, once per ket primitive
+    Pk = {}
+    for kidx in range(nket):
+        c, d = cc[kidx // nd], cd[kidx % nd]
+        for (t, u, v) in zbox[kidx]:
+            fs = [f for f in (Ek[0][(c[0], d[0], t)], Ek[1][(c[1], d[1], u)], Ek[2][(c[2], d[2], v)]) if f != "T(1)"]
+            e = "*".join(fs) if fs else "T(1)"
+            if (t + u + v) % 2:
+                e = "-(%s)" % e if e != "T(1)" else "T(-1)"
+            Pk[(kidx, t, u, v)] = em.tmp(e, "pk") if ("*" in e or e.startswith("-")) else e
+
+    def zsum(emx, Gname, tag):
+        Z = {}
+        for si in range(nsub):
+            for key in kunion:
+                terms = []
+                for kidx in range(nket):
+                    if (kidx,) + key in Pk:
+                        p = Pk[(kidx,) + key]
+                        terms.append(Gname(si, kidx) if p == "T(1)" else "%s*%s" % (Gname(si, kidx), p))
+                Z[(si,) + key] = emx.tmp(" + ".join(terms), tag)
+        return Z
+    Zf = zsum(em, lambda si, k: "T(G[%d])" % (si * nket + k), "zf")
+    for kidx in range(nket):
+        for (t, u, v) in zbox[kidx]:
+            em.raw("    T za%d_%d%d%d = T(0);" % (kidx, t, u, v))
+    em.raw("    T sp0 = T(0), sp1 = T(0), sp2 = T(0);")
+    out.extend(em.lines)
+
+    out.append("    for (int ib = 0; ib < nbp; ++ib) {")
+    out.append("      const double* pb = bra + (size_t)ib * PP_STRIDE;")
+    out.append("      const double bqlo = pb[PP_QLO], bqhi = pb[PP_QHI];")
+    out.append("      if (MODE == 1 && bqhi * kqhi < qtau) break;   // records by descending QHI")
+    out.append("      T alpha, Xpq, Ypq, Zpq, rs, Targ;")
+    out.append("      quartet_geom<T>(pb, kq, kQx, kQy, kQz, alpha, Xpq, Ypq, Zpq, rs, Targ);")
+    out.append("      const bool allin = MODE == 0 || (bqlo * kqlo >= qtau);")
+    em2 = Emitter()
+    em2.n = em.n
+    em2.raw("      const T bAx = T(pb[PP_AX]), bAy = T(pb[PP_AY]), bAz = T(pb[PP_AZ]);")
+    em2.raw("      const T bBx = T(pb[PP_BX]), bBy = T(pb[PP_BY]), bBz = T(pb[PP_BZ]);")
+    em2.raw("      const T bK = T(pb[PP_K]);")
+    em2.raw("      const T boo2p = T(pb[PP_OO2P]);")
+    em2.raw("      const T balp2 = T(2) * T(pb[PP_EA]);")
+    Eb = [hermite_E(em2, la + 1, lb, "bA" + ax, "bB" + ax, "boo2p", "b" + ax) for ax in "xyz"]
+    em2.raw("      T F[%d];" % (L + 2))
+    em2.raw("      boys_fast<T, %d, BOYS>(Targ, bt, F);" % (L + 1))
+    em2.raw("      const T pref = bK * kK * rs;")
+    em2.raw("      const T m2a = T(-2) * alpha;")
+    R = {}
+    pw = "pref"
+    for n in range(0, L + 2):
+        if n == 0:
+            R[(0, 0, 0, 0)] = em2.tmp("pref * F[0]", "r")
+        else:
+            pw = em2.tmp("%s * m2a" % pw, "w")
+            R[(n, 0, 0, 0)] = em2.tmp("%s * F[%d]" % (pw, n), "r")
+
+    def getR(n, t, u, v):
+        if t < 0 or u < 0 or v < 0:
+            return None
+        key = (n, t, u, v)
+        if key in R:
+            return R[key]
+        if t >= 1:
+            a_ = getR(n + 1, t - 2, u, v); b_ = getR(n + 1, t - 1, u, v); X = "Xpq"; c_ = t - 1
+        elif u >= 1:
+            a_ = getR(n + 1, t, u - 2, v); b_ = getR(n + 1, t, u - 1, v); X = "Ypq"; c_ = u - 1
+        else:
+            a_ = getR(n + 1, t, u, v - 2); b_ = getR(n + 1, t, u, v - 1); X = "Zpq"; c_ = v - 1
+        terms = []
+        if a_ is not None and c_ != 0:
+            terms.append("T(%d)*%s" % (c_, a_))
+        terms.append("%s*%s" % (X, b_))
+        R[key] = em2.tmp(" + ".join(terms), "r")
+        return R[key]
+
+    DA = [dict() for _ in range(3)]
+    for x in range(3):
+        for s in sub:
+            a, b = ca[s // nb], cb[s % nb]
+            key = (a[x], b[x])
+            if key in DA[x]:
+                continue
+            ai, bi = key
+            lst = []
+            for t in range(ai + bi + 2):
+                e = Eb[x][(ai + 1, bi, t)]
+                terms = ["balp2*%s" % e if e != "T(1)" else "balp2"]
+                if ai > 0 and t <= ai - 1 + bi:
+                    e = Eb[x][(ai - 1, bi, t)]
+                    terms.append("- T(%d)*%s" % (ai, e) if e != "T(1)" else "- T(%d)" % ai)
+                lst.append((t, em2.tmp(" ".join(terms), "da")))
+            DA[x][key] = lst
+    ysets = {}
+    for si, s in enumerate(sub):
+        a, b = ca[s // nb], cb[s % nb]
+        box = [a[x] + b[x] for x in range(3)]
+        ys = []
+        for t in range(box[0] + 2):
+            for u in range(box[1] + 2):
+                for v in range(box[2] + 2):
+                    over = max(0, t - box[0]) + max(0, u - box[1]) + max(0, v - box[2])
+                    if over <= 1:
+                        ys.append((t, u, v))
+        ysets[si] = ys
+    # R values used below, formed before the branch
+    for si in range(nsub):
+        for (t, u, v) in ysets[si]:
+            for (tau, nu, phi) in kunion:
+                getR(0, t + tau, u + nu, v + phi)
+    Xv = {}
+    for si, s in enumerate(sub):
+        a, b = ca[s // nb], cb[s % nb]
+        fac = [Efac(Eb[x], a[x], b[x]) for x in range(3)]
+        for (tau, nu, phi) in kunion:
+            Xv[(si, tau, nu, phi)] = em2.tmp(
+                prod_sum(fac, lambda t, u, v, tau=tau, nu=nu, phi=phi: getR(0, t + tau, u + nu, v + phi)), "X")
+
+    def body(Z, Gname, ind):
+        for si, s in enumerate(sub):
+            a, b = ca[s // nb], cb[s % nb]
+            Y = {}
+            for (t, u, v) in ysets[si]:
+                terms = []
+                for (tau, nu, phi) in kunion:
+                    r = getR(0, t + tau, u + nu, v + phi)
+                    terms.append("%s*%s" % (Z[(si, tau, nu, phi)], r))
+                Y[(t, u, v)] = em2.tmp(" + ".join(terms), "Y")
+            for k in range(3):
+                fac = [DA[x][(a[x], b[x])] if x == k else Efac(Eb[x], a[x], b[x]) for x in range(3)]
+                val = prod_sum(fac, lambda t, u, v: Y.get((t, u, v)))
+                if val:
+                    em2.raw("%sgA%d += %s;" % (ind, k, val))
+                fac = [Efac(Eb[x], a[x], b[x]) for x in range(3)]
+                sh = [1 if x == k else 0 for x in range(3)]
+                val = prod_sum(fac, lambda t, u, v: Y.get((t + sh[0], u + sh[1], v + sh[2])))
+                if val:
+                    em2.raw("%ssp%d += %s;" % (ind, k, val))
+        for kidx in range(nket):
+            for (t, u, v) in zbox[kidx]:
+                terms = ["%s*%s" % (Gname(si, kidx), Xv[(si, t, u, v)]) for si in range(nsub)]
+                em2.raw("%sza%d_%d%d%d += %s;" % (ind, kidx, t, u, v, " + ".join(terms)))
+
+    em2.raw("      double bqv[%d];" % nsub)
+    em2.raw("      if constexpr (MODE == 1) {")
+    for si, s in enumerate(sub):
+        em2.raw("        bqv[%d] = bq[ib * %d + %d];" % (si, nbra, s))
+    em2.raw("      }")
+    em2.raw("      if (MODE == 0 || __all(allin)) {")
+    body(Zf, lambda si, k: "T(G[%d])" % (si * nket + k), "        ")
+    em2.raw("      } else {")
+    gm = {}
+    for si in range(nsub):
+        for k in range(nket):
+            gm[(si, k)] = em2.tmp("(allin || !(bqv[%d] * kqv[%d] < qtau)) ? T(G[%d]) : T(0)" % (si, k, si * nket + k), "gm")
+    Zm = zsum(em2, lambda si, k: gm[(si, k)], "zm")
+    body(Zm, lambda si, k: gm[(si, k)], "        ")
+    em2.raw("      }")
+    out.extend(em2.lines)
+    out.append("    }")
+    # per ket primitive: dP and the d/dX_CD part of dC (ket tables formed again here)
+    em3 = Emitter()
+    em3.n = em2.n
+    Ek2 = [hermite_E(em3, lc + 1, ld, "kC" + ax, "kD" + ax, "koo2q", "q" + ax) for ax in "xyz"]
+    dEk = [dict() for _ in range(3)]
+    for x in range(3):
+        for c in cc:
+            for d in cd:
+                key = (c[x], d[x])
+                if key in dEk[x]:
+                    continue
+                ci, di = key
+                lst = []
+                for t in range(ci + di + 1):
+                    terms = []
+                    e = Ek2[x][(ci + 1, di, t)]
+                    terms.append("kgam2*%s" % e if e != "T(1)" else "kgam2")
+                    if ci > 0 and t <= ci - 1 + di:
+                        e = Ek2[x][(ci - 1, di, t)]
+                        terms.append("- T(%d)*%s" % (ci, e) if e != "T(1)" else "- T(%d)" % ci)
+                    if t >= 1:
+                        e = Ek2[x][(ci, di, t - 1)]
+                        terms.append("- kgoq*%s" % e if e != "T(1)" else "- kgoq")
+                    lst.append((t, em3.tmp(" ".join(terms), "dk")))
+                dEk[x][key] = lst
+    for k in range(3):
+        terms = []
+        for kidx in range(nket):
+            c, d = cc[kidx // nd], cd[kidx % nd]
+            fac = [dEk[x][(c[x], d[x])] if x == k else Efac(Ek2[x], c[x], d[x]) for x in range(3)]
+
+            def inner(t, u, v, kidx=kidx):
+                z = "za%d_%d%d%d" % (kidx, t, u, v)
+                return ("-%s" % z) if (t + u + v) % 2 else z
+            val = prod_sum(fac, inner)
+            if val:
+                terms.append(val)
+        scd = " + ".join(terms).replace("+ -", "- ") if terms else "T(0)"
+        em3.raw("    gP%d += sp%d;" % (k, k))
+        em3.raw("    gC%d += (%s) - kgoq * sp%d;" % (k, scd, k))
+    out.extend(em3.lines)
+    out.append("  }")
+    out.append("  g[0] = gA0; g[1] = gA1; g[2] = gA2;")
+    out.append("  g[3] = gP0 - gA0; g[4] = gP1 - gA1; g[5] = gP2 - gA2;")
+    out.append("  g[6] = gC0; g[7] = gC1; g[8] = gC2;")
+    out.append("}")
+    return "\n".join(out)
+
+
 # Classes with a d shell holding more contracted integrals per quartet than
 # this run on the cooperative wave kernel (per-lane registers cannot hold
 # them); the host's hfk_class_coop() uses the same bounds.
@@ -633,7 +903,7 @@ def main():
         for l in q:
             nv *= (l + 1) * (l + 2) // 2
         jk_coop = max(q) >= 2 and nv > COOP_NV_JK   # cooperative wave kernel (hfk_coop_impl.hpp)
-        grad_coop = (max(q) >= 2 and nv > COOP_NV_GRAD) or nv > COOP_NV_JK
+        grad_coop = max(q) >= 2 and nv > COOP_NV_GRAD
         body = ["// Generated by gen_eri.py -- do not edit.", "#include <hip/hip_runtime.h>"]
         if jk_coop:
             body += ['#include "hfk_coop_impl.hpp"',
@@ -656,11 +926,25 @@ def main():
         body.append("    const KR& kr, int nkp, const BoysTables& bt, T* __restrict__ V, double qtau)")
         body.append("{ eri_%s<T, BOYS, MODE, BRAK>(bra, bq, nbp, kr, nkp, bt, V, qtau); }" % tag)
         if not grad_coop:
-            body.append(gen_grad_class(*q))
-            body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, typename KR>")
+            ngrp = GRAD_SPLIT.get(q, 1)
+            body.append("// bra components per gradient launch: NBRA / kGradGroups")
+            body.append("constexpr int kGradGroups = %d;" % ngrp)
+            if ngrp == 1:
+                body.append(gen_grad_class(*q))
+                calls = ["grad_%s<T, BOYS, MODE>(bra, bq, nbp, kr, nkp, bt, G, qtau, g);" % tag]
+            else:
+                na_ = (q[0] + 1) * (q[0] + 2) // 2 * ((q[1] + 1) * (q[1] + 2) // 2)
+                ns = na_ // ngrp
+                for gi in range(ngrp):
+                    body.append(gen_grad_class_z(*q, list(range(gi * ns, (gi + 1) * ns)), "grad_%s_g%d" % (tag, gi)))
+                calls = ["grad_%s_g%d<T, BOYS, MODE>(bra, bq, nbp, kr, nkp, bt, G, qtau, g);" % (tag, gi) for gi in range(ngrp)]
+            body.append("template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, int GRP, typename KR>")
             body.append("__device__ __forceinline__ void grad_class(const double* __restrict__ bra, const double* __restrict__ bq, int nbp,")
             body.append("    const KR& kr, int nkp, const BoysTables& bt, const T* __restrict__ G, double qtau, T* __restrict__ g)")
-            body.append("{ grad_%s<T, BOYS, MODE>(bra, bq, nbp, kr, nkp, bt, G, qtau, g); }" % tag)
+            body.append("{")
+            for gi, c in enumerate(calls):
+                body.append("  %sif constexpr (GRP == %d) %s" % ("" if gi == 0 else "else ", gi, c))
+            body.append("}")
         body.append("}  // namespace")
         body.append('#include "hfk_eri_impl.hpp"')
         if grad_coop:

@@ -23,12 +23,17 @@ constexpr int grad_min_waves()
     return (LA + LB + LC + LD <= 1) ? 3 : ((LA + LB + LC + LD == 2) ? 2 : 1);
 }
 
-template <int LA, int LB, int LC, int LD, int BOYS, int MODE>
+// GRP: which of the kGradGroups blocks of bra components this launch
+// contracts (a class whose density block does not fit one lane's registers
+// splits its bra components over several launches; the partial sums add).
+template <int LA, int LB, int LC, int LD, int BOYS, int MODE, int GRP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_min_waves<LA, LB, LC, LD>())))
 k_grad_tiles(QuartetArgs a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NKET = NC * ND, NBRA = NA * NB;
+    constexpr int NSUB = NBRA / kGradGroups, SUB0 = GRP * NSUB;   // bra components [SUB0, SUB0 + NSUB)
+    static_assert(NSUB * kGradGroups == NBRA, "bra components split evenly");
     constexpr int NR = NA + NB;
     constexpr int NF = hfk_ket_fields(LC, LD);
     extern __shared__ double gsm[];
@@ -141,11 +146,10 @@ k_grad_tiles(QuartetArgs a)
         if (A == B) f *= 0.5;
         if (flags & 1) f *= 0.5;
         if (bp == a.ket_idx[pos]) f *= 0.5;
-        double G[NBRA * NKET];
+        double G[NSUB * NKET];
 #pragma unroll
-        for (int x = 0; x < NA; ++x)
-#pragma unroll
-            for (int y = 0; y < NB; ++y)
+        for (int si = 0; si < NSUB; ++si) {
+            const int x = (SUB0 + si) / NB, y = (SUB0 + si) % NB;
 #pragma unroll
                 for (int z = 0; z < NC; ++z)
 #pragma unroll
@@ -164,13 +168,14 @@ k_grad_tiles(QuartetArgs a)
                             dbc = D[(size_t)ib[y] * ld + ic[z]];
                         }
                         const double gam = Dab[x * NB + y] * dcd - 0.25 * (dac * dbd + dad * dbc);
-                        G[((x * NB + y) * NC + z) * ND + w] = (f * sa[x] * sb[y] * sc[z] * sd[w]) * gam;
+                        G[(si * NC + z) * ND + w] = (f * sa[x] * sb[y] * sc[z] * sd[w]) * gam;
                     }
+        }
         const int slen = rlen[r];
         const KetSoA<NF> kr{a.ksoa + rsoa[r] + pos + (size_t)kprim0 * NF * slen, slen};
         double g[9];
-        grad_class<LA, LB, LC, LD, double, BOYS, MODE>(sbra, sbq, nbp, kr, min(kprim1, rnpp[r]) - kprim0, a.bt, G, a.qtau,
-                                                       g);
+        grad_class<LA, LB, LC, LD, double, BOYS, MODE, GRP>(sbra, sbq, nbp, kr, min(kprim1, rnpp[r]) - kprim0, a.bt, G,
+                                                            a.qtau, g);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             gA[k] += g[k];
@@ -205,19 +210,26 @@ k_grad_tiles(QuartetArgs a)
     }
 }
 
-template <int LA, int LB, int LC, int LD, int BOYS, int MODE>
-void launch_grad_tiles(const QuartetArgs& a, hipStream_t s)
+template <int LA, int LB, int LC, int LD, int BOYS, int MODE, int GRP>
+void launch_grad_group(const QuartetArgs& a, hipStream_t s)
 {
-    if (a.ntask <= 0) return;
     const size_t shm = hfk_grad_lds_bytes(LA, LB, a.ncols, a.grad_lds ? a.natom : 0, a.drows, a.bra_stage);
     long long done = 0;
     while (done < a.ntask) {
         const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);
         QuartetArgs b = a;
         b.tasks = a.tasks + done;
-        hipLaunchKernelGGL((k_grad_tiles<LA, LB, LC, LD, BOYS, MODE>), dim3((unsigned)cnt), dim3(a.block), shm, s, b);
+        hipLaunchKernelGGL((k_grad_tiles<LA, LB, LC, LD, BOYS, MODE, GRP>), dim3((unsigned)cnt), dim3(a.block), shm, s, b);
         done += cnt;
     }
+    if constexpr (GRP + 1 < kGradGroups) launch_grad_group<LA, LB, LC, LD, BOYS, MODE, GRP + 1>(a, s);
+}
+
+template <int LA, int LB, int LC, int LD, int BOYS, int MODE>
+void launch_grad_tiles(const QuartetArgs& a, hipStream_t s)
+{
+    if (a.ntask <= 0) return;
+    launch_grad_group<LA, LB, LC, LD, BOYS, MODE, 0>(a, s);
 }
 
 template <int LA, int LB, int LC, int LD>

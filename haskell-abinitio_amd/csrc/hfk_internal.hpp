@@ -104,13 +104,13 @@ inline bool hfk_class_coop(int la, int lb, int lc, int ld)
 }
 
 // Classes whose gradient runs on the cooperative kernel (d classes with more
-// than 18 integrals per quartet, and any class with more than 54); gen_eri.py
-// uses the same bounds.  That kernel needs tasks of one run with an explicit
+// than 18 integrals per quartet; (pp|pp) splits its bra components over
+// per-lane launches instead); gen_eri.py uses the same bounds.  That kernel needs tasks of one run with an explicit
 // ket primitive range (uniform over the workgroup).
 inline bool hfk_class_grad_coop(int la, int lb, int lc, int ld)
 {
     const int nv = (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2) * ((lc + 1) * (lc + 2) / 2) * ((ld + 1) * (ld + 2) / 2);
-    return ((la >= 2 || lb >= 2 || lc >= 2 || ld >= 2) && nv > 18) || nv > 54;
+    return (la >= 2 || lb >= 2 || lc >= 2 || ld >= 2) && nv > 18;
 }
 
 // The per-lane J/K kernel stages the task's bra primitive-pair records and
