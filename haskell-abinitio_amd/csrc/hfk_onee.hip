@@ -307,10 +307,17 @@ __global__ void __launch_bounds__(128) k_onee_v(OneEArgs a, const int* plist, in
         const double Cx = a.R[3 * c], Cy = a.R[3 * c + 1], Cz = a.R[3 * c + 2];
         double hv = 0.0;
         double dA[3] = {0.0, 0.0, 0.0}, dB[3] = {0.0, 0.0, 0.0};
+        // derivatives: primitive pairs with exp(-mu |AB|^2) < 1e-20 (mu |AB|^2 > 46)
+        // are skipped; the potential itself (hcore) keeps every pair
+        const double dab2 = (r1[0] - r2[0]) * (r1[0] - r2[0]) + (r1[1] - r2[1]) * (r1[1] - r2[1]) +
+                            (r1[2] - r2[2]) * (r1[2] - r2[2]);
         for (int g1 = a.prim_off[i]; g1 < a.prim_off[i + 1]; ++g1)
             for (int g2 = a.prim_off[j]; g2 < a.prim_off[j + 1]; ++g2) {
                 const double e1 = a.expo[g1], e2 = a.expo[g2];
                 const double gamma = e1 + e2, rg = 1.0 / gamma, mu = e1 * e2 * rg;
+                if constexpr (DERIV) {
+                    if (mu * dab2 > 46.0) continue;
+                }
                 double P[3], ex[3][TN], dax[3][TN], dbx[3][TN], ab2 = 0.0;
 #pragma unroll
                 for (int x = 0; x < 3; ++x) {
