@@ -34,17 +34,27 @@ static int grad_upload_maps(hfk_ctx* c)
 // linear, F(D) = F(D0) + G(D - D0), built with the Schwarz x max|D - D0| screen
 // at tau = 1e-13 (each skipped shell quartet changes an element of F by less
 // than 4e-13).  Near convergence that costs a small part of a full build.
+// The context's gradient workspace (allocated on the first hfk_gradient and
+// kept): H, F (flat), Df, Ff, T (n*n each; T also holds reduction partials),
+// the per-atom sums.
+static int grad_workspace(hfk_ctx* c, int natom)
+{
+    if (c->d_gwork) return HFK_OK;
+    const size_t nf = (size_t)c->n * (c->n + 1) / 2, nn = std::max<size_t>((size_t)c->n * c->n, 1024);
+    return dev_alloc(&c->d_gwork, 2 * nf + 3 * nn + 3 * (size_t)natom);
+}
+
 static int grad_energy_weighted(hfk_ctx* c, const double* Dflat, double* Wflat)
 {
     const int n = c->n;
-    const size_t nf = (size_t)n * (n + 1) / 2, nn = (size_t)n * n;
-    double *H = nullptr, *F = nullptr, *Df = nullptr, *Ff = nullptr, *T = nullptr;
-    int rc = dev_alloc(&H, nf);
-    if (rc == HFK_OK) rc = dev_alloc(&F, nf);
-    if (rc == HFK_OK) rc = dev_alloc(&Df, nn);
-    if (rc == HFK_OK) rc = dev_alloc(&Ff, nn);
-    if (rc == HFK_OK) rc = dev_alloc(&T, std::max<size_t>(nn, 1024));   // also the reduction partials
-    if (rc == HFK_OK && c->scf_cached) {
+    const size_t nf = (size_t)n * (n + 1) / 2, nn = std::max<size_t>((size_t)n * n, 1024);
+    double* H = c->d_gwork;
+    double* F = H + nf;
+    double* Df = F + nf;
+    double* Ff = Df + nn;
+    double* T = Ff + nn;   // also the reduction partials
+    int rc = HFK_OK;
+    if (c->scf_cached) {
         // Df (n*n >= nf) holds D - D0, T the reduction partials, Ff[0] max|D - D0|
         hipMemcpyAsync(Df, Dflat, sizeof(double) * nf, hipMemcpyDeviceToDevice, c->stream);
         hfk_launch_axpby((long long)nf, -1.0, c->d_scf_D, 1.0, Df, c->stream);
@@ -71,11 +81,6 @@ static int grad_energy_weighted(hfk_ctx* c, const double* Dflat, double* Wflat)
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
             rc = hfk_set_error(HFK_EHIP, "hfk_gradient: energy-weighted density failed");
     }
-    dev_free(c, T);
-    dev_free(c, Ff);
-    dev_free(c, Df);
-    dev_free(c, F);
-    dev_free(c, H);
     return rc;
 }
 
@@ -89,6 +94,7 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
     if (hfk_grad_lds_bytes(0, 0, 0, natom, 0, 0) > 64 * 1024)
         return hfk_set_error(HFK_EUNSUP, "hfk_gradient: %d atoms exceed the per-workgroup accumulator", natom);
     HFK_TRY(grad_upload_maps(c));
+    HFK_TRY(grad_workspace(c, natom));
     const size_t nf = (size_t)n * (n + 1) / 2;
     HFK_HIP(hipMemcpyAsync(c->d_flatA, D_flat, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
     if (W_flat) {
@@ -96,8 +102,7 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
     } else {
         HFK_TRY(grad_energy_weighted(c, c->d_flatA, c->d_flatB));
     }
-    double* d_grad = nullptr;
-    HFK_TRY(dev_alloc(&d_grad, (size_t)3 * natom));
+    double* d_grad = c->d_gwork + 2 * ((size_t)n * (n + 1) / 2) + 3 * std::max<size_t>((size_t)n * n, 1024);
     hfk_launch_fill(d_grad, 0.0, 3LL * natom, c->stream);
     hfk_launch_flat_to_padded(n, c->d_flatA, c->d_D, c->stream);
     QuartetArgs a = hfk_ctx_args(c);
@@ -138,7 +143,6 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
             hipStreamSynchronize(c->stream) != hipSuccess)
             rc = hfk_set_error(HFK_EHIP, "hfk_gradient: kernel or copy-out failed");
     }
-    dev_free(c, d_grad);
     if (rc != HFK_OK) return rc;
     // per-class launch times
     for (size_t i = 0; i < np; ++i) {

@@ -144,6 +144,7 @@ struct hfk_ctx {
     size_t slot_bytes = 0;          // device memory of all J/K slots
     double *d_flatA = nullptr, *d_flatB = nullptr, *d_flatC = nullptr;
     int *d_sh_atom = nullptr, *d_atom_of_bf = nullptr;   // gradient (uploaded on first use)
+    double* d_gwork = nullptr;   // gradient workspace: 3 n*n + 2 n(n+1)/2 + 3 natom doubles (first use)
     int* d_vpair = nullptr;                 // function pairs grouped by l_i + l_j (one-electron V)
     std::vector<int> vpair_off;             // [6]
     hfk_stats stats = {};
