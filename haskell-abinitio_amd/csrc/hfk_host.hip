@@ -342,6 +342,7 @@ void hfk_ctx_free_device(hfk_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void* p : c->allocs) hipFree(p);
     c->allocs.clear();
+    hfk_plan_free_arrays(c);
     if (c->h_pin) hipHostFree(c->h_pin);
     c->h_pin = nullptr;
     for (hipEvent_t ev : c->events) hipEventDestroy(ev);
@@ -488,22 +489,32 @@ void hfk_pair_class_l(int k, int* la, int* lb)
     *lb = k - a * (a + 1) / 2;
 }
 
-// Device allocations of one plan entry (freed when the plan is rebuilt).
+// Device arrays of one plan entry (freed when the plan is rebuilt).
 template <typename T>
 static int entry_upload(hfk_ctx* c, PlanEntry& e, T** p, const std::vector<T>& v)
 {
-    *p = nullptr;
-    int rc = dev_upload(c, p, v);
-    if (rc == HFK_OK) e.owned.push_back((void*)*p);
+    (void)e;
+    int rc = plan_alloc(c, p, v.size());
+    if (rc == HFK_OK && !v.empty() &&
+        hipMemcpyAsync(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = hfk_set_error(HFK_EHIP, "plan upload failed");
     return rc;
 }
 template <typename T>
 static int entry_alloc(hfk_ctx* c, PlanEntry& e, T** p, size_t n)
 {
-    *p = nullptr;
-    int rc = dev_alloc(p, n);
-    if (rc == HFK_OK) e.owned.push_back((void*)*p);
-    return rc;
+    (void)e;
+    return plan_alloc(c, p, n);
+}
+
+void hfk_plan_free_arrays(hfk_ctx* c)
+{
+    if (c->plan_chunks.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (void* q : c->plan_chunks) hipFree(q);
+    c->plan_chunks.clear();
+    c->plan_cur = nullptr;
+    c->plan_left = 0;
 }
 
 static void free_plan(hfk_ctx* c)
@@ -511,6 +522,7 @@ static void free_plan(hfk_ctx* c)
     for (PlanEntry& e : c->plan)
         for (void* p : e.owned) dev_free(c, p);
     c->plan.clear();
+    hfk_plan_free_arrays(c);
     if (c->d_kpart) dev_free(c, c->d_kpart);
     if (c->d_jpart) dev_free(c, c->d_jpart);
     c->d_kpart = c->d_jpart = nullptr;

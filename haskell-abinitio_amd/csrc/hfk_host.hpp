@@ -144,6 +144,11 @@ struct hfk_ctx {
     size_t slot_bytes = 0;          // device memory of all J/K slots
     double *d_flatA = nullptr, *d_flatB = nullptr, *d_flatC = nullptr;
     int *d_sh_atom = nullptr, *d_atom_of_bf = nullptr;   // gradient (uploaded on first use)
+    // device arrays of the current plan, bump-allocated from chunks (plan_alloc);
+    // freed together when the plan is rebuilt or the context destroyed
+    std::vector<void*> plan_chunks;
+    char* plan_cur = nullptr;
+    size_t plan_left = 0;
     double* d_gwork = nullptr;   // gradient workspace: 3 n*n + 2 n(n+1)/2 + 3 natom doubles (first use)
     int* d_vpair = nullptr;                 // function pairs grouped by l_i + l_j (one-electron V)
     std::vector<int> vpair_off;             // [6]
@@ -216,6 +221,30 @@ void dev_free(hfk_ctx* c, T* p)
 
 // dev_alloc used for context-owned buffers is tracked too
 #define dev_alloc(p, n) dev_alloc_tracked(c, p, n)
+
+// A device array owned by the current plan: bump-allocated (256-byte aligned)
+// from chunks of at least 16 MB, so planning a small molecule makes a few
+// allocations instead of one per array; hfk_plan_free_arrays releases them.
+template <typename T>
+int plan_alloc(hfk_ctx* c, T** p, size_t n)
+{
+    constexpr size_t ALIGN = 256, CHUNK = 16u << 20;
+    const size_t bytes = ((std::max<size_t>(n, 1) * sizeof(T) + ALIGN - 1) / ALIGN) * ALIGN;
+    if (bytes > c->plan_left) {
+        const size_t sz = std::max(bytes, CHUNK);
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, sz);
+        if (e != hipSuccess) return hfk_set_error(HFK_ENOMEM, "hipMalloc(%zu bytes): %s", sz, hipGetErrorString(e));
+        c->plan_chunks.push_back(q);
+        c->plan_cur = (char*)q;
+        c->plan_left = sz;
+    }
+    *p = (T*)c->plan_cur;
+    c->plan_cur += bytes;
+    c->plan_left -= bytes;
+    return HFK_OK;
+}
+void hfk_plan_free_arrays(hfk_ctx* c);
 
 int hfk_ctx_init_device(hfk_ctx* c);
 // hfk_noalloc.hip: this device's copy of the fixed scratch array (once per device)

@@ -28,19 +28,34 @@
 namespace {
 
 // device temporaries of one planning call, freed at its end
+// Temporaries of one planning call, bump-allocated (256-byte aligned) from
+// chunks of at least 8 MB: a small molecule's plan makes a few allocations
+// instead of one hipMalloc/hipFree pair per array.  Freed together.
 struct Tmp {
-    std::vector<void*> p;
+    static constexpr size_t CHUNK = 8u << 20, ALIGN = 256;
+    std::vector<void*> chunks;
+    char* cur = nullptr;
+    size_t left = 0;
     ~Tmp()
     {
-        for (void* q : p) hipFree(q);
+        for (void* q : chunks) hipFree(q);
     }
     template <typename T>
     int alloc(T** x, size_t n)
     {
-        if (n == 0) n = 1;
-        hipError_t e = hipMalloc((void**)x, sizeof(T) * n);
-        if (e != hipSuccess) return hfk_set_error(HFK_ENOMEM, "planning: hipMalloc(%zu bytes) failed", sizeof(T) * n);
-        p.push_back((void*)*x);
+        const size_t bytes = ((std::max<size_t>(n, 1) * sizeof(T) + ALIGN - 1) / ALIGN) * ALIGN;
+        if (bytes > left) {
+            const size_t sz = std::max(bytes, CHUNK);
+            void* q = nullptr;
+            hipError_t e = hipMalloc(&q, sz);
+            if (e != hipSuccess) return hfk_set_error(HFK_ENOMEM, "planning: hipMalloc(%zu bytes) failed", sz);
+            chunks.push_back(q);
+            cur = (char*)q;
+            left = sz;
+        }
+        *x = (T*)cur;
+        cur += bytes;
+        left -= bytes;
         return HFK_OK;
     }
 };
@@ -682,8 +697,7 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
     unsigned long long* d_red;
     HFK_TRY(t.alloc(&d_red, 2));
     HFK_HIP(hipMemsetAsync(d_red, 0, 2 * sizeof(unsigned long long), c->stream));
-    HFK_TRY(dev_alloc(&jend, (size_t)std::max(1LL, nbk)));   // kept: the J'_cd layout reads it
-    e.owned.push_back(jend);
+    HFK_TRY(plan_alloc(c, &jend, (size_t)std::max(1LL, nbk)));   // kept: the J'_cd layout reads it
     hipLaunchKernelGGL(k_staircase, dim3(grid_of(nbk, 256)), dim3(256), 0, c->stream, nb, nkseg, bq, kq, d_kfirst,
                        d_knpp, bnpp, e.kb == e.kk ? 1 : 0, ref ? 1 : 0, c->opt.screen_tau, jend, d_red);
     if (e.split) {
@@ -722,8 +736,7 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
             npiece[(size_t)b * nkseg + s] = ((nbp + bstep - 1) / bstep) * ((nkp + kstep - 1) / kstep);
         }
     int* d_npiece;
-    HFK_TRY(dev_alloc(&d_npiece, npiece.size()));
-    e.owned.push_back(d_npiece);
+    HFK_TRY(plan_alloc(c, &d_npiece, npiece.size()));
     HFK_HIP(hipMemcpyAsync(d_npiece, npiece.data(), sizeof(int) * npiece.size(), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_merged, dim3(grid_of(nbk, 256)), dim3(256), 0, c->stream, nb, nkseg, jend, d_kfirst, d_knpp,
                        bnpp, single ? 1 : 0, per_lane, d_red + 1);
@@ -732,8 +745,7 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
     long long* jcdv;
     long long* jbase;
     HFK_TRY(t.alloc(&jcdv, (size_t)std::max(1LL, nbk)));
-    HFK_TRY(dev_alloc(&jbase, (size_t)nbk + 1));
-    e.owned.push_back(jbase);
+    HFK_TRY(plan_alloc(c, &jbase, (size_t)nbk + 1));
     hipLaunchKernelGGL(k_jcd_count, dim3(grid_of(nbk, 256)), dim3(256), 0, c->stream, nb, nkseg, nbseg, d_bfirst,
                        d_npiece, jend, d_kfirst, jcdv);
     HFK_TRY(scan_total(c, t, jcdv, jbase, nbk));
@@ -795,11 +807,9 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
     HFK_TRY(t.alloc(&cost, (size_t)ntot));
     HFK_TRY(t.alloc(&d_scal, 2));
     HFK_HIP(hipMemsetAsync(d_scal, 0, 2 * sizeof(int), c->stream));
-    HFK_TRY(dev_alloc(&tasks, (size_t)ntot));
-    e.owned.push_back(tasks);
+    HFK_TRY(plan_alloc(c, &tasks, (size_t)ntot));
     KetRun* runs;
-    HFK_TRY(dev_alloc(&runs, (size_t)rtot));
-    e.owned.push_back(runs);
+    HFK_TRY(plan_alloc(c, &runs, (size_t)rtot));
     w.tasks = tasks_raw;
     w.runs = runs;
     w.cost = cost;
@@ -858,10 +868,8 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
     HFK_TRY(t.alloc(&pos_of, (size_t)c->npair));
     HFK_TRY(t.alloc(&kcnt, (size_t)nshell));
     HFK_TRY(t.alloc(&bcnt, (size_t)std::max(1, nb)));
-    HFK_TRY(dev_alloc(&koff, (size_t)nshell + 1));
-    e.owned.push_back(koff);
-    HFK_TRY(dev_alloc(&boff, (size_t)nb + 1));
-    e.owned.push_back(boff);
+    HFK_TRY(plan_alloc(c, &koff, (size_t)nshell + 1));
+    HFK_TRY(plan_alloc(c, &boff, (size_t)nb + 1));
     HFK_HIP(hipMemsetAsync(kcnt, 0, sizeof(int) * nshell, c->stream));
     HFK_HIP(hipMemsetAsync(bcnt, 0, sizeof(int) * std::max(1, nb), c->stream));
     if (nb > 0) hipLaunchKernelGGL(k_pos_of, dim3(grid_of(nb, 256)), dim3(256), 0, c->stream, nb, d_bl, pos_of);
@@ -887,10 +895,8 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
     HFK_TRY(scan_total(c, t, bcnt, boff, nb));
     int2* kcsr;
     int* jab_slot;
-    HFK_TRY(dev_alloc(&kcsr, (size_t)2 * ntot));
-    e.owned.push_back(kcsr);
-    HFK_TRY(dev_alloc(&jab_slot, (size_t)ntot));
-    e.owned.push_back(jab_slot);
+    HFK_TRY(plan_alloc(c, &kcsr, (size_t)2 * ntot));
+    HFK_TRY(plan_alloc(c, &jab_slot, (size_t)ntot));
     if (ntot > 0)
         hipLaunchKernelGGL(k_list_out, dim3(grid_of(2 * ntot, 256)), dim3(256), 0, c->stream, ntot, NA, kval2, bval2,
                            kcsr, jab_slot);
@@ -916,12 +922,9 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
         e.nkjob = h_tot[0];
         e.nkrow = h_tot[1];
         if (e.nkjob > 0) {
-            HFK_TRY(dev_alloc(&e.d_kjob, (size_t)e.nkjob));
-            e.owned.push_back(e.d_kjob);
-            HFK_TRY(dev_alloc(&e.d_krow, (size_t)e.nkrow));
-            e.owned.push_back(e.d_krow);
-            HFK_TRY(dev_alloc(&e.d_kseg, (size_t)e.nkjob * e.ncols));
-            e.owned.push_back(e.d_kseg);
+            HFK_TRY(plan_alloc(c, &e.d_kjob, (size_t)e.nkjob));
+            HFK_TRY(plan_alloc(c, &e.d_krow, (size_t)e.nkrow));
+            HFK_TRY(plan_alloc(c, &e.d_kseg, (size_t)e.nkjob * e.ncols));
             hipLaunchKernelGGL(k_kjob_write, dim3(grid_of(nshell, 256)), dim3(256), 0, c->stream, nshell, koff,
                                c->d_sh_bf, joff, rwoff, e.d_kjob, e.d_krow);
         }
@@ -943,13 +946,10 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
         HFK_TRY(t.alloc(&pos_len, (size_t)nk));
         HFK_TRY(t.alloc(&pos_npp, (size_t)nk));
         HFK_TRY(t.alloc(&kscale, (size_t)(NC + ND) * nk));
-        HFK_TRY(dev_alloc(&pos_kseg, (size_t)nk));
-        e.owned.push_back(pos_kseg);
+        HFK_TRY(plan_alloc(c, &pos_kseg, (size_t)nk));
         if (!coop) {
-            HFK_TRY(dev_alloc(&kdesc, (size_t)W * nk));
-            e.owned.push_back(kdesc);
-            HFK_TRY(dev_alloc(&kposq, (size_t)2 * nk));
-            e.owned.push_back(kposq);
+            HFK_TRY(plan_alloc(c, &kdesc, (size_t)W * nk));
+            HFK_TRY(plan_alloc(c, &kposq, (size_t)2 * nk));
         } else {
             HFK_TRY(t.alloc(&kdesc, (size_t)W * nk));
             HFK_TRY(t.alloc(&kposq, (size_t)2 * nk));
@@ -966,8 +966,7 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
             e.d_kdesc = kdesc;
             e.d_kpos_q = kposq;
             if (h_nonunit) {
-                HFK_TRY(dev_alloc(&e.d_kscale, (size_t)(NC + ND) * nk));
-                e.owned.push_back(e.d_kscale);
+                HFK_TRY(plan_alloc(c, &e.d_kscale, (size_t)(NC + ND) * nk));
                 HFK_HIP(hipMemcpyAsync(e.d_kscale, kscale, sizeof(double) * (NC + ND) * nk, hipMemcpyDeviceToDevice,
                                        c->stream));
             }
@@ -996,12 +995,9 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
         int* d_kf;
         int4* d_ch;
         int2* d_tl;
-        HFK_TRY(dev_alloc(&d_kf, kfirst.size()));
-        e.owned.push_back(d_kf);
-        HFK_TRY(dev_alloc(&d_ch, chunks.size()));
-        e.owned.push_back(d_ch);
-        HFK_TRY(dev_alloc(&d_tl, tiles.size()));
-        e.owned.push_back(d_tl);
+        HFK_TRY(plan_alloc(c, &d_kf, kfirst.size()));
+        HFK_TRY(plan_alloc(c, &d_ch, chunks.size()));
+        HFK_TRY(plan_alloc(c, &d_tl, tiles.size()));
         HFK_HIP(hipMemcpyAsync(d_kf, kfirst.data(), sizeof(int) * kfirst.size(), hipMemcpyHostToDevice, c->stream));
         if (!chunks.empty())
             HFK_HIP(hipMemcpyAsync(d_ch, chunks.data(), sizeof(int4) * chunks.size(), hipMemcpyHostToDevice, c->stream));
