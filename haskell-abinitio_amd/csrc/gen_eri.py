@@ -909,7 +909,7 @@ def main():
             body += ['#include "hfk_coop_impl.hpp"',
                      "int hfk_dispatch_%s(int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag,
                      "{ return dispatch_coop<%d, %d, %d, %d>(op, prec, boys, mode, a, s); }" % q,
-                     "int hfk_dispatch_grad_%s(int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag,
+                     "int hfk_dispatch_grad_%s(int boys, int mode, int, const QuartetArgs& a, hipStream_t s)" % tag,
                      "{ return dispatch_coop<%d, %d, %d, %d>(HFK_OP_GRAD, 0, boys, mode, a, s); }" % q]
             path = os.path.join(outdir, "eri_%s.hip" % tag)
             txt = "\n".join(body) + "\n"
@@ -949,12 +949,12 @@ def main():
         body.append('#include "hfk_eri_impl.hpp"')
         if grad_coop:
             body.append('#include "hfk_coop_impl.hpp"')
-            body.append("int hfk_dispatch_grad_%s(int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag)
+            body.append("int hfk_dispatch_grad_%s(int boys, int mode, int, const QuartetArgs& a, hipStream_t s)" % tag)
             body.append("{ return dispatch_coop<%d, %d, %d, %d>(HFK_OP_GRAD, 0, boys, mode, a, s); }" % q)
         else:
             body.append('#include "hfk_grad_impl.hpp"')
-            body.append("int hfk_dispatch_grad_%s(int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag)
-            body.append("{ return dispatch_grad<%d, %d, %d, %d>(boys, mode, a, s); }" % q)
+            body.append("int hfk_dispatch_grad_%s(int boys, int mode, int prec, const QuartetArgs& a, hipStream_t s)" % tag)
+            body.append("{ return dispatch_grad<%d, %d, %d, %d>(boys, mode, prec, a, s); }" % q)
         body.append("int hfk_dispatch_%s(int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)" % tag)
         body.append("{ return dispatch<%d, %d, %d, %d>(op, prec, boys, mode, a, s); }" % q)
         path = os.path.join(outdir, "eri_%s.hip" % tag)
@@ -966,11 +966,11 @@ def main():
     for q in qs:
         d.append("int hfk_dispatch_%d%d%d%d(int, int, int, int, const QuartetArgs&, hipStream_t);" % q)
     for q in qs:
-        d.append("int hfk_dispatch_grad_%d%d%d%d(int, int, const QuartetArgs&, hipStream_t);" % q)
-    d.append("int hfk_launch_grad(int la, int lb, int lc, int ld, int boys, int mode, const QuartetArgs& a, hipStream_t s)")
+        d.append("int hfk_dispatch_grad_%d%d%d%d(int, int, int, const QuartetArgs&, hipStream_t);" % q)
+    d.append("int hfk_launch_grad(int la, int lb, int lc, int ld, int boys, int mode, int prec, const QuartetArgs& a, hipStream_t s)")
     d.append("{")
     for q in qs:
-        d.append("    if (la == %d && lb == %d && lc == %d && ld == %d) return hfk_dispatch_grad_%d%d%d%d(boys, mode, a, s);" % (q + q))
+        d.append("    if (la == %d && lb == %d && lc == %d && ld == %d) return hfk_dispatch_grad_%d%d%d%d(boys, mode, prec, a, s);" % (q + q))
     d.append("    return -2;")
     d.append("}")
     d.append("int hfk_launch_quartets(int la, int lb, int lc, int ld, int op, int prec, int boys, int mode, const QuartetArgs& a, hipStream_t s)")

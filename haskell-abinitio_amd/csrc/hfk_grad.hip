@@ -119,7 +119,7 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
     const size_t np = c->plan.size();
     for (size_t i = 0; i < np; ++i) {
         const PlanEntry& e = c->plan[i];
-        hfk_plan_args(c, e, a);   // the whole task list (both precision parts) in FP64
+        hfk_plan_args(c, e, a);
         // the bra shells' density rows in LDS while four waves per SIMD still fit
         a.drows = !hfk_class_grad_coop(e.la, e.lb, e.lc, e.ld) &&
                   hfk_grad_lds_bytes(e.la, e.lb, e.ncols, natom, 1, e.bra_stage) <= hfk_lds_budget(a.block);
@@ -130,7 +130,22 @@ extern "C" int hfk_gradient(hfk_ctx* c, const double* D_flat, const double* W_fl
         }
         hipStream_t s = c->stream;
         HFK_HIP(hipEventRecord(c->events[4 * np + 2 * i], s));
-        if (hfk_launch_grad(e.la, e.lb, e.lc, e.ld, c->opt.boys_mode, e.mode, a, s)) {
+        // a mixed-precision task list: its FP64 part in FP64, its FP32 part in
+        // FP32 (per-lane kernels); the cooperative kernel takes the whole list
+        // in FP64
+        const bool parts = e.split && !hfk_class_grad_coop(e.la, e.lb, e.lc, e.ld);
+        bool ok = true;
+        for (int part = 0; part < (parts ? 2 : 1) && ok; ++part) {
+            QuartetArgs b = a;
+            if (parts) {
+                const long long t0 = part == 0 ? 0 : e.ntask64;
+                b.tasks = e.d_tasks + t0;
+                b.task0 = t0;
+                b.ntask = part == 0 ? e.ntask64 : e.ntask - e.ntask64;
+            }
+            ok = hfk_launch_grad(e.la, e.lb, e.lc, e.ld, c->opt.boys_mode, e.mode, part, b, s) == 0;
+        }
+        if (!ok) {
             rc = hfk_set_error(HFK_EUNSUP, "no gradient kernel for class (%d%d|%d%d)", e.la, e.lb, e.lc, e.ld);
             break;
         }

@@ -26,7 +26,10 @@ constexpr int grad_min_waves()
 // GRP: which of the kGradGroups blocks of bra components this launch
 // contracts (a class whose density block does not fit one lane's registers
 // splits its bra components over several launches; the partial sums add).
-template <int LA, int LB, int LC, int LD, int BOYS, int MODE, int GRP>
+// T: the arithmetic of the derivative integrals (double; float for the FP32
+// part of a mixed-precision task list); the density block is formed in FP64
+// and the per-lane sums are kept in FP64.
+template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, int GRP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_min_waves<LA, LB, LC, LD>())))
 k_grad_tiles(QuartetArgs a)
 {
@@ -173,9 +176,14 @@ k_grad_tiles(QuartetArgs a)
         }
         const int slen = rlen[r];
         const KetSoA<NF> kr{a.ksoa + rsoa[r] + pos + (size_t)kprim0 * NF * slen, slen};
+        T Gt[NSUB * NKET], gt[9];
+#pragma unroll
+        for (int k = 0; k < NSUB * NKET; ++k) Gt[k] = (T)G[k];
+        grad_class<LA, LB, LC, LD, T, BOYS, MODE, GRP>(sbra, sbq, nbp, kr, min(kprim1, rnpp[r]) - kprim0, a.bt, Gt, a.qtau,
+                                                       gt);
         double g[9];
-        grad_class<LA, LB, LC, LD, double, BOYS, MODE, GRP>(sbra, sbq, nbp, kr, min(kprim1, rnpp[r]) - kprim0, a.bt, G,
-                                                            a.qtau, g);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) g[k] = (double)gt[k];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             gA[k] += g[k];
@@ -210,7 +218,7 @@ k_grad_tiles(QuartetArgs a)
     }
 }
 
-template <int LA, int LB, int LC, int LD, int BOYS, int MODE, int GRP>
+template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, int GRP>
 void launch_grad_group(const QuartetArgs& a, hipStream_t s)
 {
     const size_t shm = hfk_grad_lds_bytes(LA, LB, a.ncols, a.grad_lds ? a.natom : 0, a.drows, a.bra_stage);
@@ -219,29 +227,34 @@ void launch_grad_group(const QuartetArgs& a, hipStream_t s)
         const long long cnt = std::min<long long>(a.ntask - done, 1LL << 30);
         QuartetArgs b = a;
         b.tasks = a.tasks + done;
-        hipLaunchKernelGGL((k_grad_tiles<LA, LB, LC, LD, BOYS, MODE, GRP>), dim3((unsigned)cnt), dim3(a.block), shm, s, b);
+        hipLaunchKernelGGL((k_grad_tiles<LA, LB, LC, LD, T, BOYS, MODE, GRP>), dim3((unsigned)cnt), dim3(a.block), shm, s, b);
         done += cnt;
     }
-    if constexpr (GRP + 1 < kGradGroups) launch_grad_group<LA, LB, LC, LD, BOYS, MODE, GRP + 1>(a, s);
+    if constexpr (GRP + 1 < kGradGroups) launch_grad_group<LA, LB, LC, LD, T, BOYS, MODE, GRP + 1>(a, s);
 }
 
-template <int LA, int LB, int LC, int LD, int BOYS, int MODE>
+template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
 void launch_grad_tiles(const QuartetArgs& a, hipStream_t s)
 {
     if (a.ntask <= 0) return;
-    launch_grad_group<LA, LB, LC, LD, BOYS, MODE, 0>(a, s);
+    launch_grad_group<LA, LB, LC, LD, T, BOYS, MODE, 0>(a, s);
 }
 
+// prec: 0 FP64, 1 FP32 (the FP32 part of a mixed-precision task list)
 template <int LA, int LB, int LC, int LD>
-int dispatch_grad(int boys, int mode, const QuartetArgs& a, hipStream_t s)
+int dispatch_grad(int boys, int mode, int prec, const QuartetArgs& a, hipStream_t s)
 {
-    if (boys == 0) {
-        if (mode == 0) launch_grad_tiles<LA, LB, LC, LD, 0, 0>(a, s);
-        else launch_grad_tiles<LA, LB, LC, LD, 0, 1>(a, s);
-    } else {
-        if (mode == 0) launch_grad_tiles<LA, LB, LC, LD, 1, 0>(a, s);
-        else launch_grad_tiles<LA, LB, LC, LD, 1, 1>(a, s);
+#define HFK_G(T)                                                                     \
+    if (boys == 0) {                                                                 \
+        if (mode == 0) launch_grad_tiles<LA, LB, LC, LD, T, 0, 0>(a, s);             \
+        else launch_grad_tiles<LA, LB, LC, LD, T, 0, 1>(a, s);                       \
+    } else {                                                                         \
+        if (mode == 0) launch_grad_tiles<LA, LB, LC, LD, T, 1, 0>(a, s);             \
+        else launch_grad_tiles<LA, LB, LC, LD, T, 1, 1>(a, s);                       \
     }
+    if (prec == 0) { HFK_G(double) }
+    else { HFK_G(float) }
+#undef HFK_G
     return 0;
 }
 

@@ -158,9 +158,10 @@ inline size_t hfk_coop_jk_lds_bytes(int la, int lb, int ncols)
 int hfk_launch_quartets(int la, int lb, int lc, int ld, int op, int prec, int boys, int mode,
                         const QuartetArgs& a, hipStream_t s);
 
-// Two-electron gradient contribution of one class over a plan entry's task list
-// (FP64), accumulated into a.grad.
-int hfk_launch_grad(int la, int lb, int lc, int ld, int boys, int mode, const QuartetArgs& a, hipStream_t s);
+// Two-electron gradient contribution of one class over the tasks in a,
+// accumulated into a.grad.  prec: 0 FP64, 1 FP32 (the per-lane kernels; the
+// cooperative d-class kernel is FP64 and ignores it).
+int hfk_launch_grad(int la, int lb, int lc, int ld, int boys, int mode, int prec, const QuartetArgs& a, hipStream_t s);
 
 // Setup kernels (hfk_setup.hip)
 struct ShellDev {
