@@ -799,7 +799,8 @@ int hfk_plan_entry_device(hfk_ctx* c, PlanEntry& e)
     }
     HFK_HIP(hipStreamSynchronize(c->stream));
     const long long ntot = T[0] + T[1], rtot = R[0] + R[1];
-    if (ntot > INT32_MAX || rtot > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "task list too long");
+    // task and run indices are int; the K' reduction lists sort 2 entries per task
+    if (2 * ntot > INT32_MAX || rtot > INT32_MAX) return hfk_set_error(HFK_EUNSUP, "task list too long");
     int4 *tasks_raw, *tasks;
     long long* cost;
     int* d_scal;   // [0] bra stage, [1] longest task
