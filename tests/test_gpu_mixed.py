@@ -60,3 +60,23 @@ def test_mixed_parts_partition_the_quartets(require_gpu, water8_fp64):
         got = [c for c in parts if key(c) == k]
         for field in ("shell_quartets", "prim_quartets", "func_quartets"):
             assert sum(c[field] for c in got) == r[field], (k, field)
+
+
+@pytest.mark.parametrize("boys", [hfk.HFK_BOYS_REFERENCE, hfk.HFK_BOYS_ACCURATE])
+def test_mixed_gradient_close_to_fp64(require_gpu, water8_fp64, boys):
+    """The FP32 part of each task list runs the derivative integrals in FP32
+    (both Boys modes): at a fixed density the gradient stays within 1e-7
+    Eh/bohr of the FP64 one, and the FP32 parts are really used (the mixed
+    gradient differs from the FP64 one)."""
+    print("case: (H2O)8 mixed gradient boys=%d" % boys, flush=True)
+    mol, r64 = water8_fp64
+    D = r64.getDensity
+    with hfk.Context(mol, hfk.options(boys_mode=boys)) as c64:
+        g64 = c64.gradient(D)
+    with hfk.Context(mol, hfk.options(boys_mode=boys, fp32_threshold=1e-4)) as c32:
+        g32 = c32.gradient(D)
+        st = c32.class_stats()
+    assert any(c["prec"] == 1 and c["shell_quartets"] > 0 for c in st)
+    d = np.max(np.abs(g32 - g64))
+    assert 0.0 < d < 1e-7, d
+    assert np.max(np.abs(g32.sum(axis=0))) < 1e-9
