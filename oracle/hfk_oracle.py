@@ -33,6 +33,9 @@ flat layout                 literal restatement of fourIndex2Flat
 calcGmatrix                 brute force G from a dense 4-index tensor
 S, T, V                     textbook H2 / HeH+ values (Szabo & Ostlund)
 SCF energy                  Szabo & Ostlund H2 (-1.1167) and HeH+ (-2.860662)
+memory and UB safety        every C entry point built with AddressSanitizer and
+                            UBSan and run once (oracle/sanitize_check.c, with
+                            its own symmetry, layout and G checks)
 ==========================  ====================================================
 """
 import ctypes as C

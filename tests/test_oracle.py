@@ -558,3 +558,26 @@ def test_water_sto3g_published_energy():
     assert abs(e - (-74.942079928192)) < 1e-10
     # the reference's convention (coefficients as stored, S_ii := 1) moves E by the self-overlap defect only
     assert abs(e_stored - e) < 1e-6
+
+
+def test_oracle_under_address_and_undefined_behaviour_sanitizers():
+    """Every C entry point of the oracle, built with AddressSanitizer and
+    UBSan (oracle/Makefile target `sanitize`, single-threaded), runs without
+    a report; the same run checks F_m(0) = 1/(2m+1) to the reference nodes'
+    error, table Boys against the accurate function, Gauss-Legendre moments,
+    the 8-fold ERI symmetry over s/p/d, the flat layout maps and G against
+    its dense contraction (oracle/sanitize_check.c)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None or shutil.which("make") is None:
+        pytest.skip("needs gcc and make")
+    odir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    b = subprocess.run(["make", "-C", odir, "sanitize"], capture_output=True, text=True, timeout=300)
+    assert b.returncode == 0, b.stdout + b.stderr
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:halt_on_error=1",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    r = subprocess.run([os.path.join(odir, "_build", "sanitize_check")], capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr
+    assert "oracle sanitizer run ok" in r.stdout
