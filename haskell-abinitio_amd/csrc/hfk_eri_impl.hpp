@@ -146,18 +146,26 @@ void launch_quartets_one(const QuartetArgs& a, hipStream_t s)
 // wave the hardware orders the lanes' adds, so the LDS sums do not depend on
 // scheduling either.
 // ---------------------------------------------------------------------
-// Waves per SIMD the register allocation must leave room for: the L = 3
-// classes need a few registers over 256 (VGPRs + AGPRs), (ps|ps) one over
-// 128, which alone would leave one and three waves per SIMD (a few bytes of
-// spill per ket instead).
-template <int LA, int LB, int LC, int LD>
+// Waves per SIMD the register allocation must leave room for.  FP64: the
+// L = 3 classes need a few registers over 256 (VGPRs + AGPRs), (ps|ps) one
+// over 128, which alone would leave one and three waves per SIMD (a few bytes
+// of spill per ket instead).  FP32 s/p classes (the FP32 part of a
+// mixed-precision task list, launched without the density rows in LDS so
+// that LDS does not cap the workgroups per CU): more resident waves pay for
+// some spill, (ps|ss) 6, (ps|ps) 5, (pp|ps) 3, (pp|pp) 2; (pp|ss) spills too
+// much at 5 and keeps the 4 its registers allow.
+template <int LA, int LB, int LC, int LD, typename T>
 constexpr int jk_min_waves()
 {
-    return (LA + LB + LC + LD == 3) ? 2 : ((LA == 1 && LB == 0 && LC == 1 && LD == 0) ? 4 : 1);
+    constexpr int L = LA + LB + LC + LD;
+    constexpr bool pss = LA == 1 && LB == 0 && LC == 1 && LD == 0;
+    if constexpr (sizeof(T) == 4 && LA <= 1)
+        return L == 1 ? 6 : pss ? 5 : L == 3 ? 3 : L == 4 ? 2 : 1;
+    return L == 3 ? 2 : pss ? 4 : 1;
 }
 
 template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_waves<LA, LB, LC, LD>())))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(jk_min_waves<LA, LB, LC, LD, T>())))
 k_jk_tiles(QuartetArgs a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);

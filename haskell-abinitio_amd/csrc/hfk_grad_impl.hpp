@@ -15,12 +15,18 @@
 namespace {
 
 // Waves per SIMD the register allocation must leave room for.  The total of
-// VGPRs and AGPRs sets occupancy (512 per lane per SIMD); (ps|ps) and (pp|ss)
-// need a little over 256, which alone would leave one wave per SIMD.
-template <int LA, int LB, int LC, int LD>
+// VGPRs and AGPRs sets occupancy (512 per lane per SIMD); in FP64 (ps|ps) and
+// (pp|ss) need a little over 256, which alone would leave one wave per SIMD.
+// In FP32 (the FP32 part of a mixed-precision task list) (pp|ss) at three
+// waves and (pp|ps) at two pay for a little spill; the other FP32 classes
+// are faster at the occupancy their registers give.
+template <int LA, int LB, int LC, int LD, typename T>
 constexpr int grad_min_waves()
 {
-    return (LA + LB + LC + LD <= 1) ? 3 : ((LA + LB + LC + LD == 2) ? 2 : 1);
+    constexpr int L = LA + LB + LC + LD;
+    if constexpr (sizeof(T) == 4 && LA == 1 && LB == 1 && LC == 0) return 3;   // (pp|ss)
+    if constexpr (sizeof(T) == 4 && LA == 1 && L == 3) return 2;               // (pp|ps)
+    return L <= 1 ? 3 : (L == 2 ? 2 : 1);
 }
 
 // GRP: which of the kGradGroups blocks of bra components this launch
@@ -30,7 +36,7 @@ constexpr int grad_min_waves()
 // part of a mixed-precision task list); the density block is formed in FP64
 // and the per-lane sums are kept in FP64.
 template <int LA, int LB, int LC, int LD, typename T, int BOYS, int MODE, int GRP>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_min_waves<LA, LB, LC, LD>())))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grad_min_waves<LA, LB, LC, LD, T>())))
 k_grad_tiles(QuartetArgs a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);

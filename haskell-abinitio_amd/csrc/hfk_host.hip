@@ -757,6 +757,9 @@ int hfk_ctx_run_quartets(hfk_ctx* c, QuartetArgs a, int op)
                 b.tasks = e.d_tasks + t0;
                 b.task0 = t0;
                 b.ntask = cnt;
+                // the FP32 part of an s/p class runs at more waves per SIMD
+                // than the density rows in LDS would leave room for
+                if (part == 1 && e.la <= 1) b.drows = 0;
             }
             HFK_HIP(hipEventRecord(c->events[4 * i + 2 * part], s));
             rc = hfk_launch_quartets(e.la, e.lb, e.lc, e.ld, op, part, c->opt.boys_mode,
