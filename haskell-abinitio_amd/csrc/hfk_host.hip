@@ -232,13 +232,51 @@ extern "C" int hfk_set_options(hfk_ctx* c, const hfk_options* opt)
     return rc;
 }
 
+// Idle streams of destroyed contexts, per device.  Creating a stream costs
+// about 0.7 ms on this runtime and a context holds four; a caller that makes
+// many short-lived contexts (the fragment guess makes one per monomer) takes
+// them from here instead.  Streams are returned synchronised; at most
+// kStreamPoolMax are kept, the rest destroyed.
+namespace {
+constexpr size_t kStreamPoolMax = 64;
+std::mutex g_stream_mu;
+std::vector<std::pair<int, hipStream_t>> g_stream_pool;
+}  // namespace
+
+static hipError_t stream_acquire(int dev, hipStream_t* s)
+{
+    {
+        std::lock_guard<std::mutex> g(g_stream_mu);
+        for (size_t i = g_stream_pool.size(); i-- > 0;)
+            if (g_stream_pool[i].first == dev) {
+                *s = g_stream_pool[i].second;
+                g_stream_pool.erase(g_stream_pool.begin() + (long)i);
+                return hipSuccess;
+            }
+    }
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
+static void stream_release(int dev, hipStream_t s)
+{
+    if (!s) return;
+    if (hipStreamSynchronize(s) == hipSuccess) {
+        std::lock_guard<std::mutex> g(g_stream_mu);
+        if (g_stream_pool.size() < kStreamPoolMax) {
+            g_stream_pool.emplace_back(dev, s);
+            return;
+        }
+    }
+    hipStreamDestroy(s);
+}
+
 int hfk_ctx_init_device(hfk_ctx* c)
 {
     HFK_HIP(hipGetDevice(&c->device));
     HFK_TRY(hfk_noalloc_setup(c->device));
-    HFK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HFK_HIP(stream_acquire(c->device, &c->stream));
     for (int k = 0; k < hfk_ctx::NAUX; ++k) {
-        HFK_HIP(hipStreamCreateWithFlags(&c->aux[k], hipStreamNonBlocking));
+        HFK_HIP(stream_acquire(c->device, &c->aux[k]));
         HFK_HIP(hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming));
     }
     HFK_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -350,7 +388,7 @@ void hfk_ctx_free_device(hfk_ctx* c)
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     for (int k = 0; k < hfk_ctx::NAUX; ++k) {
-        if (c->aux[k]) hipStreamDestroy(c->aux[k]);
+        stream_release(c->device, c->aux[k]);
         if (c->ev_join[k]) hipEventDestroy(c->ev_join[k]);
         c->aux[k] = nullptr;
         c->ev_join[k] = nullptr;
@@ -358,7 +396,7 @@ void hfk_ctx_free_device(hfk_ctx* c)
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     c->ev_fork = nullptr;
     if (c->blas) hfk_linalg_destroy(c);
-    if (c->stream) hipStreamDestroy(c->stream);
+    stream_release(c->device, c->stream);
     c->stream = nullptr;
 }
 

@@ -11,26 +11,56 @@
 
 #include <chrono>
 #include <deque>
+#include <mutex>
 #include <vector>
 
 #include "hfk.h"
 #include "hfk_host.hpp"
 
+// rocBLAS handles of destroyed contexts, per device (a handle and its device
+// workspace are costly to create; short-lived contexts reuse them).  A
+// handle is owned by one context at a time and bound to its stream.
+namespace {
+constexpr size_t kBlasPoolMax = 16;
+std::mutex g_blas_mu;
+std::vector<std::pair<int, rocblas_handle>> g_blas_pool;
+}  // namespace
+
 int hfk_linalg_init(hfk_ctx* c)
 {
     if (c->blas) return HFK_OK;
     rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success)
+    {
+        std::lock_guard<std::mutex> g(g_blas_mu);
+        for (size_t i = g_blas_pool.size(); i-- > 0;)
+            if (g_blas_pool[i].first == c->device) {
+                h = g_blas_pool[i].second;
+                g_blas_pool.erase(g_blas_pool.begin() + (long)i);
+                break;
+            }
+    }
+    if (!h && rocblas_create_handle(&h) != rocblas_status_success)
         return hfk_set_error(HFK_EHIP, "rocblas_create_handle failed");
     rocblas_set_stream(h, c->stream);
     c->blas = (void*)h;
     return HFK_OK;
 }
 
+// Called with the context's stream synchronised.
 void hfk_linalg_destroy(hfk_ctx* c)
 {
-    if (c->blas) rocblas_destroy_handle((rocblas_handle)c->blas);
+    rocblas_handle h = (rocblas_handle)c->blas;
     c->blas = nullptr;
+    if (!h) return;
+    rocblas_set_stream(h, nullptr);
+    {
+        std::lock_guard<std::mutex> g(g_blas_mu);
+        if (g_blas_pool.size() < kBlasPoolMax) {
+            g_blas_pool.emplace_back(c->device, h);
+            return;
+        }
+    }
+    rocblas_destroy_handle(h);
 }
 
 // Symmetric eigen-solve (EigenValues.hs:29-31 + VectorTools.hs:27-31):
