@@ -280,8 +280,12 @@ def fock_rate(ctx, D, steps, warmup, workload):
         cs = ctx.class_stats()
         if per_launch is None:
             per_launch = [dict(c, ms_sum=0.0) for c in cs]
+        # a launch's time is its own only if it ran alone in every timed build
+        # (a context's first two builds run the classes one after another,
+        # later ones concurrently with only the longest class alone)
         for acc, c in zip(per_launch, cs):
             acc["ms_sum"] += c["ms"]
+            acc["solo"] = bool(acc["solo"]) and bool(c["solo"])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = ctx.stats()
